@@ -1,0 +1,609 @@
+"""TEST INFRASTRUCTURE ONLY: CPU restatement of GPAW's FD / multigrid hot path.
+
+The numerical kernels live in ``bmgs_oracle.c`` (built to ``liboracle.so``);
+``Kernels('reference')`` instead loads ``_ref/libgpawref.so``, the reference's
+own ``c/bmgs/bmgs.c`` + ``c/bc.c`` compiled in place, with identical entry
+points.  This module restates the Python layers that drive them:
+
+* grid geometry           gpaw/grid_descriptor.py:65-150,197-237,494-501,
+                          gpaw/domain.py:15-150
+* stencil coefficients    gpaw/fd_operators.py:20-66 (FDOperator),
+                          :123-144 (Gradient), :192-229 (LaplaceA/B),
+                          :231-292 (NewGUCLaplace and its table)
+* transformer paddings    gpaw/transformers.py:19-62
+* multigrid Poisson       gpaw/poisson.py:21-276
+* charged-system helpers  gpaw/utilities/gauss.py:47-83,
+                          gpaw/utilities/tools.py:92-124,
+                          gpaw/utilities/ewald.py:12-150
+
+It is imported only by tests, the smoke check and the CPU-baseline leg of
+the benchmark.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+# CODATA-era Bohr radius used by the reference's ASE dependency (ase.units)
+bohr_per_angstrom = 1.0 / 0.52917720859
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lp = ctypes.POINTER(ctypes.c_long)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _l(a):
+    return a.ctypes.data_as(_lp)
+
+
+class Kernels:
+    """ctypes front end to the oracle kernels.
+
+    kind='port'       our C restatement (liboracle.so)
+    kind='reference'  the reference's own C, built in place (_ref/libgpawref.so)
+    """
+
+    def __init__(self, kind='port'):
+        if kind == 'port':
+            path, prefix = os.path.join(_HERE, 'liboracle.so'), 'orc_'
+        elif kind == 'reference':
+            path, prefix = os.path.join(_HERE, '_ref', 'libgpawref.so'), 'ref_'
+        else:
+            raise ValueError(kind)
+        if not os.path.exists(path):
+            raise FileNotFoundError(
+                '%s not built (run make -C oracle)' % path)
+        self.kind = kind
+        lib = ctypes.CDLL(path)
+        self._apply = getattr(lib, prefix + 'operator_apply')
+        self._apply.argtypes = [_dp, _lp, ctypes.c_int, _lp, ctypes.c_int,
+                                _lp, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                ctypes.c_int, _dp]
+        self._relax = getattr(lib, prefix + 'operator_relax')
+        self._relax.argtypes = [_dp, _lp, ctypes.c_int, _lp, ctypes.c_int,
+                                _lp, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                ctypes.c_int, ctypes.c_double]
+        self._trans = getattr(lib, prefix + 'transformer_apply')
+        self._trans.argtypes = [_lp, _lp, ctypes.c_int, _lp, _lp, _lp, _lp,
+                                ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                ctypes.c_int, _dp]
+        self._lib = lib
+
+    @staticmethod
+    def available(kind):
+        name = 'liboracle.so' if kind == 'port' else '_ref/libgpawref.so'
+        return os.path.exists(os.path.join(_HERE, name))
+
+    def operator_apply(self, coefs, offsets, size, rng, nb, real, cfd,
+                       a, b, nin, phases):
+        ph = None
+        if not real:
+            ph = np.ascontiguousarray(phases, complex).view(float)
+        rc = self._apply(_d(coefs), _l(offsets), len(coefs), _l(size), rng,
+                         _l(nb), int(real), int(cfd), _d(a.view(float)),
+                         _d(b.view(float)), nin, _d(ph))
+        if rc:
+            raise RuntimeError('oracle operator_apply failed (%d)' % rc)
+
+    def operator_relax(self, coefs, offsets, size, rng, nb, cfd, method,
+                       f, s, nrelax, w):
+        rc = self._relax(_d(coefs), _l(offsets), len(coefs), _l(size), rng,
+                         _l(nb), int(cfd), int(method), _d(f), _d(s),
+                         int(nrelax), float(w))
+        if rc:
+            raise RuntimeError('oracle operator_relax failed (%d)' % rc)
+
+    def transformer_apply(self, size_in, size_out, k, pad, npad, skip, nb,
+                          real, interpolate, a, b, nin, phases):
+        ph = None
+        if not real:
+            ph = np.ascontiguousarray(phases, complex).view(float)
+        rc = self._trans(_l(size_in), _l(size_out), int(k), _l(pad), _l(npad),
+                         _l(skip), _l(nb), int(real), int(interpolate),
+                         _d(a.view(float)), _d(b.view(float)), nin, _d(ph))
+        if rc:
+            raise RuntimeError('oracle transformer_apply failed (%d)' % rc)
+
+
+_default = {}
+
+
+def _kernels(k):
+    if isinstance(k, Kernels):
+        return k
+    k = k or 'port'
+    if k not in _default:
+        _default[k] = Kernels(k)
+    return _default[k]
+
+
+# ---------------------------------------------------------------------------
+# Geometry: one domain of gpaw/grid_descriptor.py:65-150 + gpaw/domain.py
+# ---------------------------------------------------------------------------
+class Grid:
+    def __init__(self, N_c, cell_cv=(1, 1, 1), pbc_c=True):
+        if isinstance(pbc_c, (bool, int, np.bool_)):
+            pbc_c = (bool(pbc_c),) * 3
+        cell = np.array(cell_cv, float)
+        if cell.ndim == 1:
+            cell = np.diag(cell)
+        self.cell_cv = cell
+        self.pbc_c = np.asarray(pbc_c, bool)
+        self.N_c = np.array(N_c, int)
+        self.icell_cv = np.linalg.inv(cell).T
+        cell_c = (cell ** 2).sum(1) ** 0.5
+        ucell = np.array([cell[c] / cell_c[c] for c in range(3)])
+        self.xxxiucell_cv = np.linalg.inv(ucell.T)
+        # non-periodic axes drop global point 0 (domain.py / grid_descriptor.py:116-130)
+        self.beg_c = np.where(self.pbc_c, 0, 1)
+        self.end_c = self.N_c.copy()
+        if (self.end_c - self.beg_c <= 0).any():
+            raise ValueError('Grid too small!')
+        self.n_c = self.end_c - self.beg_c
+        self.h_cv = cell / self.N_c[:, None]
+        self.dv = abs(np.linalg.det(cell)) / self.N_c.prod()
+        self.orthogonal = not (cell - np.diag(cell.diagonal())).any()
+        L_c = (np.linalg.inv(cell) ** 2).sum(0) ** -0.5
+        h_c = L_c / self.N_c
+        if max(h_c) / min(h_c) > 1.3:
+            raise ValueError('Very anisotropic grid spacings: %s' % h_c)
+        # single domain: the neighbour is ourselves (0) or nobody (-1)
+        self.neighbor_cd = np.where(self.pbc_c[:, None], 0, -1) * np.ones(
+            (3, 2), int)
+        self.sdisp_cd = np.where(self.pbc_c[:, None], [[1, -1]], 0) * np.ones(
+            (3, 2), int)
+
+    def coarsen(self):
+        if (self.N_c % 2).any():
+            raise ValueError('Grid %s not divisible by 2!' % self.N_c)
+        return Grid(self.N_c // 2, self.cell_cv, self.pbc_c)
+
+    def refine(self):
+        return Grid(self.N_c * 2, self.cell_cv, self.pbc_c)
+
+    def zeros(self, n=(), dtype=float):
+        if isinstance(n, int):
+            n = (n,)
+        return np.zeros(tuple(n) + tuple(self.n_c), dtype)
+
+    def integrate(self, a):
+        if a.ndim == 3:
+            return a.sum() * self.dv
+        return a.reshape(a.shape[:-3] + (-1,)).sum(-1) * self.dv
+
+    def size_of_global_array(self):
+        return self.N_c - 1 + self.pbc_c
+
+    def grid_point_coordinates(self):
+        idx = np.indices(self.n_c, float).T + self.beg_c
+        return np.dot(idx, self.h_cv).T.copy()
+
+    def phase_cd(self, k_c):
+        """phase_cd of gpaw/kpt_descriptor.py:162-166 for scaled k-point k_c."""
+        return np.exp(2j * np.pi * self.sdisp_cd *
+                      np.asarray(k_c, float)[:, None])
+
+
+# ---------------------------------------------------------------------------
+# Stencil coefficients
+# ---------------------------------------------------------------------------
+# J. R. Chelikowsky et al., Phys. Rev. B 50, 11355 (1994): central-difference
+# second-derivative weights of order 2n (table of gpaw/fd_operators.py:286-292).
+LAPLACE_TABLE = [
+    [0.0],
+    [-2.0, 1.0],
+    [-5 / 2, 4 / 3, -1 / 12],
+    [-49 / 18, 3 / 2, -3 / 20, 1 / 90],
+    [-205 / 72, 8 / 5, -1 / 5, 8 / 315, -1 / 560],
+    [-5269 / 1800, 5 / 3, -5 / 21, 5 / 126, -5 / 1008, 1 / 3150],
+    [-5369 / 1800, 12 / 7, -15 / 56, 10 / 189, -1 / 112, 2 / 1925,
+     -1 / 16632]]
+
+
+def laplace_stencil(grid, scale=1.0, n=1):
+    """NewGUCLaplace coefficients and offsets (gpaw/fd_operators.py:231-277).
+
+    The 13 half-neighbour directions are ordered by |M.u|^2 with a stable
+    sort (the reference's numpy sorted 13 elements by insertion sort); up
+    to six directions are added until the second-moment equations solve.
+    """
+    M_ic = np.indices((3, 3, 3)).reshape((3, -1)).T[-13:] - 1
+    u_cv = grid.h_cv / (grid.h_cv ** 2).sum(1)[:, None] ** 0.5
+    u2_i = (np.dot(M_ic, u_cv) ** 2).sum(1)
+    i_d = u2_i.argsort(kind='stable')
+    m_mv = np.array([(2, 0, 0), (0, 2, 0), (0, 0, 2),
+                     (0, 1, 1), (1, 0, 1), (1, 1, 0)])
+    for D in range(3, 7):
+        h_dv = np.dot(M_ic[i_d[:D]], grid.h_cv)
+        A_md = (h_dv ** m_mv[:, None, :]).prod(2)
+        a_d, residual, rank, _ = np.linalg.lstsq(A_md, [1, 1, 1, 0, 0, 0],
+                                                 rcond=-1)
+        if residual.sum() < 1e-14:
+            assert rank == D
+            break
+    a_d = a_d * scale
+    offsets = [(0, 0, 0)]
+    coefs = [LAPLACE_TABLE[n][0] * a_d.sum()]
+    for d in range(D):
+        M_c = M_ic[i_d[d]]
+        for sgn in (1, -1):
+            for k in range(1, n + 1):
+                offsets.append(tuple(int(x) for x in sgn * k * M_c))
+                coefs.append(a_d[d] * LAPLACE_TABLE[n][k])
+    return np.array(coefs), np.array(offsets, int)
+
+
+def gradient_stencil(grid, v, scale=1.0, n=1):
+    """Gradient along Cartesian v (gpaw/fd_operators.py:123-144)."""
+    h = (grid.h_cv ** 2).sum(1) ** 0.5
+    d = grid.xxxiucell_cv[:, v]
+    A = np.zeros((2 * n + 1, 2 * n + 1))
+    for i, io in enumerate(range(-n, n + 1)):
+        for j in range(2 * n + 1):
+            A[i, j] = io ** j / float(math.factorial(j))
+    A[n, 0] = 1.0
+    c = np.linalg.inv(A)[1]
+    c = np.delete(c, len(c) // 2)
+    offs = np.delete(np.arange(-n, n + 1), n)
+    coefs, offsets = [], []
+    for i in range(3):
+        if abs(d[i]) > 1e-11:
+            coefs.extend(list(c * d[i] / h[i] * scale))
+            o = np.zeros((2 * n, 3), int)
+            o[:, i] = offs
+            offsets.extend(o)
+    return np.array(coefs), np.array(offsets, int)
+
+
+def mehrstellen_a(grid, scale):
+    """LaplaceA (gpaw/fd_operators.py:192-217)."""
+    c = np.divide(-1 / 12, grid.h_cv.diagonal() ** 2) * scale
+    c0, c1, c2 = c[1] + c[2], c[0] + c[2], c[1] + c[0]
+    a = -16.0 * np.sum(c)
+    b = 10.0 * c + 0.125 * a
+    coefs = [a, b[0], b[0], b[1], b[1], b[2], b[2],
+             c0, c0, c0, c0, c1, c1, c1, c1, c2, c2, c2, c2]
+    offsets = [(0, 0, 0), (-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0),
+               (0, 0, -1), (0, 0, 1),
+               (0, -1, -1), (0, -1, 1), (0, 1, -1), (0, 1, 1),
+               (-1, 0, -1), (-1, 0, 1), (1, 0, -1), (1, 0, 1),
+               (-1, -1, 0), (-1, 1, 0), (1, -1, 0), (1, 1, 0)]
+    return np.array(coefs), np.array(offsets, int)
+
+
+def mehrstellen_b():
+    """LaplaceB (gpaw/fd_operators.py:219-229)."""
+    a, b = 0.5, 1.0 / 12.0
+    return (np.array([a, b, b, b, b, b, b]),
+            np.array([(0, 0, 0), (-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0),
+                      (0, 0, -1), (0, 0, 1)], int))
+
+
+def flatten_offsets(offset_pc, n_c):
+    """Padded-buffer flat offsets, range and cfd flag
+    (gpaw/fd_operators.py:26-42)."""
+    offset_pc = np.asarray(offset_pc, int)
+    cfd = not any((o != 0).sum() >= 2 for o in offset_pc)
+    mp = int(max(offset_pc.max(0)))
+    M_c = np.asarray(n_c) + 2 * mp
+    stride_c = np.array([M_c[1] * M_c[2], M_c[2], 1])
+    return np.dot(offset_pc, stride_c).astype(np.int64), mp, cfd
+
+
+class Operator:
+    """FDOperator driving the oracle kernels (gpaw/fd_operators.py:20-90)."""
+
+    def __init__(self, coef_p, offset_pc, grid, dtype=float, kernels=None):
+        self.coefs = np.ascontiguousarray(coef_p, float)
+        self.offsets, self.mp, self.cfd = flatten_offsets(offset_pc, grid.n_c)
+        self.grid = grid
+        self.dtype = dtype
+        self.shape = tuple(grid.n_c)
+        self.size = np.array(grid.n_c, np.int64)
+        self.nb = np.ascontiguousarray(grid.neighbor_cd, np.int64)
+        self.k = _kernels(kernels)
+
+    def apply(self, in_xg, out_xg, phase_cd=None):
+        assert in_xg.shape == out_xg.shape and in_xg.shape[-3:] == self.shape
+        assert in_xg.flags.c_contiguous and out_xg.flags.c_contiguous
+        real = in_xg.dtype == np.float64
+        nin = in_xg.shape[0] if in_xg.ndim == 4 else 1
+        self.k.operator_apply(self.coefs, self.offsets, self.size, self.mp,
+                              self.nb, real, self.cfd, in_xg, out_xg, nin,
+                              phase_cd)
+
+    def relax(self, method, f_g, s_g, n, w=1.0):
+        assert f_g.dtype == np.float64 and f_g.flags.c_contiguous
+        self.k.operator_relax(self.coefs, self.offsets, self.size, self.mp,
+                              self.nb, self.cfd, method, f_g,
+                              np.ascontiguousarray(s_g), n, w)
+
+    def get_diagonal_element(self):
+        d = 0.0
+        for c, o in zip(self.coefs, self.offsets):
+            if o == 0:
+                d = c
+        return d
+
+
+def laplace(grid, scale=1.0, n=1, dtype=float, kernels=None):
+    c, o = laplace_stencil(grid, scale, n)
+    return Operator(c, o, grid, dtype, kernels)
+
+
+def transformer_plan(gdin, gdout, nn):
+    """Paddings / skips of gpaw/transformers.py:19-62."""
+    pad = np.empty((3, 2), np.int64)
+    npad = np.empty((3, 2), np.int64)
+    skip = np.zeros((3, 2), np.int64)
+    if (gdin.N_c == 2 * gdout.N_c).all():
+        pad[:, 0] = 2 * nn - 1 - 2 * gdout.beg_c + gdin.beg_c
+        pad[:, 1] = 2 * nn - 2 + 2 * gdout.end_c - gdin.end_c
+        npad[:, 0] = 2 * nn - 2 + 2 * gdout.beg_c - gdin.beg_c
+        npad[:, 1] = 2 * nn - 1 - 2 * gdout.end_c + gdin.end_c
+        interpolate = False
+    else:
+        assert (gdout.N_c == 2 * gdin.N_c).all()
+        pad[:, 0] = nn - 1 - gdout.beg_c // 2 + gdin.beg_c
+        pad[:, 1] = nn + gdout.end_c // 2 - gdin.end_c
+        npad[:, 0] = nn + gdout.beg_c // 2 - gdin.beg_c
+        npad[:, 1] = nn - 1 - gdout.end_c // 2 + gdin.end_c
+        skip[:, 0] = gdout.beg_c % 2
+        skip[:, 1] = gdout.end_c % 2
+        interpolate = True
+        assert ((gdin.n_c[0] + 2 * nn - 1) * (gdin.n_c[1] + 2 * nn - 1) <=
+                gdout.n_c[0] * gdout.n_c[1])
+    assert (pad >= 0).all()
+    return pad, npad, skip, interpolate
+
+
+class Transformer:
+    """_Transformer driving the oracle kernels (gpaw/transformers.py:19-82)."""
+
+    def __init__(self, gdin, gdout, nn=1, dtype=float, kernels=None):
+        assert 1 <= nn <= 4
+        self.pad, self.npad, self.skip, self.interpolate = transformer_plan(
+            gdin, gdout, nn)
+        self.nn = nn
+        self.size_in = np.array(gdin.n_c, np.int64)
+        self.size_out = np.array(gdout.n_c, np.int64)
+        self.nb = np.ascontiguousarray(gdin.neighbor_cd, np.int64)
+        self.ngpin, self.ngpout = tuple(gdin.n_c), tuple(gdout.n_c)
+        self.k = _kernels(kernels)
+
+    def apply(self, a, b, phases=None):
+        assert a.shape[-3:] == self.ngpin and b.shape[-3:] == self.ngpout
+        real = a.dtype == np.float64
+        nin = a.shape[0] if a.ndim == 4 else 1
+        self.k.transformer_apply(self.size_in, self.size_out, 2 * self.nn,
+                                 self.pad, self.npad, self.skip, self.nb,
+                                 real, self.interpolate, a, b, nin, phases)
+
+
+# ---------------------------------------------------------------------------
+# Charged-system helpers
+# ---------------------------------------------------------------------------
+def _erf(x):
+    return np.vectorize(math.erf, otypes=[float])(x)
+
+
+def _coordinates(grid, tiny=1e-12):
+    """gpaw/utilities/tools.py:92-124 (origin at the cell centre)."""
+    r0 = 0.5 * grid.cell_cv.sum(0)
+    r = grid.grid_point_coordinates() - r0[:, None, None, None]
+    for c in range(3):
+        if grid.pbc_c[c]:
+            L = grid.cell_cv[c, c]
+            r[c] = np.where(r[c] > 0.5 * L, r[c] - L, r[c])
+            r[c] = np.where(r[c] < -0.5 * L, r[c] + L, r[c])
+    r2 = np.sum(r ** 2, axis=0)
+    return r, np.where(r2 < tiny, tiny, r2)
+
+
+def gaussian_monopole(grid, a=19.0):
+    """L=0 generalized Gaussian density and its potential
+    (gpaw/utilities/gauss.py:61-83 with gauss_L[0], gausspot_L[0])."""
+    _, r2 = _coordinates(grid)
+    a = 4 * a * (grid.icell_cv ** 2).sum(1).max()
+    g = np.sqrt(a ** 3 * 4) / np.pi * np.exp(-a * r2)
+    r = np.sqrt(r2)
+    v = 2.0 * 1.7724538509055159 * _erf(np.sqrt(a) * r) / r
+    return g, v
+
+
+def madelung(cell, G=5.0, Ng=(9, 9, 9), Nl=(3, 3, 3)):
+    """Potential of a unit negative point charge in its own neutralising
+    background at its own site, excluding its self term
+    (gpaw/utilities/ewald.py:28-150 with r = r_B = 0, q_B = [-1])."""
+    cell = np.asarray(cell, float)
+    vol = abs(np.linalg.det(cell))
+    recip = np.linalg.inv(cell).T
+    G = G / np.sqrt((cell ** 2).sum(1).max())
+    q = -1.0
+    E_g = 0.0
+    for i in range(-Ng[0], Ng[0] + 1):
+        for j in range(-Ng[1], Ng[1] + 1):
+            for k in range(-Ng[2], Ng[2] + 1):
+                g_v = 2 * np.pi * (i * recip[0] + j * recip[1] + k * recip[2])
+                g2 = np.dot(g_v, g_v)
+                if g2 > 1e-10:
+                    x = 0.25 * g2 / G ** 2
+                    E_g += np.exp(-x) / x
+    E_r = 0.0
+    for i in range(-Nl[0], Nl[0] + 1):
+        for j in range(-Nl[1], Nl[1] + 1):
+            for k in range(-Nl[2], Nl[2] + 1):
+                l_v = i * cell[0] + j * cell[1] + k * cell[2]
+                lr = np.linalg.norm(l_v)
+                if lr > 1e-5:
+                    E_r += (1.0 - math.erf(G * lr)) / lr
+    E0 = q * E_r + q * np.pi / (vol * G ** 2) * E_g
+    E0 -= q * 2.0 * G / np.sqrt(np.pi)
+    E0 += -q * np.pi / (G ** 2 * vol)
+    return E0
+
+
+def gaussian_charges(grid, centers, widths, charges):
+    """Sum of normalised 3-D Gaussians (synthetic densities of the
+    benchmark configurations); periodic axes use the minimum image."""
+    r_vG = grid.grid_point_coordinates()
+    rho = np.zeros(tuple(grid.n_c))
+    L = np.diag(grid.cell_cv)
+    for c, s, q in zip(centers, widths, charges):
+        d = r_vG - np.asarray(c)[:, None, None, None]
+        for ax in range(3):
+            if grid.pbc_c[ax]:
+                d[ax] -= L[ax] * np.round(d[ax] / L[ax])
+        r2 = (d ** 2).sum(0)
+        rho += q * (2 * np.pi * s * s) ** -1.5 * np.exp(-r2 / (2 * s * s))
+    return rho
+
+
+# ---------------------------------------------------------------------------
+# Multigrid Poisson (gpaw/poisson.py:21-276)
+# ---------------------------------------------------------------------------
+class PoissonConvergenceError(RuntimeError):
+    pass
+
+
+class PoissonSolver:
+    def __init__(self, nn=3, relax='J', eps=2e-10, kernels=None):
+        self.nn = nn
+        self.eps = eps
+        self.relax_method = {'GS': 1, 'J': 2}[relax]
+        self.maxiter = 1000
+        self.k = _kernels(kernels)
+        self.charged_periodic_correction = None
+
+    def set_grid_descriptor(self, gd):
+        self.gd = gd
+        self.dv = gd.dv
+        scale = -0.25 / np.pi
+        if self.nn == 'M':
+            c, o = mehrstellen_a(gd, -scale)
+            self.operators = [Operator(c, o, gd, kernels=self.k)]
+            cb, ob = mehrstellen_b()
+            self.B = Operator(cb, ob, gd, kernels=self.k)
+        else:
+            self.operators = [laplace(gd, scale, self.nn, kernels=self.k)]
+            self.B = None
+        self.interpolators, self.restrictors = [], []
+        self.presmooths, self.postsmooths = [2], [1]
+        self.weights = [2.0 / 3.0]
+        level = 0
+        while level < 4:
+            try:
+                gd2 = gd.coarsen()
+            except ValueError:
+                break
+            self.operators.append(laplace(gd2, scale, 1, kernels=self.k))
+            self.interpolators.append(Transformer(gd2, gd, kernels=self.k))
+            self.restrictors.append(Transformer(gd, gd2, kernels=self.k))
+            self.presmooths.append(4)
+            self.postsmooths.append(4)
+            self.weights.append(1.0)
+            level += 1
+            gd = gd2
+        self.levels = level
+
+    def initialize(self):
+        gd = self.gd
+        self.rhos = [gd.zeros()]
+        self.phis = [None]
+        self.residuals = [gd.zeros()]
+        for _ in range(self.levels):
+            gd = gd.coarsen()
+            self.phis.append(gd.zeros())
+            self.rhos.append(gd.zeros())
+            self.residuals.append(gd.zeros())
+        self.step = 0.66666666 / self.operators[0].get_diagonal_element()
+        self.presmooths[self.levels] = 8
+        self.postsmooths[self.levels] = 8
+
+    def solve(self, phi, rho, charge=None, eps=None, maxcharge=1e-6,
+              zero_initial_phi=False):
+        if eps is None:
+            eps = self.eps
+        actual_charge = self.gd.integrate(rho)
+        background = (actual_charge / self.gd.dv /
+                      self.gd.size_of_global_array().prod())
+        if charge is None:
+            charge = actual_charge
+        if abs(charge) <= maxcharge:
+            return self.solve_neutral(phi, rho - background, eps=eps)
+        if self.gd.pbc_c.all():
+            if self.charged_periodic_correction is None:
+                self.charged_periodic_correction = madelung(self.gd.cell_cv)
+            if zero_initial_phi:
+                phi[:] = 0.0
+            else:
+                phi -= charge * self.charged_periodic_correction
+            it = self.solve_neutral(phi, rho - background, eps=eps)
+            phi += charge * self.charged_periodic_correction
+            return it
+        if not self.gd.pbc_c.any():
+            g, v = gaussian_monopole(self.gd)
+            q = actual_charge / np.sqrt(4 * np.pi)
+            rho_neutral = rho - q * g
+            if zero_initial_phi:
+                phi[:] = 0.0
+            else:
+                phi += -q * v
+            it = self.solve_neutral(phi, rho_neutral, eps=eps)
+            phi += q * v
+            return it
+        raise NotImplementedError
+
+    def solve_neutral(self, phi, rho, eps=2e-10):
+        self.phis[0] = phi
+        if self.B is None:
+            self.rhos[0][:] = rho
+        else:
+            self.B.apply(rho, self.rhos[0])
+        niter = 1
+        while self.iterate2(self.step) > eps and niter < self.maxiter:
+            niter += 1
+        if niter == self.maxiter:
+            raise PoissonConvergenceError(
+                'Poisson solver did not converge in %d iterations!' %
+                self.maxiter)
+        if self.gd.pbc_c.all():
+            phi -= np.sum(phi.ravel()) / np.prod(
+                self.gd.size_of_global_array())
+        return niter
+
+    def run_vcycles(self, phi, rho, count):
+        """Exactly `count` V-cycles on the (already neutral) rho; returns the
+        error after each cycle.  Used for fixed-cycle parity checks."""
+        self.phis[0] = phi
+        self.rhos[0][:] = rho
+        return [self.iterate2(self.step) for _ in range(count)]
+
+    def iterate2(self, step, level=0):
+        residual = self.residuals[level]
+        if level < self.levels:
+            self.operators[level].relax(self.relax_method, self.phis[level],
+                                        self.rhos[level],
+                                        self.presmooths[level],
+                                        self.weights[level])
+            self.operators[level].apply(self.phis[level], residual)
+            residual -= self.rhos[level]
+            self.restrictors[level].apply(residual, self.rhos[level + 1])
+            self.phis[level + 1][:] = 0.0
+            self.iterate2(4.0 * step, level + 1)
+            self.interpolators[level].apply(self.phis[level + 1], residual)
+            self.phis[level] -= residual
+        self.operators[level].relax(self.relax_method, self.phis[level],
+                                    self.rhos[level],
+                                    self.postsmooths[level],
+                                    self.weights[level])
+        if level == 0:
+            self.operators[level].apply(self.phis[level], residual)
+            residual -= self.rhos[level]
+            return np.dot(residual.ravel(), residual.ravel()) * self.dv
