@@ -1,0 +1,237 @@
+"""Benchmark: multigrid Poisson solves/s on one MI355X, with the FD roofline.
+
+Workload (a "step" = one full Poisson solve from phi = 0):
+  96^3 non-periodic grid (95^3 arrays), h = 0.2 A, PoissonSolver(nn=3,
+  relax='J'), 4 coarsenings, FP64; density = 8 seeded normalised Gaussians
+  (centres >= 2 A from the walls, widths U[0.5, 1.5] A, charges +-1, sum 0).
+  Solved until dv*sum(r^2) <= 2e-10 AND ||L phi - rho||_2 / ||rho||_2 <= 1e-10
+  (the reference criterion and the relative-residual target; the second is
+  the stricter one here).
+
+Also measured (same process, not the headline value):
+  * roofline of the dominant kernel of the solve (level-0 Jacobi sweep),
+    timed with HIP events on the launch stream;
+  * FD Laplace (nn=3, complex128, 256 bands x 64^3, Bloch phases) HBM GB/s;
+  * the reference's own C kernels (oracle/_ref, built from the reference
+    sources) driven by the restated V-cycle, on one host core.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+BOHR = 0.52917720859          # Angstrom per bohr (ASE of the reference era)
+PEAK_HBM_GBS = 8000.0         # MI355X HBM3E
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument('--gpus', type=int, default=1)
+    p.add_argument('--steps', type=int, default=20)
+    p.add_argument('--warmup', type=int, default=3)
+    p.add_argument('--no-cpu', action='store_true')
+    p.add_argument('--no-extra', action='store_true')
+    p.add_argument('--cpu-seconds', type=float, default=12.0)
+    return p.parse_args()
+
+
+def c2_inputs(np, seed=42):
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.utilities import gaussian_density
+    N = 96
+    h = 0.2 / BOHR
+    gd = GridDescriptor((N,) * 3, (N * h,) * 3, False)
+    rng = np.random.default_rng(seed)
+    L = np.diag(gd.cell_cv)
+    margin = 2.0 / BOHR
+    centers = [rng.uniform(margin, L - margin) for _ in range(8)]
+    widths = rng.uniform(0.5, 1.5, 8) / BOHR
+    charges = np.array([1.0, -1.0] * 4)
+    rho = gaussian_density(gd, centers, widths, charges)
+    return gd, rho
+
+
+def eps_for(np, gd, rho, rel=1e-10, eps_ref=2e-10):
+    """Reference-criterion eps that also guarantees ||r||/||rho|| <= rel
+    for the neutralised density the solver works on."""
+    bg = rho.sum() * gd.dv / gd.dv / np.prod(gd.get_size_of_global_array())
+    rn = np.linalg.norm((rho - bg).ravel())
+    return min(eps_ref, gd.dv * (rel * rn) ** 2), rn
+
+
+def event_time(torch, fn, reps):
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    fn()
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(reps):
+        fn()
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def jacobi_roofline(torch, np, ps, rho_d):
+    """Average duration of one level-0 Jacobi sweep (the dominant kernel of
+    the V-cycle) over a timed run, HIP events on torch's current stream,
+    which is the stream the library launches on."""
+    op = ps.operators[0].operator
+    n = int(np.prod(ps.gd.n_c))
+    f = torch.zeros(tuple(ps.gd.n_c), dtype=torch.float64, device='cuda')
+    reps = 200
+    t = event_time(torch, lambda: op.relax(2, f, rho_d, 2, 2.0 / 3.0), reps) / 2
+    algo = 24.0 * n          # read phi, read rho, write phi (8 B each)
+    return {'bound': 'hbm', 'achieved': round(algo / t / 1e9, 1),
+            'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
+            'frac': round(algo / t / 1e9 / PEAK_HBM_GBS, 4),
+            'traffic': None, 'kernel': 'k_star<double,3,JACOBI> level-0 sweep',
+            'bytes_per_launch': algo, 'avg_launch_us': round(t * 1e6, 2),
+            'note': '95^3 FP64 working set (20.6 MB) is cache resident'}
+
+
+def fd_laplace_c3(torch, np):
+    """C3 Laplace: complex128 nn=3 on 256 bands of 64^3, periodic, phases."""
+    from gpaw_amd.fd_operators import Laplace
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    N, nb = 64, 256
+    gd = GridDescriptor((N,) * 3, (N * 0.2 / BOHR,) * 3, True)
+    rng = np.random.default_rng(3)
+    k_c = rng.uniform(-0.5, 0.5, 3)
+    ph = gd.phase_cd(k_c)
+    lap = Laplace(gd, -0.5, 3, complex)
+    g = torch.Generator(device='cuda').manual_seed(11)
+    a = torch.randn((nb, N, N, N), dtype=torch.complex128, device='cuda',
+                    generator=g)
+    b = torch.empty_like(a)
+    t = event_time(torch, lambda: lap.apply(a, b, ph), 10)
+    algo = 2.0 * 16 * nb * N ** 3
+    gbs = algo / t / 1e9
+    del a, b
+    return {'config': 'complex128 nn=3 Laplace, 256 bands x 64^3, Bloch phases',
+            'avg_apply_ms': round(t * 1e3, 3), 'GBps': round(gbs, 1),
+            'pct_peak': round(100 * gbs / PEAK_HBM_GBS, 1),
+            'bytes_per_apply': algo}
+
+
+def cpu_baseline(np, gd_shape_args, rho, eps, budget):
+    """The reference's own C kernels (oracle/_ref, compiled from the
+    reference sources) under the restated V-cycle driver, one host core."""
+    import oracle as O
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:
+        threadpool_limits = None
+    kind = 'reference' if O.Kernels.available('reference') else 'port'
+    og = O.Grid(*gd_shape_args)
+    ps = O.PoissonSolver(nn=3, relax='J', kernels=kind)
+    ps.set_grid_descriptor(og)
+    ps.initialize()
+    ctx = threadpool_limits(1) if threadpool_limits else None
+    try:
+        try:
+            os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+        except (AttributeError, OSError):
+            pass
+        if ctx:
+            ctx.__enter__()
+        n = 0
+        t0 = time.perf_counter()
+        niter = None
+        while True:
+            phi = og.zeros()
+            niter = ps.solve(phi, rho.copy(), eps=eps)
+            n += 1
+            if time.perf_counter() - t0 > budget or n >= 10:
+                break
+        dt = time.perf_counter() - t0
+    finally:
+        if ctx:
+            ctx.__exit__(None, None, None)
+    return {'value': round(n / dt, 4), 'unit': 'solves/s', 'cores': 1,
+            'kind': kind,
+            'sample': '%d full solve(s) of the same 95^3 workload (%d V-cycles '
+                      'each), %.1f s, one pinned core, BLAS threads = 1' %
+                      (n, niter, dt)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print('bench.py runs on one GPU only (--gpus %d requested): the hot '
+              'path here is single-GPU; see DESIGN.md for how it would shard'
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from gpaw_amd.poisson import PoissonSolver
+
+    gd, rho = c2_inputs(np)
+    eps, rho_norm = eps_for(np, gd, rho)
+    ps = PoissonSolver(nn=3, relax='J', eps=eps)
+    ps.set_grid_descriptor(gd)
+    ps.initialize()
+    rho_d = torch.from_numpy(rho).to('cuda')
+    phi = torch.zeros(tuple(gd.n_c), dtype=torch.float64, device='cuda')
+
+    def step():
+        phi.zero_()
+        return ps.solve(phi, rho_d)
+
+    for _ in range(args.warmup):
+        niter = step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        niter = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+
+    # verify the stopping point independently: r = L phi - (rho - bg)
+    bg = float(rho.sum()) / np.prod(gd.get_size_of_global_array())
+    rho_n = torch.from_numpy(rho - bg).to('cuda')
+    r = torch.empty_like(phi)
+    ps.operators[0].operator.residual(phi, rho_n, r)
+    rel = float(torch.linalg.vector_norm(r).item()) / float(
+        torch.linalg.vector_norm(rho_n).item())
+
+    out = {
+        'metric': 'FP64 multigrid Poisson solves/s at 1e-10 relative residual',
+        'value': round(args.steps / dt, 3),
+        'unit': 'solves/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(dt / args.steps * 1e3, 3),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64',
+        'data': 'synthetic',
+        'config': {'workload': 'PoissonSolver(nn=3, J), 96^3 zero-boundary '
+                               '(95^3 arrays), h=0.2 A, 8 neutral Gaussians',
+                   'grid': [95, 95, 95], 'levels': ps.levels,
+                   'vcycles_per_solve': niter,
+                   'eps_reference_criterion': eps,
+                   'final_rel_residual': rel,
+                   'parallelism': 'single GPU'},
+    }
+    out['roofline'] = jacobi_roofline(torch, np, ps, rho_n)
+    if not args.no_extra:
+        out['fd_laplace'] = fd_laplace_c3(torch, np)
+    if not args.no_cpu:
+        cb = cpu_baseline(np, ((96,) * 3, gd.cell_cv, False), rho, eps,
+                          args.cpu_seconds)
+        out['cpu_baseline'] = cb
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

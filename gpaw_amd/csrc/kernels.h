@@ -1,0 +1,77 @@
+// Host-side launch interface of the gfx950 kernels (internal to the library).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "device_common.h"
+
+namespace gpaw {
+
+enum DType { DT_F64 = 0, DT_C128 = 1, DT_F32 = 2, DT_C64 = 3 };
+
+inline int dtype_size(int dt) { return dt == DT_F64 ? 8 : dt == DT_C128 ? 16 : dt == DT_F32 ? 4 : 8; }
+inline bool dtype_complex(int dt) { return dt == DT_C128 || dt == DT_C64; }
+
+enum StencilMode {
+    ST_APPLY = 0,    // out = sum_c coef_c * in[g + off_c]
+    ST_RESID = 1,    // out = apply(in) - aux
+    ST_JACOBI = 2,   // out = (1-w) in + w (aux - sum_{c>=1} ...) / coef_0
+};
+
+#define GPAW_MAXC 512
+#define GPAW_MAXR 6
+
+// A stencil: coefficients in the caller's order (the summation order) and
+// integer offsets.  `star` marks the canonical order of an orthogonal-cell
+// Laplacian, [centre, +1..+R along axis 2, -1..-R along 2, then axis 1,
+// then axis 0], for which the register-queue kernel is used.
+struct Stencil {
+    int nc = 0;
+    int range = 0;
+    int star_r = 0;                 // R of the canonical star order, 0 if not
+    double coefs[GPAW_MAXC];
+    int off[GPAW_MAXC][3];
+    double* d_coefs = nullptr;      // device copies for the generic kernel
+    int* d_off = nullptr;
+};
+
+// Work estimate used to size partial-sum buffers.
+int stencil_partials(const Stencil& s, const Geom& g, int nin);
+
+// Launch one stencil pass.  `partial` (optional) receives one FP64 partial
+// sum of |result|^2 per thread block; `write` = false skips the store.
+hipError_t launch_stencil(const Stencil& s, int dtype, int mode, const void* in, void* out,
+                          const void* aux, const Geom& g, const Phases& ph, int nin,
+                          double w, double* partial, int* npartial, bool write,
+                          hipStream_t st);
+
+// Restriction / interpolation.
+struct Transfer {
+    int K = 2;
+    int interpolate = 0;
+    Geom gin;            // input geometry (n, periodic flags)
+    int nout[3];
+    int lo[3];           // low padding per axis (maps output to input index)
+    int skip[3][2];
+};
+
+enum TransferMode {
+    TR_STORE = 0,        // out = T(in)
+    TR_SUBTRACT = 1,     // out -= T(in)   (interpolate + correct)
+};
+
+hipError_t launch_transfer(const Transfer& t, int dtype, int mode, const void* in, void* out,
+                           void* zero_out, const Phases& ph, int nin, hipStream_t st);
+
+// Reductions and element-wise helpers (FP64 real unless stated).
+hipError_t launch_reduce_partials(const double* partial, int n, double scale, double* result,
+                                  hipStream_t st);
+hipError_t launch_sum(const double* a, long long n, double* partial, int* npartial,
+                      hipStream_t st);
+hipError_t launch_axpby(double* y, const double* x, double a, double b, long long n,
+                        hipStream_t st);   // y = a*y + b*x
+hipError_t launch_sub_scalar(double* y, const double* x, double b, long long n,
+                             hipStream_t st);  // y = x - b
+hipError_t launch_fill(void* y, int dtype, long long n, hipStream_t st);
+hipError_t launch_copy(void* y, const void* x, int dtype, long long n, hipStream_t st);
+hipError_t launch_cast(void* y, int dty, const void* x, int dtx, long long n, hipStream_t st);
+
+}  // namespace gpaw

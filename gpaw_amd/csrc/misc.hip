@@ -1,0 +1,140 @@
+// Reductions and element-wise kernels used around the stencils
+// (gfx950).  FP64 sums are deterministic: one partial per block in a fixed
+// tree order, then one block sums the partials in index order.
+#include "kernels.h"
+
+namespace gpaw {
+
+__global__ __launch_bounds__(256) void k_reduce_partials(const double* __restrict__ p, int n,
+                                                         double scale, double* __restrict__ out)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += p[i];
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) out[0] = s * scale;
+}
+
+__global__ __launch_bounds__(256) void k_sum(const double* __restrict__ a, i64 n,
+                                             double* __restrict__ partial)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    const i64 stride = (i64)gridDim.x * 256;
+    for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) s += a[i];
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_axpby(double* __restrict__ y, const double* __restrict__ x,
+                                               double a, double b, i64 n)
+{
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = a * y[i] + b * x[i];
+}
+
+// y = x - b  (numpy's rho - background, phi -= phi_ave)
+__global__ __launch_bounds__(256) void k_sub_scalar(double* __restrict__ y,
+                                                    const double* __restrict__ x, double b, i64 n)
+{
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = x[i] - b;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_fill(T* __restrict__ y, i64 n)
+{
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = T(0);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_copy(T* __restrict__ y, const T* __restrict__ x, i64 n)
+{
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = x[i];
+}
+
+template <class TO, class TI>
+__global__ __launch_bounds__(256) void k_cast(TO* __restrict__ y, const TI* __restrict__ x, i64 n)
+{
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = (TO)x[i];
+}
+
+static dim3 grid1(i64 n) { return dim3((unsigned)((n + 255) / 256)); }
+
+hipError_t launch_reduce_partials(const double* partial, int n, double scale, double* result,
+                                  hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, partial, n, scale, result);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum(const double* a, long long n, double* partial, int* npartial, hipStream_t st)
+{
+    int nb = (int)((n + 255) / 256);
+    if (nb > 1024) nb = 1024;
+    if (nb < 1) nb = 1;
+    *npartial = nb;
+    hipLaunchKernelGGL(k_sum, dim3(nb), dim3(256), 0, st, a, (i64)n, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpby(double* y, const double* x, double a, double b, long long n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_axpby, grid1(n), dim3(256), 0, st, y, x, a, b, (i64)n);
+    return hipGetLastError();
+}
+
+hipError_t launch_sub_scalar(double* y, const double* x, double b, long long n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sub_scalar, grid1(n), dim3(256), 0, st, y, x, b, (i64)n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(void* y, int dtype, long long n, hipStream_t st)
+{
+    // complex arrays are filled as twice as many reals
+    i64 m = (i64)n * (dtype_complex(dtype) ? 2 : 1);
+    if (m <= 0) return hipSuccess;
+    if (dtype == DT_F64 || dtype == DT_C128)
+        hipLaunchKernelGGL(k_fill<double>, grid1(m), dim3(256), 0, st, (double*)y, m);
+    else
+        hipLaunchKernelGGL(k_fill<float>, grid1(m), dim3(256), 0, st, (float*)y, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(void* y, const void* x, int dtype, long long n, hipStream_t st)
+{
+    i64 m = (i64)n * (dtype_complex(dtype) ? 2 : 1);
+    if (m <= 0) return hipSuccess;
+    if (dtype == DT_F64 || dtype == DT_C128)
+        hipLaunchKernelGGL(k_copy<double>, grid1(m), dim3(256), 0, st, (double*)y,
+                           (const double*)x, m);
+    else
+        hipLaunchKernelGGL(k_copy<float>, grid1(m), dim3(256), 0, st, (float*)y,
+                           (const float*)x, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_cast(void* y, int dty, const void* x, int dtx, long long n, hipStream_t st)
+{
+    if (dtype_complex(dty) != dtype_complex(dtx)) return hipErrorInvalidValue;
+    i64 m = (i64)n * (dtype_complex(dty) ? 2 : 1);
+    if (m <= 0) return hipSuccess;
+    const bool yd = (dty == DT_F64 || dty == DT_C128), xd = (dtx == DT_F64 || dtx == DT_C128);
+    if (yd && !xd)
+        hipLaunchKernelGGL((k_cast<double, float>), grid1(m), dim3(256), 0, st, (double*)y,
+                           (const float*)x, m);
+    else if (!yd && xd)
+        hipLaunchKernelGGL((k_cast<float, double>), grid1(m), dim3(256), 0, st, (float*)y,
+                           (const double*)x, m);
+    else
+        return launch_copy(y, x, dty, n, st);
+    return hipGetLastError();
+}
+
+}  // namespace gpaw

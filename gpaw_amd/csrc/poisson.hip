@@ -1,0 +1,305 @@
+// Device-resident multigrid V-cycle (PoissonSolver.iterate2 /
+// solve_neutral, gpaw/poisson.py:193-218,245-276).
+//
+// The plan owns every level array: rho_l, the residual r_l and a ping-pong
+// pair for phi_l (weighted Jacobi reads the old field and writes a new one,
+// c/bmgs/relax.c:47-71).  One V-cycle is a fixed sequence of kernels:
+//
+//   level l < L:  relax(pre_l) ; r_l = L_l phi_l - rho_l ;
+//                 rho_{l+1} = R r_l and phi_{l+1} = 0 (one fused kernel) ;
+//                 V-cycle(l+1) ; phi_l -= I phi_{l+1} (fused) ;
+//   every level:  relax(post_l)
+//   level 0:      dv * sum (L phi - rho)^2 into a device scalar (no r stored)
+//
+// identical in arithmetic to the reference's sequence of Operator /
+// Transformer calls and numpy updates.  The sequence is captured once into
+// a hipGraph and replayed; when level 0 does an odd number of sweeps per
+// cycle the phi_0 buffers swap roles every cycle, so two graphs (one per
+// starting buffer) are captured and alternated.  The host reads the 8-byte
+// error after each cycle to apply the reference's stopping rule exactly.
+#include <cstdlib>
+#include <vector>
+
+#include "capi_internal.h"
+
+using namespace gpaw;
+
+struct gpaw_poisson {
+    int L = 0;                 // number of coarsenings
+    int method = 2;
+    double dv = 1.0;
+    std::vector<gpaw_operator*> ops;
+    std::vector<gpaw_transformer*> rst, itp;
+    std::vector<int> pre, post;
+    std::vector<double> w;
+    std::vector<double*> phiA, phiB, rho, res;
+    std::vector<long long> npts;
+    double* d_err = nullptr;
+    double* h_err = nullptr;
+    hipStream_t cap = nullptr;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    int use_graph = 1;
+    int swaps = 0;             // 1 if level 0 does an odd number of sweeps per cycle
+};
+
+namespace {
+
+int alloc(double** p, long long n)
+{
+    return gpaw_check(hipMalloc((void**)p, sizeof(double) * (n > 0 ? n : 1)), "poisson: level array");
+}
+
+// relax k sweeps on level l, ping-ponging between phi[l] and alt; returns via
+// `cur` which buffer holds the result
+hipError_t relax(gpaw_poisson* ps, int l, int k, double*& cur, double*& alt, hipStream_t st)
+{
+    Phases ph;
+    gpaw_phases(nullptr, ph);
+    gpaw_operator* op = ps->ops[l];
+    for (int it = 0; it < k; it++) {
+        hipError_t e = launch_stencil(op->st, DT_F64, ST_JACOBI, cur, alt, ps->rho[l], op->g, ph,
+                                      1, ps->w[l], nullptr, nullptr, true, st);
+        if (e != hipSuccess) return e;
+        double* t = cur;
+        cur = alt;
+        alt = t;
+    }
+    return hipSuccess;
+}
+
+hipError_t emit_level(gpaw_poisson* ps, int l, double*& cur0, double*& alt0, hipStream_t st)
+{
+    Phases ph;
+    gpaw_phases(nullptr, ph);
+    double* cur = l == 0 ? cur0 : ps->phiA[l];
+    double* alt = l == 0 ? alt0 : ps->phiB[l];
+    hipError_t e = hipSuccess;
+    gpaw_operator* op = ps->ops[l];
+    if (l < ps->L) {
+        e = relax(ps, l, ps->pre[l], cur, alt, st);
+        if (e != hipSuccess) return e;
+        e = launch_stencil(op->st, DT_F64, ST_RESID, cur, ps->res[l], ps->rho[l], op->g, ph, 1,
+                           0.0, nullptr, nullptr, true, st);
+        if (e != hipSuccess) return e;
+        // rho_{l+1} = R r_l ; phi_{l+1} = 0
+        e = launch_transfer(ps->rst[l]->t, DT_F64, TR_STORE, ps->res[l], ps->rho[l + 1],
+                            ps->phiA[l + 1], ph, 1, st);
+        if (e != hipSuccess) return e;
+        double* dummy0 = nullptr;
+        double* dummy1 = nullptr;
+        e = emit_level(ps, l + 1, dummy0, dummy1, st);
+        if (e != hipSuccess) return e;
+        // the coarse result sits in phiA[l+1] when the coarse level does an
+        // even number of sweeps, else in phiB[l+1]
+        const int kc = (l + 1 < ps->L ? ps->pre[l + 1] : 0) + ps->post[l + 1];
+        const double* coarse = (kc % 2) ? ps->phiB[l + 1] : ps->phiA[l + 1];
+        e = launch_transfer(ps->itp[l]->t, DT_F64, TR_SUBTRACT, coarse, cur, nullptr, ph, 1, st);
+        if (e != hipSuccess) return e;
+    }
+    e = relax(ps, l, ps->post[l], cur, alt, st);
+    if (e != hipSuccess) return e;
+    if (l == 0) {
+        int np = 0;
+        e = launch_stencil(op->st, DT_F64, ST_RESID, cur, nullptr, ps->rho[0], op->g, ph, 1, 0.0,
+                           op->partial, &np, false, st);
+        if (e != hipSuccess) return e;
+        e = launch_reduce_partials(op->partial, np, ps->dv, ps->d_err, st);
+        cur0 = cur;
+        alt0 = alt;
+    }
+    return e;
+}
+
+hipError_t emit_cycle(gpaw_poisson* ps, int start, hipStream_t st)
+{
+    double* cur = start ? ps->phiB[0] : ps->phiA[0];
+    double* alt = start ? ps->phiA[0] : ps->phiB[0];
+    return emit_level(ps, 0, cur, alt, st);
+}
+
+int capture(gpaw_poisson* ps)
+{
+    if (ps->exec[0]) return GPAW_OK;
+    for (int s = 0; s < 1 + ps->swaps; s++) {
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(ps->cap, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return gpaw_check(e, "poisson: begin capture");
+        hipError_t e1 = emit_cycle(ps, s, ps->cap);
+        e = hipStreamEndCapture(ps->cap, &g);
+        if (e1 != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return gpaw_check(e1, "poisson: captured launch");
+        }
+        if (e != hipSuccess) return gpaw_check(e, "poisson: end capture");
+        e = hipGraphInstantiate(&ps->exec[s], g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) return gpaw_check(e, "poisson: instantiate");
+    }
+    return GPAW_OK;
+}
+
+int run_cycle(gpaw_poisson* ps, int state, hipStream_t st)
+{
+    if (ps->use_graph) {
+        int rc = capture(ps);
+        if (rc) return rc;
+        return gpaw_check(hipGraphLaunch(ps->exec[state], st), "poisson: graph launch");
+    }
+    return gpaw_check(emit_cycle(ps, state, st), "poisson: V-cycle");
+}
+
+int read_error(gpaw_poisson* ps, hipStream_t st, double* err)
+{
+    hipError_t e = hipMemcpyAsync(ps->h_err, ps->d_err, sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return gpaw_check(e, "poisson: error readback");
+    *err = ps->h_err[0];
+    return GPAW_OK;
+}
+
+}  // namespace
+
+extern "C" int gpaw_poisson_new(int levels, gpaw_operator* const* ops,
+                                gpaw_transformer* const* restrictors,
+                                gpaw_transformer* const* interpolators, const int* presmooths,
+                                const int* postsmooths, const double* weights, int relax_method,
+                                double dv, gpaw_poisson** out)
+{
+    if (!out || !ops || !presmooths || !postsmooths || !weights || levels < 0 || levels > 16)
+        return gpaw_fail(GPAW_EINVAL, "poisson: bad argument");
+    *out = nullptr;
+    if (relax_method != 2)
+        return gpaw_fail(GPAW_EUNSUPPORTED,
+                         "poisson: only weighted Jacobi (method 2) runs on the device");
+    if (levels > 0 && (!restrictors || !interpolators))
+        return gpaw_fail(GPAW_EINVAL, "poisson: missing transformers");
+    for (int l = 0; l <= levels; l++) {
+        if (!ops[l] || !ops[l]->real) return gpaw_fail(GPAW_EINVAL, "poisson: operator %d", l);
+        if (presmooths[l] < 0 || postsmooths[l] < 0)
+            return gpaw_fail(GPAW_EINVAL, "poisson: negative smoothing count");
+    }
+    for (int l = 0; l < levels; l++) {
+        const Transfer& r = restrictors[l]->t;
+        const Transfer& i = interpolators[l]->t;
+        for (int a = 0; a < 3; a++) {
+            if (r.interpolate || !i.interpolate || r.gin.n[a] != ops[l]->g.n[a] ||
+                r.nout[a] != ops[l + 1]->g.n[a] || i.gin.n[a] != ops[l + 1]->g.n[a] ||
+                i.nout[a] != ops[l]->g.n[a])
+                return gpaw_fail(GPAW_EINVAL, "poisson: level %d transformers do not match "
+                                              "the operators", l);
+        }
+    }
+    gpaw_poisson* ps = new gpaw_poisson();
+    ps->L = levels;
+    ps->method = relax_method;
+    ps->dv = dv;
+    ps->ops.assign(ops, ops + levels + 1);
+    if (levels) {
+        ps->rst.assign(restrictors, restrictors + levels);
+        ps->itp.assign(interpolators, interpolators + levels);
+    }
+    ps->pre.assign(presmooths, presmooths + levels + 1);
+    ps->post.assign(postsmooths, postsmooths + levels + 1);
+    ps->w.assign(weights, weights + levels + 1);
+    ps->swaps = ((levels > 0 ? ps->pre[0] : 0) + ps->post[0]) % 2;
+    const char* env = getenv("GPAW_AMD_GRAPH");
+    ps->use_graph = !(env && env[0] == '0');
+    int rc = GPAW_OK;
+    for (int l = 0; l <= levels && !rc; l++) {
+        const Geom& g = ps->ops[l]->g;
+        const long long n = (long long)g.n[0] * g.n[1] * g.n[2];
+        ps->npts.push_back(n);
+        double *a = nullptr, *b = nullptr, *r = nullptr, *x = nullptr;
+        rc = alloc(&a, n);
+        if (!rc) rc = alloc(&b, n);
+        if (!rc) rc = alloc(&r, n);
+        if (!rc) rc = alloc(&x, n);
+        ps->phiA.push_back(a);
+        ps->phiB.push_back(b);
+        ps->rho.push_back(r);
+        ps->res.push_back(x);
+    }
+    if (!rc) rc = gpaw_op_ensure_partials(ps->ops[0]);
+    if (!rc) rc = alloc(&ps->d_err, 1);
+    if (!rc) rc = gpaw_check(hipHostMalloc((void**)&ps->h_err, sizeof(double)), "poisson: pinned");
+    if (!rc) rc = gpaw_check(hipStreamCreateWithFlags(&ps->cap, hipStreamNonBlocking),
+                             "poisson: capture stream");
+    if (rc) {
+        gpaw_poisson_free(ps);
+        return rc;
+    }
+    *out = ps;
+    return GPAW_OK;
+}
+
+static int load_inputs(gpaw_poisson* ps, const double* phi, const double* rho, hipStream_t st)
+{
+    hipError_t e = launch_copy(ps->rho[0], rho, DT_F64, ps->npts[0], st);
+    if (e == hipSuccess) e = launch_copy(ps->phiA[0], phi, DT_F64, ps->npts[0], st);
+    return gpaw_check(e, "poisson: load");
+}
+
+extern "C" int gpaw_poisson_vcycles(gpaw_poisson* ps, double* phi, const double* rho, int ncycles,
+                                    double* errors, void* stream)
+{
+    if (!ps || !phi || !rho || ncycles < 0) return gpaw_fail(GPAW_EINVAL, "poisson: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    int rc = load_inputs(ps, phi, rho, st);
+    int state = 0;
+    for (int c = 0; c < ncycles && !rc; c++) {
+        rc = run_cycle(ps, state, st);
+        state ^= ps->swaps;
+        double err = 0.0;
+        if (!rc) rc = read_error(ps, st, &err);
+        if (!rc && errors) errors[c] = err;
+    }
+    if (!rc)
+        rc = gpaw_check(launch_copy(phi, state ? ps->phiB[0] : ps->phiA[0], DT_F64, ps->npts[0], st),
+                        "poisson: store");
+    return rc;
+}
+
+extern "C" int gpaw_poisson_solve_neutral(gpaw_poisson* ps, double* phi, const double* rho,
+                                          double eps, int maxiter, int* niter, double* error,
+                                          void* stream)
+{
+    if (!ps || !phi || !rho || maxiter < 1) return gpaw_fail(GPAW_EINVAL, "poisson: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    int rc = load_inputs(ps, phi, rho, st);
+    if (rc) return rc;
+    int state = 0;
+    int n = 1;
+    double err = 0.0;
+    for (;;) {   // while iterate2() > eps and niter < maxiter: niter += 1
+        rc = run_cycle(ps, state, st);
+        if (rc) return rc;
+        state ^= ps->swaps;
+        rc = read_error(ps, st, &err);
+        if (rc) return rc;
+        if (!(err > eps) || n >= maxiter) break;
+        n++;
+    }
+    rc = gpaw_check(launch_copy(phi, state ? ps->phiB[0] : ps->phiA[0], DT_F64, ps->npts[0], st),
+                    "poisson: store");
+    if (niter) *niter = n;
+    if (error) *error = err;
+    if (rc) return rc;
+    if (n == maxiter)
+        return gpaw_fail(GPAW_ENOCONV, "Poisson solver did not converge in %d iterations!",
+                         maxiter);
+    return GPAW_OK;
+}
+
+extern "C" void gpaw_poisson_free(gpaw_poisson* ps)
+{
+    if (!ps) return;
+    for (int s = 0; s < 2; s++)
+        if (ps->exec[s]) hipGraphExecDestroy(ps->exec[s]);
+    for (auto v : {&ps->phiA, &ps->phiB, &ps->rho, &ps->res})
+        for (double* p : *v)
+            if (p) hipFree(p);
+    if (ps->d_err) hipFree(ps->d_err);
+    if (ps->h_err) hipHostFree(ps->h_err);
+    if (ps->cap) hipStreamDestroy(ps->cap);
+    delete ps;
+}

@@ -1,0 +1,323 @@
+// Finite-difference stencil kernels for gfx950: apply, residual and
+// weighted-Jacobi smoothing.
+//
+// Arithmetic follows c/bmgs/fd.c:16-51 (x = 0; x += a[off_c] * coef_c in
+// coefficient order) and c/bmgs/relax.c:47-71 (x over c >= 1, then
+// (1-w)*f + w*(s - x)/coef_0), evaluated without FMA contraction.
+//
+// Two kernels:
+//  * k_star: the hot path.  For the canonical orthogonal Laplacian order
+//    (centre, +-k along axis 2, +-k along axis 1, +-k along axis 0) each
+//    thread owns one (i1, i2) column and sweeps a run of i0 planes, keeping
+//    the 2R+1 values along i0 in a register queue, so axis-0 neighbours are
+//    read from HBM once.  In-plane neighbours come from the same plane rows
+//    that the neighbouring lanes / waves of the block load (L1/L2 hits).
+//    Waves are 64 lanes along i2, so every HBM access is a coalesced row.
+//  * k_generic: any stencil (non-orthogonal cells, Mehrstellen, gradients):
+//    one thread per point, coefficients read through the scalar cache.
+// Both fuse the residual (apply - rho) and an FP64 sum of |result|^2 per
+// block (deterministic tree order) for the multigrid convergence test.
+#include "kernels.h"
+
+namespace gpaw {
+
+struct StarCoefs {
+    double c[6 * GPAW_MAXR + 1];
+};
+
+template <class V>
+__device__ __forceinline__ double norm2(V v);
+template <> __device__ __forceinline__ double norm2<double>(double v) { return v * v; }
+template <> __device__ __forceinline__ double norm2<float>(float v) { return (double)v * (double)v; }
+template <> __device__ __forceinline__ double norm2<double2>(double2 v) { return v.x * v.x + v.y * v.y; }
+template <> __device__ __forceinline__ double norm2<float2>(float2 v)
+{
+    return (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+}
+
+// load with a precomputed wrap flag for one axis (0 inside, 1 high, 2 low, 3 zero)
+template <class V>
+__device__ __forceinline__ V flagged_load(const V* __restrict__ p, int flag, const Phases& ph, int axis)
+{
+    if (flag == 3) return vzero<V>();
+    V v = *p;
+    if (VT<V>::cplx && flag) v = pmul(v, ph.re[axis][flag - 1], ph.im[axis][flag - 1]);
+    return v;
+}
+
+template <class V, int R, int MODE, bool NORM, bool WRITE>
+__global__ __launch_bounds__(256) void k_star(const V* __restrict__ in, V* __restrict__ out,
+                                              const V* __restrict__ aux, Geom g, Phases ph,
+                                              StarCoefs sc, double w, int q, int ntz, i64 ng,
+                                              double* __restrict__ partial)
+{
+    typedef typename VT<V>::R RT;
+    const int tz = blockIdx.x % ntz, ty = blockIdx.x / ntz;
+    const int i2 = tz * blockDim.x + threadIdx.x;
+    const int i1 = ty * blockDim.y + threadIdx.y;
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    const bool active = (i1 < n1) && (i2 < n2);
+    double acc = 0.0;
+    if (active) {
+        const i64 plane = (i64)n1 * n2;
+        const i64 boff = (i64)blockIdx.z * ng;
+        const V* __restrict__ a = in + boff;
+        const int i0s = blockIdx.y * q;
+        const int i0e = min(i0s + q, n0);
+        int zc[2 * R], zf[2 * R], yc[2 * R], yf[2 * R];
+#pragma unroll
+        for (int k = 1; k <= R; k++) {
+            int j = i2 + k;
+            zf[k - 1] = wrap_index(j, n2, g.per[2]);
+            zc[k - 1] = j;
+            j = i2 - k;
+            zf[R + k - 1] = wrap_index(j, n2, g.per[2]);
+            zc[R + k - 1] = j;
+            j = i1 + k;
+            yf[k - 1] = wrap_index(j, n1, g.per[1]);
+            yc[k - 1] = j * n2;
+            j = i1 - k;
+            yf[R + k - 1] = wrap_index(j, n1, g.per[1]);
+            yc[R + k - 1] = j * n2;
+        }
+        const int row = i1 * n2;
+        V qv[2 * R + 1];
+#pragma unroll
+        for (int k = 0; k < 2 * R; k++) {
+            int p = i0s - R + k;
+            const int f = wrap_index(p, n0, g.per[0]);
+            qv[k] = flagged_load(a + (f == 3 ? 0 : (i64)p * plane + row + i2), f, ph, 0);
+        }
+        for (int i0 = i0s; i0 < i0e; i0++) {
+            {
+                int p = i0 + R;
+                const int f = wrap_index(p, n0, g.per[0]);
+                qv[2 * R] = flagged_load(a + (f == 3 ? 0 : (i64)p * plane + row + i2), f, ph, 0);
+            }
+            const V* __restrict__ pl = a + (i64)i0 * plane;
+            V x = vzero<V>();
+            if (MODE != ST_JACOBI) x = vadd(x, vmul(qv[R], (RT)sc.c[0]));
+#pragma unroll
+            for (int k = 0; k < 2 * R; k++)   // axis 2: +1..+R then -1..-R
+                x = vadd(x, vmul(flagged_load(pl + row + zc[k], zf[k], ph, 2), (RT)sc.c[1 + k]));
+#pragma unroll
+            for (int k = 0; k < 2 * R; k++)   // axis 1
+                x = vadd(x, vmul(flagged_load(pl + yc[k] + i2, yf[k], ph, 1),
+                                 (RT)sc.c[1 + 2 * R + k]));
+#pragma unroll
+            for (int k = 1; k <= R; k++)      // axis 0: +k from the queue
+                x = vadd(x, vmul(qv[R + k], (RT)sc.c[4 * R + k]));
+#pragma unroll
+            for (int k = 1; k <= R; k++)      // axis 0: -k
+                x = vadd(x, vmul(qv[R - k], (RT)sc.c[5 * R + k]));
+            const i64 idx = boff + (i64)i0 * plane + row + i2;
+            V r;
+            if (MODE == ST_APPLY) {
+                r = x;
+            } else if (MODE == ST_RESID) {
+                r = vsub(x, aux[idx]);
+            } else {
+                // (1 - w) * f + w * (s - x) / coef_0, c/bmgs/relax.c:60
+                const RT wr = (RT)w;
+                r = vadd(vmul(qv[R], (RT)1 - wr), vdiv_r(vmul(vsub(aux[idx], x), wr), (RT)sc.c[0]));
+            }
+            if (NORM) acc += norm2(r);
+            if (WRITE) out[idx] = r;
+#pragma unroll
+            for (int k = 0; k < 2 * R; k++) qv[k] = qv[k + 1];
+        }
+    }
+    if (NORM) {
+        __shared__ double red[256];
+        const double s = block_sum<256>(acc, red);
+        if (threadIdx.x == 0 && threadIdx.y == 0)
+            partial[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+    }
+}
+
+template <class V, int MODE, bool NORM, bool WRITE>
+__global__ __launch_bounds__(256) void k_generic(const V* __restrict__ in, V* __restrict__ out,
+                                                 const V* __restrict__ aux, Geom g, Phases ph,
+                                                 const double* __restrict__ coefs,
+                                                 const int* __restrict__ off, int nc, double w,
+                                                 i64 ng, double* __restrict__ partial)
+{
+    typedef typename VT<V>::R RT;
+    const i64 t = (i64)blockIdx.x * 256 + threadIdx.x;
+    double acc = 0.0;
+    if (t < ng) {
+        const int n1 = g.n[1], n2 = g.n[2];
+        const int i2 = (int)(t % n2);
+        const int i1 = (int)((t / n2) % n1);
+        const int i0 = (int)(t / ((i64)n1 * n2));
+        const i64 boff = (i64)blockIdx.y * ng;
+        const V* __restrict__ a = in + boff;
+        V x = vzero<V>();
+        for (int c = (MODE == ST_JACOBI ? 1 : 0); c < nc; c++)
+            x = vadd(x, vmul(ghost_fetch(a, i0 + off[3 * c], i1 + off[3 * c + 1],
+                                         i2 + off[3 * c + 2], g, ph),
+                             (RT)coefs[c]));
+        V r;
+        if (MODE == ST_APPLY) {
+            r = x;
+        } else if (MODE == ST_RESID) {
+            r = vsub(x, aux[boff + t]);
+        } else {
+            const RT wr = (RT)w;
+            r = vadd(vmul(a[t], (RT)1 - wr), vdiv_r(vmul(vsub(aux[boff + t], x), wr), (RT)coefs[0]));
+        }
+        if (NORM) acc = norm2(r);
+        if (WRITE) out[boff + t] = r;
+    }
+    if (NORM) {
+        __shared__ double red[256];
+        const double s = block_sum<256>(acc, red);
+        if (threadIdx.x == 0) partial[blockIdx.x + gridDim.x * blockIdx.y] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+namespace {
+
+struct StarShape {
+    dim3 grid, block;
+    int q, ntz;
+};
+
+StarShape star_shape(const Geom& g, int nin)
+{
+    StarShape s;
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    int bx = n2 > 32 ? 64 : n2 > 16 ? 32 : n2 > 8 ? 16 : 8;
+    int by = 256 / bx;
+    s.block = dim3(bx, by, 1);
+    s.ntz = (n2 + bx - 1) / bx;
+    const int nty = (n1 + by - 1) / by;
+    const long base = (long)s.ntz * nty * nin;
+    long chunks = (2048 + base - 1) / base;
+    if (chunks > n0) chunks = n0;
+    if (chunks < 1) chunks = 1;
+    int q = (int)((n0 + chunks - 1) / chunks);
+    if (q < 8) q = n0 < 8 ? n0 : 8;
+    s.q = q;
+    s.grid = dim3(s.ntz * nty, (n0 + q - 1) / q, nin);
+    return s;
+}
+
+template <class V, int R, int MODE>
+hipError_t star_launch(const Stencil& s, const void* in, void* out, const void* aux, const Geom& g,
+                       const Phases& ph, int nin, double w, double* partial, bool write,
+                       hipStream_t st)
+{
+    StarShape sh = star_shape(g, nin);
+    StarCoefs sc;
+    for (int c = 0; c < 6 * R + 1; c++) sc.c[c] = s.coefs[c];
+    const i64 ng = (i64)g.n[0] * g.n[1] * g.n[2];
+    const V* i = (const V*)in;
+    V* o = (V*)out;
+    const V* x = (const V*)aux;
+    if (partial) {
+        if (write)
+            hipLaunchKernelGGL((k_star<V, R, MODE, true, true>), sh.grid, sh.block, 0, st, i, o, x,
+                               g, ph, sc, w, sh.q, sh.ntz, ng, partial);
+        else
+            hipLaunchKernelGGL((k_star<V, R, MODE, true, false>), sh.grid, sh.block, 0, st, i, o,
+                               x, g, ph, sc, w, sh.q, sh.ntz, ng, partial);
+    } else {
+        hipLaunchKernelGGL((k_star<V, R, MODE, false, true>), sh.grid, sh.block, 0, st, i, o, x,
+                           g, ph, sc, w, sh.q, sh.ntz, ng, partial);
+    }
+    return hipGetLastError();
+}
+
+template <class V, int MODE>
+hipError_t star_dispatch_r(const Stencil& s, const void* in, void* out, const void* aux,
+                           const Geom& g, const Phases& ph, int nin, double w, double* partial,
+                           bool write, hipStream_t st)
+{
+    switch (s.star_r) {
+    case 1: return star_launch<V, 1, MODE>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+    case 2: return star_launch<V, 2, MODE>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+    case 3: return star_launch<V, 3, MODE>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+    case 4: return star_launch<V, 4, MODE>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+    case 5: return star_launch<V, 5, MODE>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+    case 6: return star_launch<V, 6, MODE>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <class V, int MODE>
+hipError_t generic_launch(const Stencil& s, const void* in, void* out, const void* aux,
+                          const Geom& g, const Phases& ph, int nin, double w, double* partial,
+                          bool write, hipStream_t st)
+{
+    const i64 ng = (i64)g.n[0] * g.n[1] * g.n[2];
+    dim3 grid((unsigned)((ng + 255) / 256), nin, 1);
+    const V* i = (const V*)in;
+    V* o = (V*)out;
+    const V* x = (const V*)aux;
+    if (partial) {
+        if (write)
+            hipLaunchKernelGGL((k_generic<V, MODE, true, true>), grid, dim3(256), 0, st, i, o, x,
+                               g, ph, s.d_coefs, s.d_off, s.nc, w, ng, partial);
+        else
+            hipLaunchKernelGGL((k_generic<V, MODE, true, false>), grid, dim3(256), 0, st, i, o,
+                               x, g, ph, s.d_coefs, s.d_off, s.nc, w, ng, partial);
+    } else {
+        hipLaunchKernelGGL((k_generic<V, MODE, false, true>), grid, dim3(256), 0, st, i, o, x, g,
+                           ph, s.d_coefs, s.d_off, s.nc, w, ng, partial);
+    }
+    return hipGetLastError();
+}
+
+template <class V>
+hipError_t dispatch_mode(const Stencil& s, int mode, const void* in, void* out, const void* aux,
+                         const Geom& g, const Phases& ph, int nin, double w, double* partial,
+                         bool write, hipStream_t st)
+{
+    if constexpr (VT<V>::cplx) {
+        // complex data: plain apply only (relax and residuals are real-only
+        // in the reference, c/operators.c:49-91)
+        if (mode != ST_APPLY || partial) return hipErrorInvalidValue;
+        if (s.star_r > 0) return star_dispatch_r<V, ST_APPLY>(s, in, out, aux, g, ph, nin, w, nullptr, true, st);
+        return generic_launch<V, ST_APPLY>(s, in, out, aux, g, ph, nin, w, nullptr, true, st);
+    }
+    if (s.star_r > 0) {
+        if (mode == ST_APPLY) return star_dispatch_r<V, ST_APPLY>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+        if (mode == ST_RESID) return star_dispatch_r<V, ST_RESID>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+        return star_dispatch_r<V, ST_JACOBI>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+    }
+    if (mode == ST_APPLY) return generic_launch<V, ST_APPLY>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+    if (mode == ST_RESID) return generic_launch<V, ST_RESID>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+    return generic_launch<V, ST_JACOBI>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+}
+
+}  // namespace
+
+int stencil_partials(const Stencil& s, const Geom& g, int nin)
+{
+    if (s.star_r > 0) {
+        StarShape sh = star_shape(g, nin);
+        return sh.grid.x * sh.grid.y * sh.grid.z;
+    }
+    const i64 ng = (i64)g.n[0] * g.n[1] * g.n[2];
+    return (int)((ng + 255) / 256) * nin;
+}
+
+hipError_t launch_stencil(const Stencil& s, int dtype, int mode, const void* in, void* out,
+                          const void* aux, const Geom& g, const Phases& ph, int nin, double w,
+                          double* partial, int* npartial, bool write, hipStream_t st)
+{
+    if (npartial) *npartial = stencil_partials(s, g, nin);
+    switch (dtype) {
+    case DT_F64: return dispatch_mode<double>(s, mode, in, out, aux, g, ph, nin, w, partial, write, st);
+    case DT_C128: return dispatch_mode<double2>(s, mode, in, out, aux, g, ph, nin, w, partial, write, st);
+    case DT_F32: return dispatch_mode<float>(s, mode, in, out, aux, g, ph, nin, w, partial, write, st);
+    case DT_C64: return dispatch_mode<float2>(s, mode, in, out, aux, g, ph, nin, w, partial, write, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace gpaw

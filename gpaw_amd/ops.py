@@ -1,0 +1,210 @@
+"""Thin Python handles over the C ABI (include/gpaw_amd.h).
+
+These are the drop-ins for the reference's compiled objects:
+  OperatorHandle     <- _gpaw.Operator    (c/operators.c:465-498)
+  TransformerHandle  <- _gpaw.Transformer (c/transformers.c:222-265)
+Their methods take the same arguments as the reference objects
+(apply(in, out, phases), relax(method, f, s, n, w), get_diagonal_element(),
+get_async_sizes()), accepting GPU tensors or numpy arrays.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _arrays, _lib
+from ._arrays import Arg
+
+
+def _phases(phases):
+    if phases is None:
+        return None, None
+    ph = np.ascontiguousarray(np.asarray(phases, np.complex128).reshape(3, 2))
+    flat = ph.view(np.float64).ravel().copy()
+    return flat, flat.ctypes.data_as(_lib._dp)
+
+
+def _nin(shape, size):
+    if len(shape) == 3:
+        return 1
+    if len(shape) == 4:
+        return shape[0]
+    # the reference reads dims[0] only for 4-D input; other ranks flatten
+    return int(np.prod(shape[:-3]))
+
+
+class OperatorHandle:
+    def __init__(self, coef_p, offset_p, size_c, mp, neighbor_cd, real,
+                 comm=None, cfd=True):
+        if comm is not None:
+            raise NotImplementedError('MPI communicators are not supported')
+        L = _lib.lib()
+        self._coefs, cp = _lib.doubles(coef_p)
+        self._offs, op = _lib.longs(offset_p)
+        self._size, sp = _lib.longs(size_c)
+        self._nb, nbp = _lib.longs(neighbor_cd)
+        h = ctypes.c_void_p()
+        _lib.check(L.gpaw_operator_new(cp, op, len(self._coefs), sp, int(mp),
+                                       nbp, int(bool(real)), int(bool(cfd)),
+                                       ctypes.byref(h)))
+        self.h = h
+        self.real = bool(real)
+        self.shape = tuple(int(x) for x in size_c)
+
+    def __del__(self):
+        h = getattr(self, 'h', None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().gpaw_operator_free(h)
+            except Exception:
+                pass
+            self.h = None
+
+    def apply(self, in_xg, out_xg, phases=None):
+        a = Arg(in_xg, name='input')
+        b = Arg(out_xg, write=True, name='output')
+        if a.shape != b.shape or a.shape[-3:] != self.shape:
+            raise ValueError('operator apply: shapes %s -> %s for grid %s' %
+                             (a.shape, b.shape, self.shape))
+        if a.code != b.code:
+            raise TypeError('operator apply: input and output dtypes differ')
+        _, pp = _phases(phases)
+        _lib.check(_lib.lib().gpaw_operator_apply(
+            self.h, a.ptr, b.ptr, _nin(a.shape, self.shape), a.code, pp,
+            _arrays.stream()))
+        b.finish()
+
+    def relax(self, relax_method, f_g, s_g, nrelax, w=1.0):
+        if w is None:   # PyArg_ParseTuple("iOOi|d") rejects None
+            raise TypeError('a float is required')
+        f = Arg(f_g, write=True, name='f')
+        s = Arg(s_g, name='s')
+        if f.shape != self.shape or s.shape != self.shape:
+            raise ValueError('relax: shapes %s, %s for grid %s' %
+                             (f.shape, s.shape, self.shape))
+        if f.code != _lib.GPAW_F64 or s.code != _lib.GPAW_F64:
+            raise TypeError('relax works on float64 arrays')
+        _lib.check(_lib.lib().gpaw_operator_relax(
+            self.h, int(relax_method), f.ptr, s.ptr, int(nrelax), float(w),
+            _arrays.stream()))
+        f.finish()
+
+    def residual(self, phi, rho, r=None, norm_out=None, scale=1.0):
+        """r = L phi - rho (and scale * sum r^2 into the 1-element tensor
+        norm_out), fused."""
+        p = Arg(phi, name='phi')
+        q = Arg(rho, name='rho')
+        rr = Arg(r, write=True, name='r') if r is not None else None
+        nn = Arg(norm_out, write=True, name='norm') if norm_out is not None \
+            else None
+        _lib.check(_lib.lib().gpaw_operator_residual(
+            self.h, p.ptr, q.ptr, rr.ptr if rr else None,
+            nn.ptr if nn else None, float(scale), _arrays.stream()))
+        if rr:
+            rr.finish()
+        if nn:
+            nn.finish()
+
+    def get_diagonal_element(self):
+        return _lib.lib().gpaw_operator_get_diagonal_element(self.h)
+
+    def get_async_sizes(self):
+        return (0, 1, 1)
+
+
+class TransformerHandle:
+    def __init__(self, size_in, size_out, k, pad_cd, npad_cd, skip_cd,
+                 neighbor_cd, real, comm=None, interpolate=False):
+        if comm is not None:
+            raise NotImplementedError('MPI communicators are not supported')
+        L = _lib.lib()
+        keep = []
+
+        def lp(a):
+            arr, p = _lib.longs(a)
+            keep.append(arr)
+            return p
+        h = ctypes.c_void_p()
+        _lib.check(L.gpaw_transformer_new(
+            lp(size_in), lp(size_out), int(k), lp(pad_cd), lp(npad_cd),
+            lp(skip_cd), lp(neighbor_cd), int(bool(real)), int(bool(interpolate)),
+            ctypes.byref(h)))
+        self.h = h
+        self.ngpin = tuple(int(x) for x in size_in)
+        self.ngpout = tuple(int(x) for x in size_out)
+
+    def __del__(self):
+        h = getattr(self, 'h', None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().gpaw_transformer_free(h)
+            except Exception:
+                pass
+            self.h = None
+
+    def apply(self, input, output, phases=None):
+        a = Arg(input, name='input')
+        b = Arg(output, write=True, name='output')
+        if a.shape[-3:] != self.ngpin or b.shape[-3:] != self.ngpout or \
+                a.shape[:-3] != b.shape[:-3]:
+            raise ValueError('transformer apply: shapes %s -> %s for %s -> %s'
+                             % (a.shape, b.shape, self.ngpin, self.ngpout))
+        if a.code != b.code:
+            raise TypeError('transformer apply: input and output dtypes differ')
+        _, pp = _phases(phases)
+        _lib.check(_lib.lib().gpaw_transformer_apply(
+            self.h, a.ptr, b.ptr, _nin(a.shape, self.ngpin), a.code, pp,
+            _arrays.stream()))
+        b.finish()
+
+    def get_async_sizes(self):
+        return (0, 1)
+
+
+def integrate(gd, a_xg, b_xg=None):
+    """Device version of GridDescriptor.integrate for float64 tensors."""
+    t = _arrays.torch()
+    if b_xg is not None:
+        raise NotImplementedError('device integrate of a product')
+    if a_xg.dtype != t.float64:
+        raise TypeError('device integrate needs float64')
+    a = Arg(a_xg)
+    lead = a.shape[:-3]
+    ng = int(np.prod(a.shape[-3:]))
+    out = t.empty(max(1, int(np.prod(lead))), dtype=t.float64, device='cuda')
+    L = _lib.lib()
+    st = _arrays.stream()
+    for i in range(out.numel()):
+        _lib.check(L.gpaw_sum(a.ptr + 8 * ng * i, ng, float(gd.dv),
+                              out.data_ptr() + 8 * i, st))
+    vals = out.cpu().numpy()
+    if not lead:
+        return float(vals[0])
+    return vals.reshape(lead)
+
+
+def sub_scalar(y, x, b):
+    """y = x - b on float64 device tensors."""
+    yy = Arg(y, write=True)
+    xx = Arg(x)
+    _lib.check(_lib.lib().gpaw_sub_scalar(yy.ptr, xx.ptr, float(b),
+                                          int(np.prod(yy.shape)),
+                                          _arrays.stream()))
+    yy.finish()
+
+
+def axpby(y, x, a, b):
+    """y = a*y + b*x on float64 device tensors."""
+    yy = Arg(y, write=True)
+    xx = Arg(x)
+    _lib.check(_lib.lib().gpaw_axpby(yy.ptr, xx.ptr, float(a), float(b),
+                                     int(np.prod(yy.shape)), _arrays.stream()))
+    yy.finish()
+
+
+def cast(y, x):
+    """Precision conversion (float64 <-> float32, complex128 <-> complex64)."""
+    yy = Arg(y, write=True)
+    xx = Arg(x)
+    _lib.check(_lib.lib().gpaw_cast(yy.ptr, yy.code, xx.ptr, xx.code,
+                                    int(np.prod(yy.shape)), _arrays.stream()))
+    yy.finish()

@@ -1,0 +1,250 @@
+"""Multigrid Poisson solver on the GPU (drop-in for gpaw/poisson.py:21-304).
+
+Same constructor, hierarchy (up to four coarsenings, nn-point operator on
+the finest grid, 7-point operators and K=2 transformers below it), smoothing
+counts (2/1 on level 0, 4/4 below, 8 on the coarsest), Jacobi weights
+(2/3, then 1), stopping rule (dv * sum r^2 <= eps, at most maxiter V-cycles)
+and charged-system branches.  The V-cycle itself runs in libgpaw_amd.so as
+one captured hipGraph per cycle (gpaw_poisson_solve_neutral); Python only
+prepares the neutral density and reads back one number per cycle.
+"""
+import ctypes
+from math import pi, sqrt
+
+import numpy as np
+
+from . import _arrays, _lib, ops
+from ._arrays import Arg
+from .fd_operators import Laplace, LaplaceA, LaplaceB
+from .transformers import Transformer
+from .utilities import Gaussian, madelung
+
+
+class PoissonConvergenceError(RuntimeError):
+    pass
+
+
+def _handle(obj):
+    t = getattr(obj, 'transformer', None)
+    if t is not None:
+        if hasattr(t, 'transformer'):       # debug wrapper
+            t = t.transformer
+        return t.h
+    return obj.operator.h
+
+
+class PoissonSolver:
+    def __init__(self, nn=3, relax='J', eps=2e-10):
+        self.relax = relax
+        self.nn = nn
+        self.eps = eps
+        self.charged_periodic_correction = None
+        self.maxiter = 1000
+        if relax == 'GS':
+            self.relax_method = 1
+        elif relax == 'J':
+            self.relax_method = 2
+        else:
+            raise NotImplementedError('Relaxation method %s' % relax)
+        self._plan = None
+
+    def get_method(self):
+        return ['Gauss-Seidel', 'Jacobi'][self.relax_method - 1]
+
+    def get_stencil(self):
+        return self.nn
+
+    def set_grid_descriptor(self, gd):
+        self.gd = gd
+        self.dv = gd.dv
+        scale = -0.25 / pi
+        if self.nn == 'M':
+            if not gd.orthogonal:
+                raise RuntimeError('Cannot use Mehrstellen stencil with '
+                                   'non orthogonal cell.')
+            self.operators = [LaplaceA(gd, -scale, allocate=False)]
+            self.B = LaplaceB(gd, allocate=False)
+        else:
+            self.operators = [Laplace(gd, scale, self.nn, allocate=False)]
+            self.B = None
+        self.interpolators = []
+        self.restrictors = []
+        level = 0
+        self.presmooths = [2]
+        self.postsmooths = [1]
+        self.weights = [2.0 / 3.0]
+        while level < 4:
+            try:
+                gd2 = gd.coarsen()
+            except ValueError:
+                break
+            self.operators.append(Laplace(gd2, scale, 1, allocate=False))
+            self.interpolators.append(Transformer(gd2, gd, allocate=False))
+            self.restrictors.append(Transformer(gd, gd2, allocate=False))
+            self.presmooths.append(4)
+            self.postsmooths.append(4)
+            self.weights.append(1.0)
+            level += 1
+            gd = gd2
+        self.levels = level
+
+    def initialize(self, load_gauss=False):
+        gd = self.gd
+        self.rhos = [gd.empty()]
+        self.phis = [None]
+        self.residuals = [gd.empty()]
+        for level in range(self.levels):
+            gd = gd.coarsen()
+            self.phis.append(gd.empty())
+            self.rhos.append(gd.empty())
+            self.residuals.append(gd.empty())
+        for obj in self.operators + self.interpolators + self.restrictors:
+            obj.allocate()
+        if self.B is not None:
+            self.B.allocate()
+        self.step = 0.66666666 / self.operators[0].get_diagonal_element()
+        self.presmooths[self.levels] = 8
+        self.postsmooths[self.levels] = 8
+        self._make_plan()
+        if load_gauss:
+            self.load_gauss()
+
+    def _make_plan(self):
+        L = self.levels
+        VP = ctypes.c_void_p
+        opa = (VP * (L + 1))(*[_handle(o) for o in self.operators])
+        ra = (VP * max(L, 1))(*[_handle(t) for t in self.restrictors])
+        ia = (VP * max(L, 1))(*[_handle(t) for t in self.interpolators])
+        pre = (ctypes.c_int * (L + 1))(*self.presmooths)
+        post = (ctypes.c_int * (L + 1))(*self.postsmooths)
+        w = (ctypes.c_double * (L + 1))(*self.weights)
+        h = VP()
+        _lib.check(_lib.lib().gpaw_poisson_new(
+            L, opa, ra, ia, pre, post, w, self.relax_method, float(self.dv),
+            ctypes.byref(h)))
+        self._plan = h
+
+    def __del__(self):
+        p = getattr(self, '_plan', None)
+        if p is not None and p.value:
+            try:
+                _lib.lib().gpaw_poisson_free(p)
+            except Exception:
+                pass
+            self._plan = None
+
+    def load_gauss(self):
+        if not hasattr(self, 'rho_gauss'):
+            gauss = Gaussian(self.gd)
+            self.rho_gauss = _arrays.torch().from_numpy(
+                gauss.get_gauss(0)).to('cuda')
+            self.phi_gauss = _arrays.torch().from_numpy(
+                gauss.get_gauss_pot(0)).to('cuda')
+
+    # ------------------------------------------------------------------ solve
+    def solve(self, phi, rho, charge=None, eps=None, maxcharge=1e-6,
+              zero_initial_phi=False):
+        if eps is None:
+            eps = self.eps
+        P = Arg(phi, write=True, name='phi')
+        R = Arg(rho, name='rho')
+        phi_d, rho_d = P.tensor, R.tensor
+        actual_charge = self.gd.integrate(rho_d)
+        background = (actual_charge / self.gd.dv /
+                      self.gd.get_size_of_global_array().prod())
+        if charge is None:
+            charge = actual_charge
+        if abs(charge) <= maxcharge:
+            rho_n = _arrays.torch().empty_like(rho_d)
+            ops.sub_scalar(rho_n, rho_d, background)
+            niter = self.solve_neutral(phi_d, rho_n, eps=eps)
+        elif abs(charge) > maxcharge and self.gd.pbc_c.all():
+            if self.charged_periodic_correction is None:
+                self.charged_periodic_correction = madelung(self.gd.cell_cv)
+            shift = charge * self.charged_periodic_correction
+            if zero_initial_phi:
+                phi_d.zero_()
+            else:
+                ops.sub_scalar(phi_d, phi_d, shift)
+            rho_n = _arrays.torch().empty_like(rho_d)
+            ops.sub_scalar(rho_n, rho_d, background)
+            niter = self.solve_neutral(phi_d, rho_n, eps=eps)
+            ops.sub_scalar(phi_d, phi_d, -shift)
+        elif abs(charge) > maxcharge and not self.gd.pbc_c.any():
+            self.load_gauss()
+            q = actual_charge / sqrt(4 * pi)
+            rho_n = rho_d.clone()
+            ops.axpby(rho_n, self.rho_gauss, 1.0, -q)
+            if zero_initial_phi:
+                phi_d.zero_()
+            else:
+                ops.axpby(phi_d, self.phi_gauss, 1.0, -q)
+            niter = self.solve_neutral(phi_d, rho_n, eps=eps)
+            ops.axpby(phi_d, self.phi_gauss, 1.0, q)
+        else:
+            raise NotImplementedError
+        P.finish()
+        return niter
+
+    def solve_neutral(self, phi, rho, eps=2e-10):
+        if self.relax_method != 2:
+            raise NotImplementedError('Gauss-Seidel relaxation does not run '
+                                      'on the GPU path')
+        if self.B is not None:
+            raise NotImplementedError('Mehrstellen (nn="M") right-hand side '
+                                      'is not on the GPU path yet')
+        P = Arg(phi, write=True, name='phi')
+        R = Arg(rho, name='rho')
+        self.phis[0] = P.tensor
+        niter = ctypes.c_int(0)
+        err = ctypes.c_double(0.0)
+        rc = _lib.lib().gpaw_poisson_solve_neutral(
+            self._plan, P.ptr, R.ptr, float(eps), int(self.maxiter),
+            ctypes.byref(niter), ctypes.byref(err), _arrays.stream())
+        self.error = err.value
+        if rc == _lib.GPAW_ENOCONV:
+            P.finish()
+            raise PoissonConvergenceError(
+                'Poisson solver did not converge in %d iterations!' %
+                self.maxiter)
+        _lib.check(rc)
+        if np.all(self.gd.pbc_c):
+            phi_ave = self.gd.integrate(P.tensor) / self.gd.dv
+            phi_ave /= np.prod(self.gd.get_size_of_global_array())
+            ops.sub_scalar(P.tensor, P.tensor, phi_ave)
+        P.finish()
+        return niter.value
+
+    def iterate2(self, step, level=0):
+        """One V-cycle on (phis[0], rhos[0]); returns dv * sum r^2."""
+        assert level == 0
+        errs = (ctypes.c_double * 1)()
+        P = Arg(self.phis[0], write=True)
+        R = Arg(self.rhos[0])
+        _lib.check(_lib.lib().gpaw_poisson_vcycles(
+            self._plan, P.ptr, R.ptr, 1, errs, _arrays.stream()))
+        P.finish()
+        return errs[0]
+
+    def vcycles(self, phi, rho, count):
+        """Exactly `count` V-cycles on an already neutral rho; returns the
+        error after each cycle (fixed-cycle comparisons)."""
+        errs = (ctypes.c_double * max(count, 1))()
+        P = Arg(phi, write=True)
+        R = Arg(rho)
+        _lib.check(_lib.lib().gpaw_poisson_vcycles(
+            self._plan, P.ptr, R.ptr, int(count), errs, _arrays.stream()))
+        P.finish()
+        return list(errs[:count])
+
+    def estimate_memory(self, mem):
+        gdbytes = self.gd.bytecount()
+        nbytes = 0
+        for level in range(self.levels + 1):
+            nbytes += 4 * gdbytes        # phi pair, rho, residual
+            gdbytes //= 8
+        mem.subnode('phi, rho, residual [%d levels]' % self.levels, nbytes)
+
+    def __repr__(self):
+        template = 'PoissonSolver(relax=\'%s\', nn=%s, eps=%e)'
+        return template % (self.relax, repr(self.nn), self.eps)

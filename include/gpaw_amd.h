@@ -1,0 +1,153 @@
+/*
+ * gpaw_amd.h -- C ABI of the MI355X (gfx950) finite-difference / multigrid
+ * library (libgpaw_amd.so).
+ *
+ * The entry points replace the reference's compiled Python-extension
+ * objects one for one; the citation on each names the reference interface
+ * it stands in for (paths relative to the reference tree).  All array
+ * arguments are DEVICE pointers (e.g. torch tensor data_ptr()), C-contiguous
+ * [band][i0][i1][i2].  `stream` is a hipStream_t (NULL = default stream);
+ * work is stream-ordered and the calls return without synchronising.
+ * Every function returns 0 on success or a negative GPAW_E* code, and
+ * gpaw_last_error() describes the last failure of the calling thread.
+ *
+ * Halo semantics are those of the reference's single-domain bc plan
+ * (c/bc.c:17-250): neighbours[a][d] == 0 means "periodic self-wrap"
+ * (the domain is its own neighbour), -1 means "zero boundary"; other ranks
+ * (MPI domain decomposition) are not supported on one GPU.
+ */
+#ifndef GPAW_AMD_H
+#define GPAW_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* storage types of grid functions */
+#define GPAW_F64 0   /* float64                       */
+#define GPAW_C128 1  /* complex128 (interleaved re,im) */
+#define GPAW_F32 2   /* float32                       */
+#define GPAW_C64 3   /* complex64                     */
+
+#define GPAW_OK 0
+#define GPAW_EINVAL (-1)       /* bad argument / inconsistent sizes      */
+#define GPAW_EUNSUPPORTED (-2) /* valid in the reference, not on a GPU   */
+#define GPAW_EDEVICE (-3)      /* HIP runtime error                      */
+#define GPAW_ENOMEM (-4)       /* device allocation failed               */
+#define GPAW_ENOCONV (-5)      /* Poisson solver did not converge        */
+
+typedef struct gpaw_operator gpaw_operator;
+typedef struct gpaw_transformer gpaw_transformer;
+typedef struct gpaw_poisson gpaw_poisson;
+
+const char* gpaw_last_error(void);
+
+/* ---------------------------------------------------------------- Operator
+ * Replaces _gpaw.Operator(coefs, offsets, size, range, neighbors, real,
+ * comm, cfd): NewOperatorObject, c/operators.c:465-498 (Python caller
+ * gpaw/fd_operators.py:57-74).  `offsets` are the reference's flat offsets
+ * into the padded buffer of extent size + 2*range
+ * (gpaw/fd_operators.py:39-42); comm must be None on one GPU and is not
+ * passed.  neighbors is long[3][2]. */
+int gpaw_operator_new(const double* coefs, const long* offsets, int ncoefs,
+                      const long size[3], int range, const long* neighbors,
+                      int real, int cfd, gpaw_operator** op);
+
+/* Operator.apply(in, out, phases): Operator_apply, c/operators.c:316-405.
+ * nin bands of `dtype`; phases = complex128 phase_cd[3][2] as 12 doubles
+ * in HOST memory (NULL: all phases 1). */
+int gpaw_operator_apply(gpaw_operator* op, const void* in, void* out,
+                        long nin, int dtype, const double* phases,
+                        void* stream);
+
+/* Operator.relax(method, f, s, nrelax, w): Operator_relax,
+ * c/operators.c:49-91 -> bmgs_relax, c/bmgs/relax.c:7-73.  float64 only;
+ * f is updated in place.  method 2 = weighted Jacobi, 1 = Gauss-Seidel. */
+int gpaw_operator_relax(gpaw_operator* op, int method, double* f,
+                        const double* s, int nrelax, double w, void* stream);
+
+/* Operator.get_diagonal_element(): c/operators.c:408-421. */
+double gpaw_operator_get_diagonal_element(const gpaw_operator* op);
+
+/* Fused residual r = L phi - rho of the multigrid loop
+ * (gpaw/poisson.py:257-258,271-275: operator.apply then numpy r -= rho).
+ * r may be NULL (not stored).  If norm2 is not NULL it receives, in DEVICE
+ * memory, scale * sum(r^2) (the reference's dv * dot(r, r)). */
+int gpaw_operator_residual(gpaw_operator* op, const double* phi,
+                           const double* rho, double* r, double* norm2,
+                           double scale, void* stream);
+
+void gpaw_operator_free(gpaw_operator* op);
+
+/* ------------------------------------------------------------ Transformer
+ * Replaces _gpaw.Transformer(size_in, size_out, k, paddings, npaddings,
+ * skip, neighbors, real, comm, interpolate): NewTransformerObject,
+ * c/transformers.c:222-265 (Python caller gpaw/transformers.py:64-79).
+ * paddings, npaddings, skip, neighbors are long[3][2]. */
+int gpaw_transformer_new(const long size_in[3], const long size_out[3],
+                         int k, const long* paddings, const long* npaddings,
+                         const long* skip, const long* neighbors, int real,
+                         int interpolate, gpaw_transformer** tr);
+
+/* Transformer.apply(in, out, phases): Transformer_apply,
+ * c/transformers.c:127-185. */
+int gpaw_transformer_apply(gpaw_transformer* tr, const void* in, void* out,
+                           long nin, int dtype, const double* phases,
+                           void* stream);
+
+void gpaw_transformer_free(gpaw_transformer* tr);
+
+/* ------------------------------------------------------ multigrid Poisson
+ * The V-cycle loop of PoissonSolver.solve_neutral / iterate2
+ * (gpaw/poisson.py:193-218,245-276), run on the device.  The reference
+ * drives it from Python through the Operator / Transformer objects above;
+ * here one plan owns the level arrays and replays the V-cycle.
+ * ops[0..levels], restrictors/interpolators[0..levels-1] are borrowed and
+ * must outlive the plan; presmooths/postsmooths/weights have levels+1
+ * entries (gpaw/poisson.py:64-89,111-113). */
+int gpaw_poisson_new(int levels, gpaw_operator* const* ops,
+                     gpaw_transformer* const* restrictors,
+                     gpaw_transformer* const* interpolators,
+                     const int* presmooths, const int* postsmooths,
+                     const double* weights, int relax_method, double dv,
+                     gpaw_poisson** ps);
+
+/* solve_neutral(phi, rho, eps): V-cycles until dv*sum(r^2) <= eps or
+ * maxiter cycles (niter counts as the reference does).  phi is updated in
+ * place (device), rho is read.  Returns GPAW_ENOCONV when maxiter is
+ * reached, with *niter and *error set. */
+int gpaw_poisson_solve_neutral(gpaw_poisson* ps, double* phi,
+                               const double* rho, double eps, int maxiter,
+                               int* niter, double* error, void* stream);
+
+/* Exactly ncycles V-cycles; errors[i] (host) gets dv*sum(r^2) after
+ * cycle i.  Used for fixed-cycle comparisons. */
+int gpaw_poisson_vcycles(gpaw_poisson* ps, double* phi, const double* rho,
+                         int ncycles, double* errors, void* stream);
+
+void gpaw_poisson_free(gpaw_poisson* ps);
+
+/* -------------------------------------------------------------- helpers */
+/* Deterministic FP64 sum of n doubles times scale into *result (device):
+ * gd.integrate / comm.sum of the reference (gpaw/grid_descriptor.py:197-208). */
+int gpaw_sum(const double* a, long n, double scale, double* result,
+             void* stream);
+
+/* y = a*y + b*x over n doubles (numpy's in-place updates around the
+ * solver, e.g. phi += q*phi_gauss, gpaw/poisson.py:180,186). */
+int gpaw_axpby(double* y, const double* x, double a, double b, long n,
+               void* stream);
+
+/* y = x - b over n doubles (rho - background, phi -= phi_ave:
+ * gpaw/poisson.py:138,216). */
+int gpaw_sub_scalar(double* y, const double* x, double b, long n,
+                    void* stream);
+
+/* Precision conversion between GPAW_F64 <-> GPAW_F32 or GPAW_C128 <->
+ * GPAW_C64 (n elements). */
+int gpaw_cast(void* y, int dty, const void* x, int dtx, long n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPAW_AMD_H */

@@ -1,0 +1,137 @@
+"""Parity of the device multigrid Poisson solver with the oracle's restatement
+of gpaw/poisson.py driving the oracle kernels.
+
+solve_neutral on the same neutral density must give the same number of
+V-cycles and a bit-identical potential (the V-cycle arithmetic is exact;
+only the convergence norm is summed in a different order, which could flip
+niter only if dv*sum(r^2) landed within rounding of eps).  The full solve()
+adds the charge integral, summed in a different order than numpy, so the
+potential is compared to 1e-13 relative max-norm.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BOHR = 0.52917720859
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip('no GPU')
+    return torch
+
+
+def dev(a):
+    return _torch().from_numpy(np.ascontiguousarray(a)).to('cuda')
+
+
+def rel_maxnorm(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def _density(og, seed, count, wmin, wmax, margin):
+    rng = np.random.default_rng(seed)
+    L = np.diag(og.cell_cv)
+    centers = [rng.uniform(margin, L - margin) for _ in range(count)]
+    widths = rng.uniform(wmin, wmax, count)
+    charges = [1.0 if i % 2 == 0 else -1.0 for i in range(count)]
+    return O.gaussian_charges(og, centers, widths, charges)
+
+
+def _pair(N, h, pbc, nn=3):
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.poisson import PoissonSolver
+    cell = (N * h,) * 3
+    og = O.Grid((N,) * 3, cell, pbc)
+    gd = GridDescriptor((N,) * 3, cell, pbc)
+    ops = O.PoissonSolver(nn=nn, relax='J')
+    ops.set_grid_descriptor(og)
+    ops.initialize()
+    gps = PoissonSolver(nn=nn, relax='J')
+    gps.set_grid_descriptor(gd)
+    gps.initialize()
+    return og, gd, ops, gps
+
+
+@pytest.mark.parametrize('N,pbc', [(32, False), (32, True), (48, False)])
+def test_solve_neutral_exact(N, pbc):
+    print('case solve_neutral', N, pbc, flush=True)
+    h = 0.2 / BOHR
+    og, gd, ops, gps = _pair(N, h, pbc)
+    rho = _density(og, N, 4, 0.5 / BOHR, 1.5 / BOHR, 2.0 / BOHR)
+    rho -= og.integrate(rho) / og.dv / og.size_of_global_array().prod()
+    phi_ref = og.zeros()
+    n_ref = ops.solve_neutral(phi_ref, rho.copy(), eps=2e-10)
+    phi = dev(np.zeros(tuple(og.n_c)))
+    n = gps.solve_neutral(phi, dev(rho), eps=2e-10)
+    assert n == n_ref
+    got = phi.cpu().numpy()
+    if pbc:   # mean removal sums in a different order than numpy
+        assert rel_maxnorm(got, phi_ref) < 1e-13
+    else:
+        assert np.array_equal(got, phi_ref)
+
+
+def test_c2_config_solve():
+    """96^3 non-periodic (95^3 arrays), h = 0.2 A, nn=3 Jacobi, eps=2e-10,
+    eight seeded neutral Gaussians: the plain FP64 configuration."""
+    print('case C2 96^3', flush=True)
+    h = 0.2 / BOHR
+    og, gd, ops, gps = _pair(96, h, False)
+    rho = _density(og, 1, 8, 0.5 / BOHR, 1.5 / BOHR, 2.0 / BOHR)
+    phi_ref = og.zeros()
+    n_ref = ops.solve(phi_ref, rho.copy())
+    phi = dev(np.zeros(tuple(og.n_c)))
+    n = gps.solve(phi, dev(rho))
+    assert n == n_ref
+    assert rel_maxnorm(phi.cpu().numpy(), phi_ref) < 1e-13
+
+
+def test_charged_zero_boundary():
+    """The Gaussian-monopole branch (gpaw/poisson.py:163-188)."""
+    print('case charged zero boundary', flush=True)
+    og, gd, ops, gps = _pair(32, 0.3, False)
+    rho = _density(og, 5, 3, 0.8, 1.2, 2.5)
+    phi_ref = og.zeros()
+    n_ref = ops.solve(phi_ref, rho.copy())
+    phi = dev(np.zeros(tuple(og.n_c)))
+    n = gps.solve(phi, dev(rho))
+    assert n == n_ref
+    assert rel_maxnorm(phi.cpu().numpy(), phi_ref) < 1e-12
+
+
+def test_charged_periodic():
+    """The Madelung-shift branch (gpaw/poisson.py:140-161)."""
+    print('case charged periodic', flush=True)
+    og, gd, ops, gps = _pair(16, 0.35, True, nn=2)
+    rho = _density(og, 9, 3, 0.6, 0.9, 1.0)
+    phi_ref = og.zeros()
+    n_ref = ops.solve(phi_ref, rho.copy())
+    phi = dev(np.zeros(tuple(og.n_c)))
+    n = gps.solve(phi, dev(rho))
+    assert n == n_ref
+    assert rel_maxnorm(phi.cpu().numpy(), phi_ref) < 1e-12
+
+
+def test_nonconvergence_raises():
+    print('case maxiter', flush=True)
+    from gpaw_amd.poisson import PoissonConvergenceError
+    og, gd, ops, gps = _pair(16, 0.3, False)
+    gps.maxiter = 3
+    rho = _density(og, 2, 2, 0.8, 1.0, 1.2)
+    with pytest.raises(PoissonConvergenceError):
+        gps.solve(dev(np.zeros(tuple(og.n_c))), dev(rho), eps=1e-30)
+
+
+def test_numpy_phi_is_updated():
+    og, gd, ops, gps = _pair(16, 0.3, False)
+    rho = _density(og, 4, 2, 0.8, 1.0, 1.2)
+    phi = np.zeros(tuple(og.n_c))
+    gps.solve(phi, rho)
+    ref = og.zeros()
+    ops.solve(ref, rho.copy())
+    assert rel_maxnorm(phi, ref) < 1e-13
