@@ -97,6 +97,17 @@ def jacobi_roofline(torch, np, ps, rho_d):
             'note': '95^3 FP64 working set (20.6 MB) is cache resident'}
 
 
+def copy_bandwidth(torch):
+    """Device-to-device copy of 2 GiB (read + write 4 GiB) timed with events:
+    the attainable HBM bandwidth on this box, reported beside the peak."""
+    n = 1 << 28
+    a = torch.empty(n, dtype=torch.float64, device='cuda').fill_(1.0)
+    b = torch.empty_like(a)
+    t = event_time(torch, lambda: b.copy_(a), 10)
+    del a, b
+    return round(2 * 8 * n / t / 1e9, 1)
+
+
 def fd_laplace_c3(torch, np):
     """C3 Laplace: complex128 nn=3 on 256 bands of 64^3, periodic, phases."""
     from gpaw_amd.fd_operators import Laplace
@@ -224,6 +235,7 @@ def main():
                    'parallelism': 'single GPU'},
     }
     out['roofline'] = jacobi_roofline(torch, np, ps, rho_n)
+    out['roofline']['copy_GBps_measured'] = copy_bandwidth(torch)
     if not args.no_extra:
         out['fd_laplace'] = fd_laplace_c3(torch, np)
     if not args.no_cpu:

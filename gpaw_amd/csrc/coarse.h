@@ -1,0 +1,35 @@
+// Plan of the fused coarse-level V-cycle kernel (device memory, uploaded
+// once per Poisson plan).
+#pragma once
+#include "transfer_common.h"
+
+namespace gpaw {
+
+#define GPAW_MAX_LEVELS 8
+
+struct CoarseLevel {
+    Geom g;
+    int off;              // LDS mode: offset (doubles) of this level's 4 arrays
+    double c[7];          // 7-point star: centre, z+, z-, y+, y-, x+, x-
+    double w;             // Jacobi weight
+    int pre, post;        // smoothing counts
+    double* phiA;         // phi ping-pong pair (A holds phi at cycle start)
+    double* phiB;
+    double* rho;
+    double* res;
+    TrArgs rst;           // restriction from this level (input = this level)
+    TrArgs itp;           // interpolation into this level (input = next level)
+};
+
+struct CoarsePlan {
+    int nlev;
+    int lds;              // 1: all level arrays live in LDS (lds_doubles total)
+    int lds_doubles;
+    CoarseLevel lev[GPAW_MAX_LEVELS];
+};
+
+TrArgs make_trargs(const Transfer& tr);
+hipError_t launch_coarse_vcycle(const CoarsePlan* d_plan, int lds_bytes, hipStream_t st);
+hipError_t coarse_set_lds_limit(int bytes);
+
+}  // namespace gpaw

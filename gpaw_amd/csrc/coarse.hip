@@ -1,0 +1,178 @@
+// Fused coarse-level V-cycle (gfx950): the smallest levels of the multigrid
+// hierarchy are smoothed, restricted and corrected by ONE workgroup of 1024
+// threads in one launch, with a workgroup barrier between passes instead of
+// a kernel boundary.  On these grids (11^3 and below for a 95^3 problem) a
+// separate launch per pass costs more than the pass itself.  When the level
+// arrays fit, they live in LDS: only the first level's right-hand side is
+// read from HBM and its solution written back.
+//
+// The passes and their arithmetic are those of iterate2
+// (gpaw/poisson.py:245-270) on 7-point operators (Laplace nn=1) with K=2
+// transformers, exactly as the standalone kernels evaluate them:
+//   Jacobi    x = sum_{c>=1} coef_c phi[g+off_c] (order z+, z-, y+, y-, x+, x-),
+//             phi' = (1-w) phi + w (rho - x) / coef_0   (c/bmgs/relax.c:47-71)
+//   residual  r = sum_c coef_c phi[g+off_c] - rho       (c/bmgs/fd.c:41-44)
+//   restrict / interpolate   transfer_common.h
+// so the fused path is bit-identical to launching the passes one by one.
+#include "kernels.h"
+#include "coarse.h"
+#include "transfer_common.h"
+
+namespace gpaw {
+
+namespace {
+
+__device__ __forceinline__ double star7(const double* __restrict__ a, int i0, int i1, int i2,
+                                        const Geom& g, const Phases& ph, const double* c,
+                                        bool centre)
+{
+    double x = 0.0;
+    if (centre) x = x + a[((i64)i0 * g.n[1] + i1) * g.n[2] + i2] * c[0];
+    x = x + ghost_fetch(a, i0, i1, i2 + 1, g, ph) * c[1];
+    x = x + ghost_fetch(a, i0, i1, i2 - 1, g, ph) * c[2];
+    x = x + ghost_fetch(a, i0, i1 + 1, i2, g, ph) * c[3];
+    x = x + ghost_fetch(a, i0, i1 - 1, i2, g, ph) * c[4];
+    x = x + ghost_fetch(a, i0 + 1, i1, i2, g, ph) * c[5];
+    x = x + ghost_fetch(a, i0 - 1, i1, i2, g, ph) * c[6];
+    return x;
+}
+
+__device__ __forceinline__ void split(int idx, const Geom& g, int& i0, int& i1, int& i2)
+{
+    i2 = idx % g.n[2];
+    const int t = idx / g.n[2];
+    i1 = t % g.n[1];
+    i0 = t / g.n[1];
+}
+
+// level arrays: phi A/B, rho, residual
+enum { PA = 0, PB = 1, RHO = 2, RES = 3 };
+
+__device__ void jacobi_sweeps(const CoarseLevel& L, double* const* ptr, int k, int& cur,
+                              const Phases& ph)
+{
+    const int n = L.g.n[0] * L.g.n[1] * L.g.n[2];
+    const double* __restrict__ rho = ptr[RHO];
+    for (int it = 0; it < k; it++) {
+        const double* __restrict__ src = ptr[cur];
+        double* __restrict__ dst = ptr[cur ^ 1];
+        for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+            int i0, i1, i2;
+            split(idx, L.g, i0, i1, i2);
+            const double x = star7(src, i0, i1, i2, L.g, ph, L.c, false);
+            dst[idx] = (1.0 - L.w) * src[idx] + L.w * (rho[idx] - x) / L.c[0];
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __restrict__ P)
+{
+    extern __shared__ double smem[];
+    __shared__ double* ptr[GPAW_MAX_LEVELS][4];
+    Phases ph;
+    for (int a = 0; a < 3; a++)
+        for (int d = 0; d < 2; d++) {
+            ph.re[a][d] = 1.0;
+            ph.im[a][d] = 0.0;
+        }
+    const int nl = P->nlev;
+    const bool lds = P->lds;
+    if (threadIdx.x < nl) {   // level arrays: global, or four LDS blocks per level
+        const CoarseLevel& L = P->lev[threadIdx.x];
+        if (lds) {
+            const int n = L.g.n[0] * L.g.n[1] * L.g.n[2];
+            double* base = smem + L.off;
+            for (int k = 0; k < 4; k++) ptr[threadIdx.x][k] = base + k * n;
+        } else {
+            ptr[threadIdx.x][PA] = L.phiA;
+            ptr[threadIdx.x][PB] = L.phiB;
+            ptr[threadIdx.x][RHO] = L.rho;
+            ptr[threadIdx.x][RES] = L.res;
+        }
+    }
+    __syncthreads();
+    if (lds) {   // right-hand side of the first coarse level, phi = 0
+        const CoarseLevel& G = P->lev[0];
+        const int n = G.g.n[0] * G.g.n[1] * G.g.n[2];
+        for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+            ptr[0][RHO][idx] = G.rho[idx];
+            ptr[0][PA][idx] = 0.0;
+        }
+        __syncthreads();
+    }
+    // down: pre-smooth, residual, restrict (phi of the next level zeroed)
+    for (int l = 0; l < nl - 1; l++) {
+        const CoarseLevel& L = P->lev[l];
+        int cur = PA;
+        jacobi_sweeps(L, ptr[l], L.pre, cur, ph);
+        const int n = L.g.n[0] * L.g.n[1] * L.g.n[2];
+        const double* __restrict__ phi = ptr[l][cur];
+        const double* __restrict__ rho = ptr[l][RHO];
+        double* __restrict__ res = ptr[l][RES];
+        for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+            int i0, i1, i2;
+            split(idx, L.g, i0, i1, i2);
+            res[idx] = star7(phi, i0, i1, i2, L.g, ph, L.c, true) - rho[idx];
+        }
+        __syncthreads();
+        const CoarseLevel& N = P->lev[l + 1];
+        const int m = N.g.n[0] * N.g.n[1] * N.g.n[2];
+        double* __restrict__ nrho = ptr[l + 1][RHO];
+        double* __restrict__ nphi = ptr[l + 1][PA];
+        for (int idx = threadIdx.x; idx < m; idx += blockDim.x) {
+            int o0, o1, o2;
+            split(idx, N.g, o0, o1, o2);
+            nrho[idx] = restrict_point<double, 2>(res, o0, o1, o2, L.rst, ph);
+            nphi[idx] = 0.0;
+        }
+        __syncthreads();
+    }
+    {   // bottom: post-smooth only
+        const CoarseLevel& L = P->lev[nl - 1];
+        int cur = PA;
+        jacobi_sweeps(L, ptr[nl - 1], L.post, cur, ph);
+    }
+    // up: correct from the coarser level, post-smooth
+    for (int l = nl - 2; l >= 0; l--) {
+        const CoarseLevel& L = P->lev[l];
+        const CoarseLevel& N = P->lev[l + 1];
+        int cur = (L.pre & 1) ? PB : PA;
+        const int kc = (l + 1 < nl - 1 ? N.pre : 0) + N.post;
+        const double* __restrict__ coarse = ptr[l + 1][(kc & 1) ? PB : PA];
+        double* __restrict__ phi = ptr[l][cur];
+        const int n = L.g.n[0] * L.g.n[1] * L.g.n[2];
+        for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+            int i0, i1, i2;
+            split(idx, L.g, i0, i1, i2);
+            phi[idx] = phi[idx] - interp_point<double, 2>(coarse, i0, i1, i2, L.itp, ph);
+        }
+        __syncthreads();
+        jacobi_sweeps(L, ptr[l], L.post, cur, ph);
+    }
+    if (lds) {   // hand the first level's solution back where the caller expects it
+        const CoarseLevel& G = P->lev[0];
+        const int k = (nl > 1 ? G.pre : 0) + G.post;
+        const double* src = ptr[0][(k & 1) ? PB : PA];
+        double* dst = (k & 1) ? G.phiB : G.phiA;
+        const int n = G.g.n[0] * G.g.n[1] * G.g.n[2];
+        for (int idx = threadIdx.x; idx < n; idx += blockDim.x) dst[idx] = src[idx];
+    }
+}
+
+hipError_t coarse_set_lds_limit(int bytes)
+{
+    return hipFuncSetAttribute((const void*)k_coarse_vcycle,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+hipError_t launch_coarse_vcycle(const CoarsePlan* d_plan, int lds_bytes, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_coarse_vcycle, dim3(1), dim3(1024), lds_bytes, st, d_plan);
+    return hipGetLastError();
+}
+
+}  // namespace gpaw

@@ -18,9 +18,11 @@
 // starting buffer) are captured and alternated.  The host reads the 8-byte
 // error after each cycle to apply the reference's stopping rule exactly.
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "capi_internal.h"
+#include "coarse.h"
 
 using namespace gpaw;
 
@@ -40,7 +42,12 @@ struct gpaw_poisson {
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     int use_graph = 1;
     int swaps = 0;             // 1 if level 0 does an odd number of sweeps per cycle
+    int lc = -1;               // first level run by the fused coarse kernel (-1: none)
+    CoarsePlan* d_coarse = nullptr;
+    int coarse_lds = 0;        // dynamic LDS bytes of the fused kernel
 };
+
+static int setup_coarse(gpaw_poisson* ps);
 
 namespace {
 
@@ -71,6 +78,7 @@ hipError_t emit_level(gpaw_poisson* ps, int l, double*& cur0, double*& alt0, hip
 {
     Phases ph;
     gpaw_phases(nullptr, ph);
+    if (l == ps->lc) return launch_coarse_vcycle(ps->d_coarse, ps->coarse_lds, st);
     double* cur = l == 0 ? cur0 : ps->phiA[l];
     double* alt = l == 0 ? alt0 : ps->phiB[l];
     hipError_t e = hipSuccess;
@@ -220,6 +228,7 @@ extern "C" int gpaw_poisson_new(int levels, gpaw_operator* const* ops,
         ps->res.push_back(x);
     }
     if (!rc) rc = gpaw_op_ensure_partials(ps->ops[0]);
+    if (!rc) rc = setup_coarse(ps);
     if (!rc) rc = alloc(&ps->d_err, 1);
     if (!rc) rc = gpaw_check(hipHostMalloc((void**)&ps->h_err, sizeof(double)), "poisson: pinned");
     if (!rc) rc = gpaw_check(hipStreamCreateWithFlags(&ps->cap, hipStreamNonBlocking),
@@ -229,6 +238,66 @@ extern "C" int gpaw_poisson_new(int levels, gpaw_operator* const* ops,
         return rc;
     }
     *out = ps;
+    return GPAW_OK;
+}
+
+// Levels from lc down are run by one fused kernel when they are small and
+// all 7-point stars with K=2 transfers (the Poisson hierarchy below level 0).
+static int setup_coarse(gpaw_poisson* ps)
+{
+    const char* env = getenv("GPAW_AMD_COARSE_MAX");
+    const long long limit = env ? atoll(env) : 2000;
+    int lc = -1;
+    for (int l = 1; l <= ps->L; l++)
+        if (ps->npts[l] <= limit) {
+            lc = l;
+            break;
+        }
+    if (lc < 0 || ps->L - lc + 1 > GPAW_MAX_LEVELS) return GPAW_OK;
+    CoarsePlan P;
+    memset(&P, 0, sizeof P);
+    P.nlev = ps->L - lc + 1;
+    for (int l = lc; l <= ps->L; l++) {
+        const gpaw_operator* op = ps->ops[l];
+        if (op->st.star_r != 1) return GPAW_OK;
+        if (l < ps->L && (ps->rst[l]->t.K != 2 || ps->itp[l]->t.K != 2)) return GPAW_OK;
+        CoarseLevel& C = P.lev[l - lc];
+        C.g = op->g;
+        for (int c = 0; c < 7; c++) C.c[c] = op->st.coefs[c];
+        C.w = ps->w[l];
+        C.pre = l < ps->L ? ps->pre[l] : 0;
+        C.post = ps->post[l];
+        C.phiA = ps->phiA[l];
+        C.phiB = ps->phiB[l];
+        C.rho = ps->rho[l];
+        C.res = ps->res[l];
+        if (l < ps->L) {
+            C.rst = make_trargs(ps->rst[l]->t);
+            C.itp = make_trargs(ps->itp[l]->t);
+        }
+    }
+    // keep the coarse arrays in LDS when they fit (4 arrays per level)
+    long long nd = 0;
+    for (int l = lc; l <= ps->L; l++) {
+        P.lev[l - lc].off = (int)nd;
+        nd += 4 * ps->npts[l];
+    }
+    if (nd * 8 <= 96 * 1024) {
+        P.lds = 1;
+        P.lds_doubles = (int)nd;
+        ps->coarse_lds = (int)(nd * 8);
+        if (ps->coarse_lds > 64 * 1024) {
+            hipError_t e = coarse_set_lds_limit(ps->coarse_lds);
+            if (e != hipSuccess) {
+                P.lds = 0;
+                ps->coarse_lds = 0;
+            }
+        }
+    }
+    hipError_t e = hipMalloc((void**)&ps->d_coarse, sizeof(CoarsePlan));
+    if (e == hipSuccess) e = hipMemcpy(ps->d_coarse, &P, sizeof P, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return gpaw_check(e, "poisson: coarse plan");
+    ps->lc = lc;
     return GPAW_OK;
 }
 
@@ -299,6 +368,7 @@ extern "C" void gpaw_poisson_free(gpaw_poisson* ps)
         for (double* p : *v)
             if (p) hipFree(p);
     if (ps->d_err) hipFree(ps->d_err);
+    if (ps->d_coarse) hipFree(ps->d_coarse);
     if (ps->h_err) hipHostFree(ps->h_err);
     if (ps->cap) hipStreamDestroy(ps->cap);
     delete ps;

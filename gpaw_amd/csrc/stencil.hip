@@ -17,9 +17,21 @@
 //    one thread per point, coefficients read through the scalar cache.
 // Both fuse the residual (apply - rho) and an FP64 sum of |result|^2 per
 // block (deterministic tree order) for the multigrid convergence test.
+#include <cstdlib>
+
 #include "kernels.h"
+#include "stencil_tile.h"
 
 namespace gpaw {
+
+hipError_t tile_launch_f64(const Stencil&, int, const void*, void*, const void*, const Geom&,
+                          const Phases&, double, int, const TileShape&, double*, bool, hipStream_t);
+hipError_t tile_launch_c128(const Stencil&, int, const void*, void*, const void*, const Geom&,
+                           const Phases&, double, int, const TileShape&, double*, bool, hipStream_t);
+hipError_t tile_launch_f32(const Stencil&, int, const void*, void*, const void*, const Geom&,
+                          const Phases&, double, int, const TileShape&, double*, bool, hipStream_t);
+hipError_t tile_launch_c64(const Stencil&, int, const void*, void*, const void*, const Geom&,
+                          const Phases&, double, int, const TileShape&, double*, bool, hipStream_t);
 
 struct StarCoefs {
     double c[6 * GPAW_MAXR + 1];
@@ -190,19 +202,27 @@ StarShape star_shape(const Geom& g, int nin)
 {
     StarShape s;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
-    int bx = n2 > 32 ? 64 : n2 > 16 ? 32 : n2 > 8 ? 16 : 8;
+    // widest row segment that wastes at most ~10% of the lanes
+    int bx = 8;
+    for (int c : {64, 32, 16}) {
+        const int tiles = (n2 + c - 1) / c;
+        if (tiles * c - n2 <= (tiles * c) / 10 || c == 16) {
+            bx = c;
+            break;
+        }
+    }
+    if (n2 <= 8) bx = 8;
     int by = 256 / bx;
     s.block = dim3(bx, by, 1);
     s.ntz = (n2 + bx - 1) / bx;
     const int nty = (n1 + by - 1) / by;
     const long base = (long)s.ntz * nty * nin;
+    // enough blocks to fill 256 CUs several times; short i0 runs on small grids
     long chunks = (2048 + base - 1) / base;
     if (chunks > n0) chunks = n0;
     if (chunks < 1) chunks = 1;
-    int q = (int)((n0 + chunks - 1) / chunks);
-    if (q < 8) q = n0 < 8 ? n0 : 8;
-    s.q = q;
-    s.grid = dim3(s.ntz * nty, (n0 + q - 1) / q, nin);
+    s.q = (int)((n0 + chunks - 1) / chunks);
+    s.grid = dim3(s.ntz * nty, (n0 + s.q - 1) / s.q, nin);
     return s;
 }
 
@@ -296,11 +316,70 @@ hipError_t dispatch_mode(const Stencil& s, int mode, const void* in, void* out, 
 
 }  // namespace
 
+// Kernel choice.  GPAW_AMD_STENCIL=tile|queue|generic forces one (timing
+// experiments); the default is the LDS tile kernel on large grids, the
+// register-queue kernel on other star stencils, the generic one otherwise.
+enum { K_GENERIC = 0, K_QUEUE = 1, K_TILE = 2 };
+
+static int forced_kind()
+{
+    static int f = -2;
+    if (f == -2) {
+        const char* e = getenv("GPAW_AMD_STENCIL");
+        f = !e ? -1 : e[0] == 't' ? K_TILE : e[0] == 'q' ? K_QUEUE : e[0] == 'g' ? K_GENERIC : -1;
+    }
+    return f;
+}
+
+bool tile_shape(const Geom& g, int R, int nin, int& TZ, TileShape& sh)
+{
+    if (R < 1 || R > 4) return false;
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    TZ = 64;
+    {
+        const int w64 = ((n2 + 63) / 64) * 64 - n2, w32 = ((n2 + 31) / 32) * 32 - n2;
+        if (w32 * 100 < w64 * 100 - 5 * n2) TZ = 32;
+    }
+    const int rows = R <= 3 ? 4 : 2;
+    const int TY = (256 / TZ) * rows;
+    sh.ntz = (n2 + TZ - 1) / TZ;
+    sh.nty = (n1 + TY - 1) / TY;
+    // every coordinate a tile touches (tile + halo) must lie within one
+    // period on either side of the grid: ghost indices wrap only once
+    if (sh.ntz * TZ + R > 2 * n2 || sh.nty * TY + R > 2 * n1 || n0 < R || n1 < R || n2 < R)
+        return false;
+    const long nt = (long)sh.ntz * sh.nty * nin;
+    long nch = (2048 + nt - 1) / nt;
+    const long maxch = n0 / 16 > 1 ? n0 / 16 : 1;
+    if (nch > maxch) nch = maxch;
+    if (nch < 1) nch = 1;
+    sh.q = (int)((n0 + nch - 1) / nch);
+    sh.nchunk = (n0 + sh.q - 1) / sh.q;
+    const long nb = nt * sh.nchunk;
+    if (nb < 256 || nb > 0x7fffffffL) return false;
+    sh.nblocks = (int)nb;
+    return true;
+}
+
+static int stencil_kind(const Stencil& s, const Geom& g, int nin, int& TZ, TileShape& sh)
+{
+    const int f = forced_kind();
+    const bool tile_ok = s.star_r > 0 && tile_shape(g, s.star_r, nin, TZ, sh);
+    if (f == K_GENERIC && s.d_coefs) return K_GENERIC;
+    if (f == K_QUEUE && s.star_r > 0) return K_QUEUE;
+    if (tile_ok && f != K_QUEUE) return K_TILE;
+    return s.star_r > 0 ? K_QUEUE : K_GENERIC;
+}
+
 int stencil_partials(const Stencil& s, const Geom& g, int nin)
 {
-    if (s.star_r > 0) {
-        StarShape sh = star_shape(g, nin);
-        return sh.grid.x * sh.grid.y * sh.grid.z;
+    int TZ;
+    TileShape sh;
+    const int kind = stencil_kind(s, g, nin, TZ, sh);
+    if (kind == K_TILE) return sh.nblocks;
+    if (kind == K_QUEUE) {
+        StarShape q = star_shape(g, nin);
+        return q.grid.x * q.grid.y * q.grid.z;
     }
     const i64 ng = (i64)g.n[0] * g.n[1] * g.n[2];
     return (int)((ng + 255) / 256) * nin;
@@ -310,12 +389,26 @@ hipError_t launch_stencil(const Stencil& s, int dtype, int mode, const void* in,
                           const void* aux, const Geom& g, const Phases& ph, int nin, double w,
                           double* partial, int* npartial, bool write, hipStream_t st)
 {
+    int TZ = 64;
+    TileShape sh;
+    const int kind = stencil_kind(s, g, nin, TZ, sh);
     if (npartial) *npartial = stencil_partials(s, g, nin);
+    if (kind == K_TILE) {
+        switch (dtype) {
+        case DT_F64: return tile_launch_f64(s, mode, in, out, aux, g, ph, w, TZ, sh, partial, write, st);
+        case DT_C128: return tile_launch_c128(s, mode, in, out, aux, g, ph, w, TZ, sh, partial, write, st);
+        case DT_F32: return tile_launch_f32(s, mode, in, out, aux, g, ph, w, TZ, sh, partial, write, st);
+        case DT_C64: return tile_launch_c64(s, mode, in, out, aux, g, ph, w, TZ, sh, partial, write, st);
+        }
+        return hipErrorInvalidValue;
+    }
+    Stencil sq = s;
+    if (kind == K_GENERIC) sq.star_r = 0;
     switch (dtype) {
-    case DT_F64: return dispatch_mode<double>(s, mode, in, out, aux, g, ph, nin, w, partial, write, st);
-    case DT_C128: return dispatch_mode<double2>(s, mode, in, out, aux, g, ph, nin, w, partial, write, st);
-    case DT_F32: return dispatch_mode<float>(s, mode, in, out, aux, g, ph, nin, w, partial, write, st);
-    case DT_C64: return dispatch_mode<float2>(s, mode, in, out, aux, g, ph, nin, w, partial, write, st);
+    case DT_F64: return dispatch_mode<double>(sq, mode, in, out, aux, g, ph, nin, w, partial, write, st);
+    case DT_C128: return dispatch_mode<double2>(sq, mode, in, out, aux, g, ph, nin, w, partial, write, st);
+    case DT_F32: return dispatch_mode<float>(sq, mode, in, out, aux, g, ph, nin, w, partial, write, st);
+    case DT_C64: return dispatch_mode<float2>(sq, mode, in, out, aux, g, ph, nin, w, partial, write, st);
     }
     return hipErrorInvalidValue;
 }

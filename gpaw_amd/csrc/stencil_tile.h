@@ -1,0 +1,223 @@
+// LDS-tiled 2.5-D star-stencil kernel for large grids (gfx950).
+//
+// A 256-thread block owns a TY x TZ tile of the (i1, i2) plane and sweeps a
+// run of i0 planes.  Each thread owns ROWS consecutive i1 rows of one i2
+// column:
+//   * axis-0 neighbours come from a register queue of 2R+1 planes per row
+//     (every input value is read from HBM once per sweep);
+//   * the current plane, with an R-wide halo in i1 and i2, is staged in LDS
+//     (double-buffered, one barrier per plane);
+//   * axis-1 neighbours use a register window of ROWS + 2R values per
+//     column, so only the 2R window ends are LDS reads;
+//   * the next plane's queue values and halo cells are loaded before the
+//     barrier, overlapping their latency with the current plane's work.
+// Waves are TZ lanes wide along the contiguous axis, so every HBM access is
+// a full coalesced row segment.  Tiles are ordered so that one XCD (one L2)
+// processes a contiguous range of tiles: the halo rows a block re-reads
+// were loaded by its neighbour on the same XCD.
+//
+// The sum runs in the reference's coefficient order (centre, +-k along
+// axis 2, axis 1, axis 0: c/bmgs/fd.c:41-44 with the NewGUCLaplace order),
+// so results are bit-identical to the reference C.
+#pragma once
+#include "kernels.h"
+
+namespace gpaw {
+
+struct TileCoefs {
+    double c[6 * GPAW_MAXR + 1];
+};
+
+struct TileShape {
+    int ntz, nty, nchunk, q;   // tiles along i2, i1; chunks along i0 of q planes
+    int nblocks;
+};
+
+template <class V>
+__device__ __forceinline__ double tnorm2(V v);
+template <> __device__ __forceinline__ double tnorm2<double>(double v) { return v * v; }
+template <> __device__ __forceinline__ double tnorm2<float>(float v) { return (double)v * (double)v; }
+template <> __device__ __forceinline__ double tnorm2<double2>(double2 v) { return v.x * v.x + v.y * v.y; }
+template <> __device__ __forceinline__ double tnorm2<float2>(float2 v)
+{
+    return (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+}
+
+template <class V, int R, int TZ, int ROWS, int MODE, bool NORM, bool WRITE>
+__global__ __launch_bounds__(256) void k_star_tile(const V* __restrict__ in, V* __restrict__ out,
+                                                   const V* __restrict__ aux, Geom g, Phases ph,
+                                                   TileCoefs sc, double w, TileShape sh, i64 ng,
+                                                   double* __restrict__ partial)
+{
+    typedef typename VT<V>::R RT;
+    constexpr int NTY = 256 / TZ;
+    constexpr int TY = NTY * ROWS;
+    constexpr int LW = TZ + 2 * R;              // LDS row length
+    constexpr int LH = TY + 2 * R;              // LDS rows
+    constexpr int NHALO = 2 * R * (TZ + TY);    // halo cells per plane
+    constexpr int NH = (NHALO + 255) / 256;     // halo cells per thread
+    __shared__ V lds[2][LH * LW];
+
+    // XCD-aware tile order: block b runs on XCD b % 8; give each XCD a
+    // contiguous range of tiles
+    const int nb = sh.nblocks;
+    const int b = blockIdx.x;
+    int t = b;
+    if ((nb & 7) == 0) t = (b & 7) * (nb >> 3) + (b >> 3);
+    const int tz = t % sh.ntz;
+    int rest = t / sh.ntz;
+    const int ty = rest % sh.nty;
+    rest /= sh.nty;
+    const int ch = rest % sh.nchunk;
+    const int band = rest / sh.nchunk;
+
+    const int tx = threadIdx.x, tyi = threadIdx.y;
+    const int tid = tx + TZ * tyi;
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    const i64 plane = (i64)n1 * n2;
+    const i64 boff = (i64)band * ng;
+    const V* __restrict__ a = in + boff;
+    const int z0 = tz * TZ, y0 = ty * TY;
+    const int i2 = z0 + tx;
+    const int i0s = ch * sh.q;
+    const int i0e = min(i0s + sh.q, n0);
+
+    // owned rows
+    int i1r[ROWS];
+    bool inr[ROWS];
+#pragma unroll
+    for (int j = 0; j < ROWS; j++) {
+        i1r[j] = y0 + tyi * ROWS + j;
+        inr[j] = (i1r[j] < n1) && (i2 < n2);
+    }
+    // halo cells of this thread: LDS offset and (i1, i2)
+    int hoff[NH], hy[NH], hz[NH];
+#pragma unroll
+    for (int k = 0; k < NH; k++) {
+        const int h = tid + 256 * k;
+        int lr, lc;
+        if (h < 2 * R * TZ) {
+            const int yr = h / TZ;
+            lc = R + h % TZ;
+            lr = yr < R ? yr : TY + yr;
+        } else {
+            const int hh = h - 2 * R * TZ;
+            const int zc = hh % (2 * R);
+            lr = R + hh / (2 * R);
+            lc = zc < R ? zc : TZ + zc;
+        }
+        hoff[k] = (h < NHALO) ? lr * LW + lc : -1;
+        hy[k] = y0 + lr - R;
+        hz[k] = z0 + lc - R;
+    }
+
+    auto qload = [&](int p, int j) -> V {
+        if (inr[j]) {
+            int pp = p;
+            const int f = wrap_index(pp, n0, g.per[0]);
+            if (f == 3) return vzero<V>();
+            V v = a[(i64)pp * plane + (i64)i1r[j] * n2 + i2];
+            if (VT<V>::cplx && f) v = pmul(v, ph.re[0][f - 1], ph.im[0][f - 1]);
+            return v;
+        }
+        return ghost_fetch(a, p, i1r[j], i2, g, ph);
+    };
+
+    V q[ROWS][2 * R + 1];
+    V qn[ROWS];
+    V hn[NH];
+#pragma unroll
+    for (int j = 0; j < ROWS; j++) {
+#pragma unroll
+        for (int k = 0; k < 2 * R; k++) q[j][k] = qload(i0s - R + k, j);
+        qn[j] = qload(i0s + R, j);
+    }
+#pragma unroll
+    for (int k = 0; k < NH; k++)
+        hn[k] = hoff[k] >= 0 ? ghost_fetch(a, i0s, hy[k], hz[k], g, ph) : vzero<V>();
+
+    double acc = 0.0;
+    for (int i0 = i0s; i0 < i0e; i0++) {
+        V* B = lds[(i0 - i0s) & 1];
+#pragma unroll
+        for (int j = 0; j < ROWS; j++) {
+            q[j][2 * R] = qn[j];
+            B[(R + tyi * ROWS + j) * LW + R + tx] = q[j][R];
+        }
+#pragma unroll
+        for (int k = 0; k < NH; k++)
+            if (hoff[k] >= 0) B[hoff[k]] = hn[k];
+        V s[ROWS];
+        if (MODE != ST_APPLY) {
+#pragma unroll
+            for (int j = 0; j < ROWS; j++)
+                s[j] = inr[j] ? aux[boff + (i64)i0 * plane + (i64)i1r[j] * n2 + i2] : vzero<V>();
+        }
+        if (i0 + 1 < i0e) {
+#pragma unroll
+            for (int j = 0; j < ROWS; j++) qn[j] = qload(i0 + 1 + R, j);
+#pragma unroll
+            for (int k = 0; k < NH; k++)
+                if (hoff[k] >= 0) hn[k] = ghost_fetch(a, i0 + 1, hy[k], hz[k], g, ph);
+        }
+        __syncthreads();
+        // register window along axis 1: rows tyi*ROWS - R .. tyi*ROWS + ROWS + R - 1
+        V win[ROWS + 2 * R];
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            win[k] = B[(tyi * ROWS + k) * LW + R + tx];
+            win[ROWS + R + k] = B[(tyi * ROWS + ROWS + R + k) * LW + R + tx];
+        }
+#pragma unroll
+        for (int j = 0; j < ROWS; j++) win[R + j] = q[j][R];
+#pragma unroll
+        for (int j = 0; j < ROWS; j++) {
+            const V* row = B + (R + tyi * ROWS + j) * LW + R + tx;
+            V x = vzero<V>();
+            if (MODE != ST_JACOBI) x = vadd(x, vmul(q[j][R], (RT)sc.c[0]));
+#pragma unroll
+            for (int k = 1; k <= R; k++) x = vadd(x, vmul(row[k], (RT)sc.c[k]));
+#pragma unroll
+            for (int k = 1; k <= R; k++) x = vadd(x, vmul(row[-k], (RT)sc.c[R + k]));
+#pragma unroll
+            for (int k = 1; k <= R; k++) x = vadd(x, vmul(win[R + j + k], (RT)sc.c[2 * R + k]));
+#pragma unroll
+            for (int k = 1; k <= R; k++) x = vadd(x, vmul(win[R + j - k], (RT)sc.c[3 * R + k]));
+#pragma unroll
+            for (int k = 1; k <= R; k++) x = vadd(x, vmul(q[j][R + k], (RT)sc.c[4 * R + k]));
+#pragma unroll
+            for (int k = 1; k <= R; k++) x = vadd(x, vmul(q[j][R - k], (RT)sc.c[5 * R + k]));
+            V r;
+            if (MODE == ST_APPLY) {
+                r = x;
+            } else if (MODE == ST_RESID) {
+                r = vsub(x, s[j]);
+            } else {
+                const RT wr = (RT)w;
+                r = vadd(vmul(q[j][R], (RT)1 - wr), vdiv_r(vmul(vsub(s[j], x), wr), (RT)sc.c[0]));
+            }
+            if (inr[j]) {
+                if (NORM) acc += tnorm2(r);
+                if (WRITE) out[boff + (i64)i0 * plane + (i64)i1r[j] * n2 + i2] = r;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < ROWS; j++)
+#pragma unroll
+            for (int k = 0; k < 2 * R; k++) q[j][k] = q[j][k + 1];
+    }
+    if (NORM) {
+        __shared__ double red[256];
+        const double sum = block_sum<256>(acc, red);
+        if (tid == 0) partial[b] = sum;
+    }
+}
+
+// host-side shape selection and launch (stencil_tile_*.hip)
+bool tile_shape(const Geom& g, int R, int nin, int& TZ, TileShape& sh);
+hipError_t launch_stencil_tile(const Stencil& s, int dtype, int mode, const void* in, void* out,
+                               const void* aux, const Geom& g, const Phases& ph, int nin,
+                               double w, double* partial, int* npartial, bool write,
+                               hipStream_t st, bool* handled);
+
+}  // namespace gpaw

@@ -1,0 +1,65 @@
+// Host launcher of k_star_tile for one storage type (instantiated once per
+// type in stencil_tile_<type>.hip so the variants compile in parallel).
+#pragma once
+#include "stencil_tile.h"
+
+namespace gpaw {
+
+template <class V, int R, int TZ, int MODE, bool NORM, bool WRITE>
+hipError_t tile_go(const Stencil& s, const void* in, void* out, const void* aux, const Geom& g,
+                   const Phases& ph, double w, const TileShape& sh, double* partial,
+                   hipStream_t st)
+{
+    constexpr int ROWS = R <= 3 ? 4 : 2;
+    TileCoefs sc;
+    for (int c = 0; c < 6 * R + 1; c++) sc.c[c] = s.coefs[c];
+    const i64 ng = (i64)g.n[0] * g.n[1] * g.n[2];
+    hipLaunchKernelGGL((k_star_tile<V, R, TZ, ROWS, MODE, NORM, WRITE>), dim3(sh.nblocks),
+                       dim3(TZ, 256 / TZ), 0, st, (const V*)in, (V*)out, (const V*)aux, g, ph, sc,
+                       w, sh, ng, partial);
+    return hipGetLastError();
+}
+
+template <class V, int R, int TZ>
+hipError_t tile_modes(const Stencil& s, int mode, const void* in, void* out, const void* aux,
+                      const Geom& g, const Phases& ph, double w, const TileShape& sh,
+                      double* partial, bool write, hipStream_t st)
+{
+    if constexpr (VT<V>::cplx) {
+        if (mode != ST_APPLY || partial) return hipErrorInvalidValue;
+        return tile_go<V, R, TZ, ST_APPLY, false, true>(s, in, out, aux, g, ph, w, sh, nullptr, st);
+    } else {
+        if (mode == ST_APPLY)
+            return tile_go<V, R, TZ, ST_APPLY, false, true>(s, in, out, aux, g, ph, w, sh, nullptr, st);
+        if (mode == ST_JACOBI)
+            return tile_go<V, R, TZ, ST_JACOBI, false, true>(s, in, out, aux, g, ph, w, sh, nullptr, st);
+        if (!partial)
+            return tile_go<V, R, TZ, ST_RESID, false, true>(s, in, out, aux, g, ph, w, sh, nullptr, st);
+        if (write)
+            return tile_go<V, R, TZ, ST_RESID, true, true>(s, in, out, aux, g, ph, w, sh, partial, st);
+        return tile_go<V, R, TZ, ST_RESID, true, false>(s, in, out, aux, g, ph, w, sh, partial, st);
+    }
+}
+
+template <class V>
+hipError_t tile_dispatch(const Stencil& s, int mode, const void* in, void* out, const void* aux,
+                         const Geom& g, const Phases& ph, double w, int TZ, const TileShape& sh,
+                         double* partial, bool write, hipStream_t st)
+{
+#define GPAW_TILE_R(RR)                                                                      \
+    case RR:                                                                                 \
+        return TZ == 64 ? tile_modes<V, RR, 64>(s, mode, in, out, aux, g, ph, w, sh, partial, \
+                                                 write, st)                                  \
+                        : tile_modes<V, RR, 32>(s, mode, in, out, aux, g, ph, w, sh, partial, \
+                                                 write, st);
+    switch (s.star_r) {
+        GPAW_TILE_R(1)
+        GPAW_TILE_R(2)
+        GPAW_TILE_R(3)
+        GPAW_TILE_R(4)
+    }
+#undef GPAW_TILE_R
+    return hipErrorInvalidValue;
+}
+
+}  // namespace gpaw
