@@ -176,9 +176,9 @@ def cpu_baseline(np, gd_shape_args, rho, eps, budget):
 def main():
     args = parse()
     if args.gpus != 1:
-        print('bench.py runs on one GPU only (--gpus %d requested): the hot '
-              'path here is single-GPU; see DESIGN.md for how it would shard'
-              % args.gpus, file=sys.stderr)
+        print('bench.py runs on one GPU only (--gpus %d requested): this '
+              'build measures the single-GPU share of the path; multi-GPU '
+              'domain decomposition is not built' % args.gpus, file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
