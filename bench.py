@@ -132,6 +132,59 @@ def fd_laplace_c3(torch, np):
             'bytes_per_apply': algo}
 
 
+def mixed_vs_fp64(torch, np, N=160, h=0.18, count=32, reps=3):
+    """Mixed-precision vs plain FP64 multigrid on the 160^3 zero-boundary
+    configuration (159^3 arrays, h = 0.18 A, 32 seeded neutral Gaussians of
+    widths 0.4-1.2 A), both to ||L phi - rho|| / ||rho|| <= 1e-10."""
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.poisson import MixedPoissonSolver, PoissonSolver
+    from gpaw_amd.utilities import gaussian_density, random_gaussian_charges
+    gd = GridDescriptor((N,) * 3, (N * h / BOHR,) * 3, False)
+    rng = np.random.default_rng(160)
+    c, w, q = random_gaussian_charges(gd, rng, count, 0.4 / BOHR, 1.2 / BOHR,
+                                      2.0 / BOHR)
+    rho = gaussian_density(gd, c, w, q)
+    rho_d = torch.from_numpy(rho).to('cuda')
+    eps, _ = eps_for(np, gd, rho)
+    res = {}
+    phis = {}
+    for name, solver in (('fp64', PoissonSolver(nn=3, eps=eps)),
+                         ('mixed', MixedPoissonSolver(nn=3, rel_tol=1e-10))):
+        solver.set_grid_descriptor(gd)
+        solver.initialize()
+        phi = torch.zeros(tuple(gd.n_c), dtype=torch.float64, device='cuda')
+        it = solver.solve(phi, rho_d)          # warm-up (graph capture)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            phi.zero_()
+            it = solver.solve(phi, rho_d)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        res[name] = {'solves_per_s': round(1.0 / dt, 3),
+                     'ms_per_solve': round(dt * 1e3, 3),
+                     'iterations': it}
+        phis[name] = phi
+        del solver
+    bg = float(rho.sum()) / np.prod(gd.get_size_of_global_array())
+    rho_n = torch.from_numpy(rho - bg).to('cuda')
+    from gpaw_amd.fd_operators import Laplace
+    lap = Laplace(gd, -0.25 / np.pi, 3)
+    r = torch.empty_like(rho_n)
+    for name in phis:
+        lap.operator.residual(phis[name], rho_n, r)
+        res[name]['rel_residual'] = float(torch.linalg.vector_norm(r).item() /
+                                          torch.linalg.vector_norm(rho_n).item())
+    d = (phis['mixed'] - phis['fp64']).abs().max().item()
+    res['mixed']['rel_maxnorm_err_vs_fp64'] = d / phis['fp64'].abs().max().item()
+    res['speedup_mixed_over_fp64'] = round(
+        res['fp64']['ms_per_solve'] / res['mixed']['ms_per_solve'], 3)
+    res['config'] = ('160^3 zero boundary (159^3 arrays), h=0.18 A, 32 neutral '
+                     'Gaussians; FP64 = reference V-cycles; mixed = FP32 '
+                     'V-cycle preconditioning an FP64 flexible-CG refinement')
+    return res
+
+
 def cpu_baseline(np, gd_shape_args, rho, eps, budget):
     """The reference's own C kernels (oracle/_ref, compiled from the
     reference sources) under the restated V-cycle driver, one host core."""
@@ -238,6 +291,7 @@ def main():
     out['roofline']['copy_GBps_measured'] = copy_bandwidth(torch)
     if not args.no_extra:
         out['fd_laplace'] = fd_laplace_c3(torch, np)
+        out['mixed_precision'] = mixed_vs_fp64(torch, np)
     if not args.no_cpu:
         cb = cpu_baseline(np, ((96,) * 3, gd.cell_cv, False), rho, eps,
                           args.cpu_seconds)

@@ -40,6 +40,8 @@ PROTOTYPES = {
     'gpaw_poisson_solve_neutral': (_i, [_vp, _vp, _vp, _d, _i, _ip, _dp,
                                         _vp]),
     'gpaw_poisson_vcycles': (_i, [_vp, _vp, _vp, _i, _dp, _vp]),
+    'gpaw_poisson_solve_mixed': (_i, [_vp, _vp, _vp, _d, _i, _i, _i, _ip, _dp,
+                                      _vp]),
     'gpaw_poisson_free': (None, [_vp]),
     'gpaw_sum': (_i, [_vp, _l, _d, _vp, _vp]),
     'gpaw_axpby': (_i, [_vp, _vp, _d, _d, _l, _vp]),

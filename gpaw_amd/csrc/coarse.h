@@ -31,5 +31,8 @@ struct CoarsePlan {
 TrArgs make_trargs(const Transfer& tr);
 hipError_t launch_coarse_vcycle(const CoarsePlan* d_plan, int lds_bytes, hipStream_t st);
 hipError_t coarse_set_lds_limit(int bytes);
+// FP64 coarse solve inside an FP32 V-cycle: first-level rho / phi are float
+// (LDS mode only)
+hipError_t launch_coarse_vcycle_f32(const CoarsePlan* d_plan, int lds_bytes, hipStream_t st);
 
 }  // namespace gpaw

@@ -69,6 +69,10 @@ __device__ void jacobi_sweeps(const CoarseLevel& L, double* const* ptr, int k, i
 
 }  // namespace
 
+// TB: storage type of the first level's right-hand side and solution in HBM
+// (double, or float when the fused kernel is the FP64 coarse solve of an
+// FP32 V-cycle; then the arrays must live in LDS and are converted there).
+template <class TB>
 __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __restrict__ P)
 {
     extern __shared__ double smem[];
@@ -98,8 +102,9 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
     if (lds) {   // right-hand side of the first coarse level, phi = 0
         const CoarseLevel& G = P->lev[0];
         const int n = G.g.n[0] * G.g.n[1] * G.g.n[2];
+        const TB* __restrict__ grho = (const TB*)G.rho;
         for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
-            ptr[0][RHO][idx] = G.rho[idx];
+            ptr[0][RHO][idx] = (double)grho[idx];
             ptr[0][PA][idx] = 0.0;
         }
         __syncthreads();
@@ -157,21 +162,31 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
         const CoarseLevel& G = P->lev[0];
         const int k = (nl > 1 ? G.pre : 0) + G.post;
         const double* src = ptr[0][(k & 1) ? PB : PA];
-        double* dst = (k & 1) ? G.phiB : G.phiA;
+        TB* dst = (TB*)((k & 1) ? G.phiB : G.phiA);
         const int n = G.g.n[0] * G.g.n[1] * G.g.n[2];
-        for (int idx = threadIdx.x; idx < n; idx += blockDim.x) dst[idx] = src[idx];
+        for (int idx = threadIdx.x; idx < n; idx += blockDim.x) dst[idx] = (TB)src[idx];
     }
 }
 
 hipError_t coarse_set_lds_limit(int bytes)
 {
-    return hipFuncSetAttribute((const void*)k_coarse_vcycle,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    hipError_t e = hipFuncSetAttribute((const void*)k_coarse_vcycle<double>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_coarse_vcycle<float>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    return e;
 }
 
 hipError_t launch_coarse_vcycle(const CoarsePlan* d_plan, int lds_bytes, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_coarse_vcycle, dim3(1), dim3(1024), lds_bytes, st, d_plan);
+    hipLaunchKernelGGL(k_coarse_vcycle<double>, dim3(1), dim3(1024), lds_bytes, st, d_plan);
+    return hipGetLastError();
+}
+
+hipError_t launch_coarse_vcycle_f32(const CoarsePlan* d_plan, int lds_bytes, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_coarse_vcycle<float>, dim3(1), dim3(1024), lds_bytes, st, d_plan);
     return hipGetLastError();
 }
 

@@ -1,0 +1,434 @@
+// Mixed-precision multigrid Poisson solver (gfx950).
+//
+// Precision split:
+//   FP64  residual d = L phi - rho (the reference operator, c/bmgs/fd.c),
+//         correction accumulation into phi, all dot products and the
+//         convergence test, the charge-neutrality terms (Python side), and
+//         the coarsest levels (the fused LDS coarse kernel computes in FP64;
+//         only its first-level right-hand side and solution are FP32);
+//   FP32  smoothing (weighted Jacobi), residuals, restriction and
+//         interpolation of the inner V-cycle on every level above that.
+//
+// Outer loop (FP64), either
+//   mode 0  iterative refinement: d = L phi - rho; e = V32^k(d); phi -= e
+//   mode 1  flexible preconditioned conjugate gradients with one FP32
+//           V-cycle as preconditioner (Polak-Ribiere beta, which tolerates
+//           the preconditioner's FP32 rounding), i.e. iterative refinement
+//           with a Krylov recurrence on the corrections.
+// The inner V-cycle uses the reference hierarchy (gpaw/poisson.py:64-113)
+// with symmetric level-0 smoothing (2 pre + 2 post by default) so that it is
+// a symmetric preconditioner.  Stopping rule: ||d||_2 <= tol * ||rho||_2.
+//
+// Scalars (alpha, beta, dot products) stay on the device; the host reads
+// one 8-byte norm per outer iteration.  The FP32 V-cycle and the per-
+// iteration kernel sequences are captured as hipGraphs.
+#include <cstdlib>
+#include <cstring>
+
+#include "poisson_internal.h"
+
+using namespace gpaw;
+
+struct MixedState {
+    int L = 0;
+    int lc = -1;                        // fused FP64 coarse level (-1: none)
+    std::vector<int> pre, post;
+    std::vector<float*> phiA, phiB, rho, res;
+    CoarsePlan* d_coarse32 = nullptr;
+    double *p = nullptr, *q = nullptr;  // FP64 search direction, L p
+    double* S = nullptr;                // device scalars
+    double* h_S = nullptr;              // pinned readback
+    double* partial = nullptr;
+    int nblk = 0;
+    hipStream_t cap = nullptr;
+    hipGraphExec_t g_vcycle = nullptr;  // one FP32 V-cycle from zero
+    hipGraphExec_t g_vwarm = nullptr;   // one FP32 V-cycle from the last result
+    hipGraphExec_t g_a = nullptr;       // PCG: q = Lp, alpha, update, |d|^2
+    hipGraphExec_t g_b = nullptr;       // PCG: z = V32(d), beta, p = z + beta p
+};
+
+// device scalars
+enum { S_DZ = 0, S_PQ = 1, S_DD = 2, S_ZQ = 3, S_DZOLD = 4, S_ALPHA = 5, S_N = 8 };
+
+namespace {
+
+constexpr int NB = 1024;   // blocks of the reduction kernels
+
+__global__ __launch_bounds__(256) void k_dot(const double* __restrict__ x,
+                                             const double* __restrict__ y, i64 n,
+                                             double* __restrict__ partial)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256)
+        s += x[i] * y[i];
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// S[slot] = sum of partial[0..n)
+__global__ __launch_bounds__(256) void k_red1(const double* __restrict__ p, int n,
+                                              double* __restrict__ S, int slot)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += p[i];
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) S[slot] = s;
+}
+
+// S[DZOLD] = S[DZ]; S[DZ] = sum p[0..n); S[ZQ] = sum p[n..2n)
+__global__ __launch_bounds__(256) void k_red2(const double* __restrict__ p, int n,
+                                              double* __restrict__ S)
+{
+    __shared__ double red[256];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        a += p[i];
+        b += p[n + i];
+    }
+    a = block_sum<256>(a, red);
+    b = block_sum<256>(b, red);
+    if (threadIdx.x == 0) {
+        S[S_DZOLD] = S[S_DZ];
+        S[S_DZ] = a;
+        S[S_ZQ] = b;
+    }
+}
+
+// alpha = dz / pq ; phi -= alpha p ; d -= alpha q ; d32 = d ; partial |d|^2
+__global__ __launch_bounds__(256) void k_pcg_update(double* __restrict__ phi, double* __restrict__ d,
+                                                    const double* __restrict__ p,
+                                                    const double* __restrict__ q,
+                                                    float* __restrict__ d32, double* __restrict__ S,
+                                                    i64 n, double* __restrict__ partial)
+{
+    __shared__ double red[256];
+    const double alpha = S[S_DZ] / S[S_PQ];
+    double s = 0.0;
+    for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256) {
+        phi[i] = phi[i] - alpha * p[i];
+        const double di = d[i] - alpha * q[i];
+        d[i] = di;
+        d32[i] = (float)di;
+        s += di * di;
+    }
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    if (blockIdx.x == 0 && threadIdx.x == 0) S[S_ALPHA] = alpha;
+}
+
+// partials of d.z and z.q (z from the FP32 preconditioner)
+__global__ __launch_bounds__(256) void k_dot_dz_zq(const double* __restrict__ d,
+                                                   const float* __restrict__ z,
+                                                   const double* __restrict__ q, i64 n,
+                                                   double* __restrict__ partial)
+{
+    __shared__ double red[256];
+    double a = 0.0, b = 0.0;
+    for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256) {
+        const double zi = (double)z[i];
+        a += d[i] * zi;
+        b += zi * q[i];
+    }
+    a = block_sum<256>(a, red);
+    b = block_sum<256>(b, red);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = a;
+        partial[gridDim.x + blockIdx.x] = b;
+    }
+}
+
+// p = z + beta p, beta = -alpha (z.q) / (d_old.z_old) (Polak-Ribiere); first: p = z
+__global__ __launch_bounds__(256) void k_p_update(double* __restrict__ p, const float* __restrict__ z,
+                                                  const double* __restrict__ S, i64 n, int first)
+{
+    const double beta = first ? 0.0 : -S[S_ALPHA] * S[S_ZQ] / S[S_DZOLD];
+    for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256)
+        p[i] = first ? (double)z[i] : (double)z[i] + beta * p[i];
+}
+
+// phi -= e (FP64 accumulation of an FP32 correction)
+__global__ __launch_bounds__(256) void k_sub_f32(double* __restrict__ phi, const float* __restrict__ e,
+                                                 i64 n)
+{
+    for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256)
+        phi[i] = phi[i] - (double)e[i];
+}
+
+dim3 gridn(i64 n)
+{
+    i64 b = (n + 255) / 256;
+    return dim3((unsigned)(b < NB ? b : NB));
+}
+
+hipError_t relax32(gpaw_poisson* ps, MixedState* m, int l, int k, float*& cur, float*& alt,
+                   hipStream_t st)
+{
+    Phases ph;
+    gpaw_phases(nullptr, ph);
+    gpaw_operator* op = ps->ops[l];
+    for (int it = 0; it < k; it++) {
+        hipError_t e = launch_stencil(op->st, DT_F32, ST_JACOBI, cur, alt, m->rho[l], op->g, ph,
+                                      1, ps->w[l], nullptr, nullptr, true, st);
+        if (e != hipSuccess) return e;
+        float* t = cur;
+        cur = alt;
+        alt = t;
+    }
+    return hipSuccess;
+}
+
+// one FP32 V-cycle on level l (level 0: rho = d32, result in phiA[0])
+hipError_t emit32(gpaw_poisson* ps, MixedState* m, int l, bool zero0, hipStream_t st)
+{
+    Phases ph;
+    gpaw_phases(nullptr, ph);
+    if (l == m->lc) return launch_coarse_vcycle_f32(m->d_coarse32, ps->coarse_lds, st);
+    gpaw_operator* op = ps->ops[l];
+    float* cur = m->phiA[l];
+    float* alt = m->phiB[l];
+    hipError_t e = hipSuccess;
+    if (l == 0 && zero0) e = launch_fill(cur, DT_F32, ps->npts[0], st);
+    if (e != hipSuccess) return e;
+    if (l < m->L) {
+        e = relax32(ps, m, l, m->pre[l], cur, alt, st);
+        if (e != hipSuccess) return e;
+        e = launch_stencil(op->st, DT_F32, ST_RESID, cur, m->res[l], m->rho[l], op->g, ph, 1, 0.0,
+                           nullptr, nullptr, true, st);
+        if (e != hipSuccess) return e;
+        e = launch_transfer(ps->rst[l]->t, DT_F32, TR_STORE, m->res[l], m->rho[l + 1],
+                            m->phiA[l + 1], ph, 1, st);
+        if (e != hipSuccess) return e;
+        e = emit32(ps, m, l + 1, false, st);
+        if (e != hipSuccess) return e;
+        const int kc = (l + 1 < m->L ? m->pre[l + 1] : 0) + m->post[l + 1];
+        const float* coarse = (kc % 2) ? m->phiB[l + 1] : m->phiA[l + 1];
+        e = launch_transfer(ps->itp[l]->t, DT_F32, TR_SUBTRACT, coarse, cur, nullptr, ph, 1, st);
+        if (e != hipSuccess) return e;
+    }
+    return relax32(ps, m, l, m->post[l], cur, alt, st);
+}
+
+float* z_of(MixedState* m) { return m->phiA[0]; }   // even level-0 sweep count
+
+template <class F>
+int capture_graph(hipStream_t cap, hipGraphExec_t* out, F&& body)
+{
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return gpaw_check(e, "mixed: begin capture");
+    hipError_t e1 = body(cap);
+    e = hipStreamEndCapture(cap, &g);
+    if (e1 != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return gpaw_check(e1, "mixed: captured launch");
+    }
+    if (e != hipSuccess) return gpaw_check(e, "mixed: end capture");
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    return gpaw_check(e, "mixed: instantiate");
+}
+
+int setup(gpaw_poisson* ps)
+{
+    if (ps->mixed) return GPAW_OK;
+    MixedState* m = new MixedState();
+    ps->mixed = m;
+    m->L = ps->L;
+    m->pre = ps->pre;
+    m->post = ps->post;
+    int s0 = 2, s1 = 2;
+    if (const char* env = getenv("GPAW_AMD_MIXED_SMOOTH")) sscanf(env, "%d,%d", &s0, &s1);
+    if ((s0 + s1) % 2) s1++;             // result must land in phiA[0]
+    m->pre[0] = s0;
+    m->post[0] = s1;
+    if (ps->L == 0) m->post[0] = s0 + s1;
+    int rc = GPAW_OK;
+    for (int l = 0; l <= ps->L && !rc; l++) {
+        float *a = nullptr, *b = nullptr, *r = nullptr, *x = nullptr;
+        const size_t n = sizeof(float) * (ps->npts[l] > 0 ? ps->npts[l] : 1);
+        rc = gpaw_check(hipMalloc((void**)&a, n), "mixed: level arrays");
+        if (!rc) rc = gpaw_check(hipMalloc((void**)&b, n), "mixed: level arrays");
+        if (!rc) rc = gpaw_check(hipMalloc((void**)&r, n), "mixed: level arrays");
+        if (!rc) rc = gpaw_check(hipMalloc((void**)&x, n), "mixed: level arrays");
+        m->phiA.push_back(a);
+        m->phiB.push_back(b);
+        m->rho.push_back(r);
+        m->res.push_back(x);
+    }
+    const size_t n0 = sizeof(double) * ps->npts[0];
+    if (!rc) rc = gpaw_check(hipMalloc((void**)&m->p, n0), "mixed: p");
+    if (!rc) rc = gpaw_check(hipMalloc((void**)&m->q, n0), "mixed: q");
+    if (!rc) rc = gpaw_check(hipMalloc((void**)&m->S, sizeof(double) * S_N), "mixed: scalars");
+    if (!rc) rc = gpaw_check(hipHostMalloc((void**)&m->h_S, sizeof(double) * S_N), "mixed: pinned");
+    if (!rc) rc = gpaw_check(hipMalloc((void**)&m->partial, sizeof(double) * 2 * NB), "mixed: partials");
+    if (!rc) rc = gpaw_check(hipStreamCreateWithFlags(&m->cap, hipStreamNonBlocking), "mixed: stream");
+    if (!rc) rc = gpaw_check(hipMemset(m->S, 0, sizeof(double) * S_N), "mixed: scalars");
+    // the fused FP64 coarse solve needs its arrays in LDS (FP32 boundary)
+    if (!rc && ps->lc > 0 && ps->h_coarse.lds) {
+        CoarsePlan P = ps->h_coarse;
+        P.lev[0].rho = (double*)m->rho[ps->lc];
+        P.lev[0].phiA = (double*)m->phiA[ps->lc];
+        P.lev[0].phiB = (double*)m->phiB[ps->lc];
+        rc = gpaw_check(hipMalloc((void**)&m->d_coarse32, sizeof P), "mixed: coarse plan");
+        if (!rc) rc = gpaw_check(hipMemcpy(m->d_coarse32, &P, sizeof P, hipMemcpyHostToDevice),
+                                 "mixed: coarse plan");
+        if (!rc) m->lc = ps->lc;
+    }
+    return rc;
+}
+
+int capture_all(gpaw_poisson* ps)
+{
+    MixedState* m = ps->mixed;
+    if (m->g_vcycle) return GPAW_OK;
+    const i64 n = ps->npts[0];
+    gpaw_operator* op = ps->ops[0];
+    int rc = capture_graph(m->cap, &m->g_vcycle,
+                           [&](hipStream_t st) { return emit32(ps, m, 0, true, st); });
+    if (!rc)
+        rc = capture_graph(m->cap, &m->g_vwarm,
+                           [&](hipStream_t st) { return emit32(ps, m, 0, false, st); });
+    if (!rc)
+        rc = capture_graph(m->cap, &m->g_a, [&](hipStream_t st) -> hipError_t {
+            Phases ph;
+            gpaw_phases(nullptr, ph);
+            hipError_t e = launch_stencil(op->st, DT_F64, ST_APPLY, m->p, m->q, nullptr, op->g,
+                                          ph, 1, 0.0, nullptr, nullptr, true, st);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_dot, gridn(n), dim3(256), 0, st, m->p, m->q, n, m->partial);
+            hipLaunchKernelGGL(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x,
+                               m->S, (int)S_PQ);
+            hipLaunchKernelGGL(k_pcg_update, gridn(n), dim3(256), 0, st, ps->phiA[0], ps->res[0],
+                               m->p, m->q, m->rho[0], m->S, n, m->partial);
+            hipLaunchKernelGGL(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x,
+                               m->S, (int)S_DD);
+            return hipGetLastError();
+        });
+    if (!rc)
+        rc = capture_graph(m->cap, &m->g_b, [&](hipStream_t st) -> hipError_t {
+            hipError_t e = emit32(ps, m, 0, true, st);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_dot_dz_zq, gridn(n), dim3(256), 0, st, ps->res[0], z_of(m), m->q,
+                               n, m->partial);
+            hipLaunchKernelGGL(k_red2, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x,
+                               m->S);
+            hipLaunchKernelGGL(k_p_update, gridn(n), dim3(256), 0, st, m->p, z_of(m), m->S, n, 0);
+            return hipGetLastError();
+        });
+    return rc;
+}
+
+int read_scalar(MixedState* m, int slot, hipStream_t st, double* v)
+{
+    hipError_t e = hipMemcpyAsync(m->h_S, m->S, sizeof(double) * S_N, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return gpaw_check(e, "mixed: readback");
+    *v = m->h_S[slot];
+    return GPAW_OK;
+}
+
+// d = L phi - rho (FP64, stored in res[0]) with |d|^2 into S[DD]; d32 = d
+int residual64(gpaw_poisson* ps, MixedState* m, hipStream_t st)
+{
+    gpaw_operator* op = ps->ops[0];
+    Phases ph;
+    gpaw_phases(nullptr, ph);
+    int np = 0;
+    hipError_t e = launch_stencil(op->st, DT_F64, ST_RESID, ps->phiA[0], ps->res[0], ps->rho[0],
+                                  op->g, ph, 1, 0.0, op->partial, &np, true, st);
+    if (e == hipSuccess) e = launch_reduce_partials(op->partial, np, 1.0, m->S + S_DD, st);
+    if (e == hipSuccess) e = launch_cast(m->rho[0], DT_F32, ps->res[0], DT_F64, ps->npts[0], st);
+    return gpaw_check(e, "mixed: residual");
+}
+
+}  // namespace
+
+void gpaw_mixed_free(MixedState* m)
+{
+    if (!m) return;
+    for (hipGraphExec_t g : {m->g_vcycle, m->g_vwarm, m->g_a, m->g_b})
+        if (g) hipGraphExecDestroy(g);
+    for (auto v : {&m->phiA, &m->phiB, &m->rho, &m->res})
+        for (float* p : *v)
+            if (p) hipFree(p);
+    for (double* p : {m->p, m->q, m->S, m->partial})
+        if (p) hipFree(p);
+    if (m->h_S) hipHostFree(m->h_S);
+    if (m->d_coarse32) hipFree(m->d_coarse32);
+    if (m->cap) hipStreamDestroy(m->cap);
+    delete m;
+}
+
+extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const double* rho,
+                                        double rel_tol, int maxiter, int mode, int inner,
+                                        int* niter, double* relres, void* stream)
+{
+    if (!ps || !phi || !rho || maxiter < 1 || rel_tol <= 0.0 || (mode != 0 && mode != 1) ||
+        inner < 1)
+        return gpaw_fail(GPAW_EINVAL, "mixed: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    int rc = setup(ps);
+    if (!rc) rc = capture_all(ps);
+    if (rc) return rc;
+    MixedState* m = ps->mixed;
+    const i64 n = ps->npts[0];
+    hipError_t e = launch_copy(ps->rho[0], rho, DT_F64, n, st);
+    if (e == hipSuccess) e = launch_copy(ps->phiA[0], phi, DT_F64, n, st);
+    if (e == hipSuccess) {   // |rho|^2 for the relative criterion
+        hipLaunchKernelGGL(k_dot, gridn(n), dim3(256), 0, st, ps->rho[0], ps->rho[0], n, m->partial);
+        hipLaunchKernelGGL(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x, m->S,
+                           (int)S_DZOLD);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) return gpaw_check(e, "mixed: load");
+    double rr = 0.0, dd = 0.0;
+    rc = read_scalar(m, S_DZOLD, st, &rr);
+    if (rc) return rc;
+    const double target = rel_tol * rel_tol * rr;
+    rc = residual64(ps, m, st);
+    if (!rc) rc = read_scalar(m, S_DD, st, &dd);
+    int it = 0;
+    if (!rc && mode == 1 && dd > target) {
+        // z = V32(d); dz = d.z; p = z
+        rc = gpaw_check(hipGraphLaunch(m->g_vcycle, st), "mixed: V-cycle");
+        if (!rc) {
+            hipLaunchKernelGGL(k_dot_dz_zq, gridn(n), dim3(256), 0, st, ps->res[0], z_of(m),
+                               ps->res[0], n, m->partial);
+            hipLaunchKernelGGL(k_red2, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x,
+                               m->S);
+            hipLaunchKernelGGL(k_p_update, gridn(n), dim3(256), 0, st, m->p, z_of(m), m->S, n, 1);
+            rc = gpaw_check(hipGetLastError(), "mixed: start");
+        }
+        while (!rc && dd > target && it < maxiter) {
+            rc = gpaw_check(hipGraphLaunch(m->g_a, st), "mixed: iteration");
+            if (!rc) rc = read_scalar(m, S_DD, st, &dd);
+            it++;
+            if (!rc && dd > target && it < maxiter)
+                rc = gpaw_check(hipGraphLaunch(m->g_b, st), "mixed: preconditioner");
+        }
+    } else if (!rc && mode == 0) {
+        while (!rc && dd > target && it < maxiter) {
+            rc = gpaw_check(hipGraphLaunch(m->g_vcycle, st), "mixed: V-cycle");
+            for (int k = 1; k < inner && !rc; k++)
+                rc = gpaw_check(hipGraphLaunch(m->g_vwarm, st), "mixed: V-cycle");
+            if (!rc) {
+                hipLaunchKernelGGL(k_sub_f32, gridn(n), dim3(256), 0, st, ps->phiA[0], z_of(m), n);
+                rc = gpaw_check(hipGetLastError(), "mixed: correction");
+            }
+            if (!rc) rc = residual64(ps, m, st);
+            if (!rc) rc = read_scalar(m, S_DD, st, &dd);
+            it++;
+        }
+    }
+    if (rc) return rc;
+    rc = gpaw_check(launch_copy(phi, ps->phiA[0], DT_F64, n, st), "mixed: store");
+    if (niter) *niter = it;
+    if (relres) *relres = rr > 0 ? sqrt(dd / rr) : 0.0;
+    if (rc) return rc;
+    if (dd > target)
+        return gpaw_fail(GPAW_ENOCONV, "mixed-precision Poisson solver did not reach %g in %d "
+                                       "iterations", rel_tol, maxiter);
+    return GPAW_OK;
+}

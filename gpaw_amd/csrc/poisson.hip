@@ -21,31 +21,10 @@
 #include <cstring>
 #include <vector>
 
-#include "capi_internal.h"
-#include "coarse.h"
+#include "poisson_internal.h"
 
 using namespace gpaw;
 
-struct gpaw_poisson {
-    int L = 0;                 // number of coarsenings
-    int method = 2;
-    double dv = 1.0;
-    std::vector<gpaw_operator*> ops;
-    std::vector<gpaw_transformer*> rst, itp;
-    std::vector<int> pre, post;
-    std::vector<double> w;
-    std::vector<double*> phiA, phiB, rho, res;
-    std::vector<long long> npts;
-    double* d_err = nullptr;
-    double* h_err = nullptr;
-    hipStream_t cap = nullptr;
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
-    int use_graph = 1;
-    int swaps = 0;             // 1 if level 0 does an odd number of sweeps per cycle
-    int lc = -1;               // first level run by the fused coarse kernel (-1: none)
-    CoarsePlan* d_coarse = nullptr;
-    int coarse_lds = 0;        // dynamic LDS bytes of the fused kernel
-};
 
 static int setup_coarse(gpaw_poisson* ps);
 
@@ -294,6 +273,7 @@ static int setup_coarse(gpaw_poisson* ps)
             }
         }
     }
+    ps->h_coarse = P;
     hipError_t e = hipMalloc((void**)&ps->d_coarse, sizeof(CoarsePlan));
     if (e == hipSuccess) e = hipMemcpy(ps->d_coarse, &P, sizeof P, hipMemcpyHostToDevice);
     if (e != hipSuccess) return gpaw_check(e, "poisson: coarse plan");
@@ -369,6 +349,7 @@ extern "C" void gpaw_poisson_free(gpaw_poisson* ps)
             if (p) hipFree(p);
     if (ps->d_err) hipFree(ps->d_err);
     if (ps->d_coarse) hipFree(ps->d_coarse);
+    if (ps->mixed) gpaw_mixed_free(ps->mixed);
     if (ps->h_err) hipHostFree(ps->h_err);
     if (ps->cap) hipStreamDestroy(ps->cap);
     delete ps;

@@ -1,0 +1,34 @@
+// Layout of the device Poisson plan, shared by the FP64 V-cycle
+// (poisson.hip) and the mixed-precision solver (mixed.hip).
+#pragma once
+#include <vector>
+
+#include "capi_internal.h"
+#include "coarse.h"
+
+struct MixedState;
+
+struct gpaw_poisson {
+    int L = 0;                 // number of coarsenings
+    int method = 2;
+    double dv = 1.0;
+    std::vector<gpaw_operator*> ops;
+    std::vector<gpaw_transformer*> rst, itp;
+    std::vector<int> pre, post;
+    std::vector<double> w;
+    std::vector<double*> phiA, phiB, rho, res;
+    std::vector<long long> npts;
+    double* d_err = nullptr;
+    double* h_err = nullptr;
+    hipStream_t cap = nullptr;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    int use_graph = 1;
+    int swaps = 0;             // 1 if level 0 does an odd number of sweeps per cycle
+    int lc = -1;               // first level run by the fused coarse kernel (-1: none)
+    gpaw::CoarsePlan* d_coarse = nullptr;
+    int coarse_lds = 0;        // dynamic LDS bytes of the fused kernel
+    gpaw::CoarsePlan h_coarse; // host copy of the coarse plan
+    MixedState* mixed = nullptr;
+};
+
+void gpaw_mixed_free(MixedState* m);

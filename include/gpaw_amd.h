@@ -125,6 +125,20 @@ int gpaw_poisson_solve_neutral(gpaw_poisson* ps, double* phi,
 int gpaw_poisson_vcycles(gpaw_poisson* ps, double* phi, const double* rho,
                          int ncycles, double* errors, void* stream);
 
+/* Mixed-precision solve of the same problem (no reference counterpart: the
+ * FP32 / FP64 split of the configuration "mixed-precision multigrid
+ * Poisson").  FP64 outer loop (residual L phi - rho, correction
+ * accumulation, dot products, convergence) around FP32 V-cycles
+ * (smoothing, residuals, restriction, interpolation in FP32; the fused
+ * coarsest levels in FP64).  mode 0: iterative refinement with `inner`
+ * V-cycles per step; mode 1: flexible preconditioned CG with one V-cycle
+ * per iteration.  Stops when ||L phi - rho||_2 <= rel_tol * ||rho||_2
+ * (rho as given, already neutral); *relres receives the final ratio. */
+int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi,
+                             const double* rho, double rel_tol, int maxiter,
+                             int mode, int inner, int* niter, double* relres,
+                             void* stream);
+
 void gpaw_poisson_free(gpaw_poisson* ps);
 
 /* -------------------------------------------------------------- helpers */
