@@ -135,7 +135,7 @@ extern "C" int gpaw_operator_new(const double* coefs, const long* offsets, int n
         if (offsets[c] == 0) op->diag = coefs[c];
     s.star_r = star_range(s);
     hipGetDevice(&op->device);
-    if (!s.star_r) {
+    if (!s.star_r) {   // device copy of the stencil for the generic kernel
         hipError_t e = hipMalloc(&s.d_coefs, sizeof(double) * ncoefs);
         if (e == hipSuccess) e = hipMalloc(&s.d_off, sizeof(int) * 3 * ncoefs);
         if (e == hipSuccess)
@@ -232,11 +232,23 @@ extern "C" int gpaw_operator_relax(gpaw_operator* op, int method, double* f, con
     if (!op->real) return gpaw_fail(GPAW_EINVAL, "relax: operator is complex");
     if (!f || !s) return gpaw_fail(GPAW_EINVAL, "relax: NULL array");
     if (nrelax <= 0) return GPAW_OK;
-    if (method == 1)
-        return gpaw_fail(GPAW_EUNSUPPORTED, "relax: Gauss-Seidel (method 1) is not implemented");
-    if (method != 2) return gpaw_fail(GPAW_EINVAL, "relax: unknown method %d", method);
+    if (method != 1 && method != 2) return gpaw_fail(GPAW_EINVAL, "relax: unknown method %d", method);
     int rc = gpaw_op_ensure_scratch(op);
     if (rc) return rc;
+    if (method == 1) {
+        if (!op->st.d_coefs) {   // star stencils upload their table on first use
+            hipError_t e = hipMalloc(&op->st.d_coefs, sizeof(double) * op->st.nc);
+            if (e == hipSuccess) e = hipMalloc(&op->st.d_off, sizeof(int) * 3 * op->st.nc);
+            if (e == hipSuccess)
+                e = hipMemcpy(op->st.d_coefs, op->st.coefs, sizeof(double) * op->st.nc,
+                              hipMemcpyHostToDevice);
+            if (e == hipSuccess)
+                e = hipMemcpy(op->st.d_off, op->st.off, sizeof(int) * 3 * op->st.nc,
+                              hipMemcpyHostToDevice);
+            if (e != hipSuccess) return gpaw_check(e, "relax: stencil upload");
+        }
+        return gpaw_op_gauss_seidel(op, f, s, nrelax, op->scratch, (hipStream_t)stream);
+    }
     return gpaw_op_jacobi(op, f, s, nrelax, w, op->scratch, (hipStream_t)stream);
 }
 

@@ -27,5 +27,7 @@ int gpaw_op_ensure_partials(gpaw_operator* op);
 int gpaw_op_ensure_scratch(gpaw_operator* op);
 int gpaw_op_jacobi(gpaw_operator* op, double* f, const double* s, int nrelax, double w,
                    double* scratch, hipStream_t st);
+int gpaw_op_gauss_seidel(gpaw_operator* op, double* f, const double* s, int nrelax,
+                         double* snap, hipStream_t st);
 int gpaw_op_residual(gpaw_operator* op, const double* phi, const double* rho, double* r,
                      double* norm2, double scale, double* partial, hipStream_t st);

@@ -42,6 +42,12 @@ hipError_t relax(gpaw_poisson* ps, int l, int k, double*& cur, double*& alt, hip
     Phases ph;
     gpaw_phases(nullptr, ph);
     gpaw_operator* op = ps->ops[l];
+    if (ps->method == 1) {   // Gauss-Seidel: in place, alt is the halo snapshot
+        if (k <= 0) return hipSuccess;
+        return gpaw_op_gauss_seidel(op, cur, ps->rho[l], k, alt, st) == GPAW_OK
+                   ? hipSuccess
+                   : hipErrorLaunchFailure;
+    }
     for (int it = 0; it < k; it++) {
         hipError_t e = launch_stencil(op->st, DT_F64, ST_JACOBI, cur, alt, ps->rho[l], op->g, ph,
                                       1, ps->w[l], nullptr, nullptr, true, st);
@@ -79,7 +85,8 @@ hipError_t emit_level(gpaw_poisson* ps, int l, double*& cur0, double*& alt0, hip
         // the coarse result sits in phiA[l+1] when the coarse level does an
         // even number of sweeps, else in phiB[l+1]
         const int kc = (l + 1 < ps->L ? ps->pre[l + 1] : 0) + ps->post[l + 1];
-        const double* coarse = (kc % 2) ? ps->phiB[l + 1] : ps->phiA[l + 1];
+        const double* coarse =
+            (ps->method == 2 && kc % 2) ? ps->phiB[l + 1] : ps->phiA[l + 1];
         e = launch_transfer(ps->itp[l]->t, DT_F64, TR_SUBTRACT, coarse, cur, nullptr, ph, 1, st);
         if (e != hipSuccess) return e;
     }
@@ -155,9 +162,8 @@ extern "C" int gpaw_poisson_new(int levels, gpaw_operator* const* ops,
     if (!out || !ops || !presmooths || !postsmooths || !weights || levels < 0 || levels > 16)
         return gpaw_fail(GPAW_EINVAL, "poisson: bad argument");
     *out = nullptr;
-    if (relax_method != 2)
-        return gpaw_fail(GPAW_EUNSUPPORTED,
-                         "poisson: only weighted Jacobi (method 2) runs on the device");
+    if (relax_method != 1 && relax_method != 2)
+        return gpaw_fail(GPAW_EINVAL, "poisson: unknown relaxation method %d", relax_method);
     if (levels > 0 && (!restrictors || !interpolators))
         return gpaw_fail(GPAW_EINVAL, "poisson: missing transformers");
     for (int l = 0; l <= levels; l++) {
@@ -188,7 +194,7 @@ extern "C" int gpaw_poisson_new(int levels, gpaw_operator* const* ops,
     ps->pre.assign(presmooths, presmooths + levels + 1);
     ps->post.assign(postsmooths, postsmooths + levels + 1);
     ps->w.assign(weights, weights + levels + 1);
-    ps->swaps = ((levels > 0 ? ps->pre[0] : 0) + ps->post[0]) % 2;
+    ps->swaps = relax_method == 2 ? ((levels > 0 ? ps->pre[0] : 0) + ps->post[0]) % 2 : 0;
     const char* env = getenv("GPAW_AMD_GRAPH");
     ps->use_graph = !(env && env[0] == '0');
     int rc = GPAW_OK;
@@ -207,7 +213,21 @@ extern "C" int gpaw_poisson_new(int levels, gpaw_operator* const* ops,
         ps->res.push_back(x);
     }
     if (!rc) rc = gpaw_op_ensure_partials(ps->ops[0]);
-    if (!rc) rc = setup_coarse(ps);
+    if (!rc && relax_method == 2) rc = setup_coarse(ps);
+    for (int l = 0; l <= levels && !rc && relax_method == 1; l++) {
+        gpaw_operator* op = ps->ops[l];
+        if (!op->st.d_coefs) {   // Gauss-Seidel reads the stencil table on the device
+            hipError_t e = hipMalloc(&op->st.d_coefs, sizeof(double) * op->st.nc);
+            if (e == hipSuccess) e = hipMalloc(&op->st.d_off, sizeof(int) * 3 * op->st.nc);
+            if (e == hipSuccess)
+                e = hipMemcpy(op->st.d_coefs, op->st.coefs, sizeof(double) * op->st.nc,
+                              hipMemcpyHostToDevice);
+            if (e == hipSuccess)
+                e = hipMemcpy(op->st.d_off, op->st.off, sizeof(int) * 3 * op->st.nc,
+                              hipMemcpyHostToDevice);
+            rc = gpaw_check(e, "poisson: stencil upload");
+        }
+    }
     if (!rc) rc = alloc(&ps->d_err, 1);
     if (!rc) rc = gpaw_check(hipHostMalloc((void**)&ps->h_err, sizeof(double)), "poisson: pinned");
     if (!rc) rc = gpaw_check(hipStreamCreateWithFlags(&ps->cap, hipStreamNonBlocking),

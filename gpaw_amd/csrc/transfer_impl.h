@@ -9,8 +9,10 @@
 // (gpaw/poisson.py:263-264), and restriction can zero the next level's phi
 // in the same pass (gpaw/poisson.py:259-261).
 #pragma once
+#include <cstdlib>
 #include "kernels.h"
 #include "transfer_common.h"
+#include "transfer_tile.h"
 
 namespace gpaw {
 
@@ -140,6 +142,24 @@ hipError_t tr_launch(const Transfer& tr, int mode, const void* in, void* out, vo
         else
             hipLaunchKernelGGL((k_interp2<V, TR_STORE>), g2, dim3(256), 0, st, (const V*)in,
                                (V*)out, t, nc[0], nc[1], nc[2], ph);
+        return hipGetLastError();
+    }
+    // wide transfers on large planes: LDS-tiled plane sweep (8 x 32 tiles)
+    if (K >= 4 && t.nout[1] >= 4 && t.nout[2] >= 16 && !getenv("GPAW_AMD_POINTWISE_TRANSFER")) {
+        TileTr tt;
+        tt.nt1 = (t.nout[1] + 7) / 8;
+        tt.nt2 = (t.nout[2] + 31) / 32;
+        dim3 gt((unsigned)(tt.nt1 * tt.nt2 * nin));
+        if (!tr.interpolate)
+            hipLaunchKernelGGL((k_restrict_tile<V, K, 8, 32>), gt, dim3(256), 0, st, (const V*)in,
+                               (V*)out, t, tt, ph);
+        else if (mode == TR_SUBTRACT)
+            hipLaunchKernelGGL((k_interp_tile<V, K, 8, 32, TR_SUBTRACT>), gt, dim3(256), 0, st,
+                               (const V*)in, (V*)out, t, tt, ph);
+        else
+            hipLaunchKernelGGL((k_interp_tile<V, K, 8, 32, TR_STORE>), gt, dim3(256), 0, st,
+                               (const V*)in, (V*)out, t, tt, ph);
+        if (!tr.interpolate && zero_out) return launch_fill(zero_out, VT<V>::cplx ? (sizeof(typename VT<V>::R) == 8 ? DT_C128 : DT_C64) : (sizeof(typename VT<V>::R) == 8 ? DT_F64 : DT_F32), t.ngout * nin, st);
         return hipGetLastError();
     }
     if (!tr.interpolate) {

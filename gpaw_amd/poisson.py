@@ -187,14 +187,13 @@ class PoissonSolver:
         return niter
 
     def solve_neutral(self, phi, rho, eps=2e-10):
-        if self.relax_method != 2:
-            raise NotImplementedError('Gauss-Seidel relaxation does not run '
-                                      'on the GPU path')
-        if self.B is not None:
-            raise NotImplementedError('Mehrstellen (nn="M") right-hand side '
-                                      'is not on the GPU path yet')
         P = Arg(phi, write=True, name='phi')
         R = Arg(rho, name='rho')
+        if self.B is not None:
+            # Mehrstellen: the right-hand side is B rho (gpaw/poisson.py:196-199)
+            brho = _arrays.torch().empty_like(R.tensor)
+            self.B.apply(R.tensor, brho)
+            R = Arg(brho, name='B rho')
         self.phis[0] = P.tensor
         niter = ctypes.c_int(0)
         err = ctypes.c_double(0.0)

@@ -252,6 +252,14 @@ class Server:
                         group.append(q)
             nbytes = sum(request_bytes(q) for q in group)
             if not self.adm.try_admit(need_plan, key, plan_bytes(*key), nbytes):
+                if self.busy_slots == 0 and not batches:
+                    # nothing runs and nothing can be evicted: it never fits
+                    r.error = ('needs %d device bytes, HBM budget is %d' %
+                               (nbytes + plan_bytes(*key), self.adm.budget))
+                    r.finish = self.clock() - self.t0
+                    self.waiting.popleft()
+                    self.done.append(r)
+                    continue
                 break
             for pid in self.adm.evicted:
                 self._drop_plan(pid)

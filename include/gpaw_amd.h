@@ -62,7 +62,9 @@ int gpaw_operator_apply(gpaw_operator* op, const void* in, void* out,
 
 /* Operator.relax(method, f, s, nrelax, w): Operator_relax,
  * c/operators.c:49-91 -> bmgs_relax, c/bmgs/relax.c:7-73.  float64 only;
- * f is updated in place.  method 2 = weighted Jacobi, 1 = Gauss-Seidel. */
+ * f is updated in place.  method 2 = weighted Jacobi, 1 = lexicographic
+ * Gauss-Seidel (star stencils and stencils of range 1; w is ignored, as in
+ * the reference). */
 int gpaw_operator_relax(gpaw_operator* op, int method, double* f,
                         const double* s, int nrelax, double w, void* stream);
 

@@ -60,8 +60,7 @@ def test_operator_golden(path):
         op = _fdop(d['coefs'], d['offsets_pc'], gd, float)
         f = dev(d['f0'])
         op.relax(int(d['method']), f, dev(d['s']), int(d['nrelax']),
-                 float(d['w'])) if int(d['method']) == 2 else pytest.skip(
-            'Gauss-Seidel is not on the GPU path')
+                 float(d['w']))
         assert np.array_equal(host(f), d['f'])
         return
     a = d['a']

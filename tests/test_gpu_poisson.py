@@ -135,3 +135,54 @@ def test_numpy_phi_is_updated():
     ref = og.zeros()
     ops.solve(ref, rho.copy())
     assert rel_maxnorm(phi, ref) < 1e-13
+
+
+@pytest.mark.parametrize('relax,nn,pbc', [('GS', 1, True), ('GS', 3, False),
+                                          ('J', 'M', False)])
+def test_solver_variants(relax, nn, pbc):
+    """Gauss-Seidel smoothing and the Mehrstellen operator: same niter and
+    potential as the oracle's restatement."""
+    print('case variant', relax, nn, pbc, flush=True)
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.poisson import PoissonSolver
+    N, h = 16, 0.3
+    og = O.Grid((N,) * 3, (N * h,) * 3, pbc)
+    gd = GridDescriptor((N,) * 3, (N * h,) * 3, pbc)
+    rho = _density(og, 11, 2, 0.7, 1.0, 1.2)
+    rho -= og.integrate(rho) / og.dv / og.size_of_global_array().prod()
+    ops = O.PoissonSolver(nn=nn, relax=relax)
+    ops.set_grid_descriptor(og)
+    ops.initialize()
+    ref = og.zeros()
+    n_ref = ops.solve_neutral(ref, rho.copy(), eps=1e-14)
+    gps = PoissonSolver(nn=nn, relax=relax)
+    gps.set_grid_descriptor(gd)
+    gps.initialize()
+    phi = dev(np.zeros(tuple(og.n_c)))
+    n = gps.solve_neutral(phi, dev(rho), eps=1e-14)
+    assert n == n_ref
+    got = phi.cpu().numpy()
+    if pbc:
+        assert rel_maxnorm(got, ref) < 1e-13
+    else:
+        assert np.array_equal(got, ref)
+
+
+def test_reference_gs_symmetry():
+    """gpaw/test/poisson.py with relax='GS' (the reference's own bound)."""
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.poisson import PoissonSolver
+    N = 16
+    L = 2.87 / 0.529177
+    og = O.Grid((N, N, N), (L, L, L))
+    gd = GridDescriptor((N, N, N), (L, L, L))
+    f = O.gaussian_charges(og, [(0.0, 0.0, 0.0)], [0.8], [1.0])
+    rho = f + np.roll(f, (8, 8, 8), (0, 1, 2))
+    rho -= og.integrate(rho) / L ** 3
+    ps = PoissonSolver(nn=1, relax='GS')
+    ps.set_grid_descriptor(gd)
+    ps.initialize()
+    phi = dev(np.zeros((N, N, N)))
+    ps.solve(phi, dev(rho), charge=0)
+    b = phi.cpu().numpy()
+    assert abs(b[0, 0, 0] - b[8, 8, 8]) < 3e-6
