@@ -43,6 +43,13 @@ PROTOTYPES = {
     'gpaw_poisson_solve_mixed': (_i, [_vp, _vp, _vp, _d, _i, _i, _i, _ip, _dp,
                                       _vp]),
     'gpaw_poisson_free': (None, [_vp]),
+    'gpaw_preconditioner_new': (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                     ctypes.POINTER(_vp)]),
+    'gpaw_preconditioner_step': (_d, [_vp]),
+    'gpaw_preconditioner_workspace': (_i, [_vp, _l, _i, ctypes.POINTER(_l)]),
+    'gpaw_preconditioner_apply': (_i, [_vp, _vp, _vp, _l, _i, _dp, _vp, _l,
+                                       _vp]),
+    'gpaw_preconditioner_free': (None, [_vp]),
     'gpaw_sum': (_i, [_vp, _l, _d, _vp, _vp]),
     'gpaw_axpby': (_i, [_vp, _vp, _d, _d, _l, _vp]),
     'gpaw_sub_scalar': (_i, [_vp, _vp, _d, _l, _vp]),

@@ -151,7 +151,7 @@ extern "C" int gpaw_operator_new(const double* coefs, const long* offsets, int n
     return GPAW_OK;
 }
 
-static int check_dtype(int real, int dtype)
+int gpaw_check_dtype(int real, int dtype)
 {
     if (dtype < GPAW_F64 || dtype > GPAW_C64) return gpaw_fail(GPAW_EINVAL, "unknown dtype %d", dtype);
     if (real && dtype_complex(dtype))
@@ -185,7 +185,7 @@ extern "C" int gpaw_operator_apply(gpaw_operator* op, const void* in, void* out,
                                    int dtype, const double* phases, void* stream)
 {
     if (!op) return gpaw_fail(GPAW_EINVAL, "operator: NULL handle");
-    int rc = check_dtype(op->real, dtype);
+    int rc = gpaw_check_dtype(op->real, dtype);
     if (rc) return rc;
     if (nin < 0 || nin > 65535) return gpaw_fail(GPAW_EINVAL, "operator: nin = %ld", nin);
     if (nin == 0) return GPAW_OK;
@@ -372,7 +372,7 @@ extern "C" int gpaw_transformer_apply(gpaw_transformer* tr, const void* in, void
                                       int dtype, const double* phases, void* stream)
 {
     if (!tr) return gpaw_fail(GPAW_EINVAL, "transformer: NULL handle");
-    int rc = check_dtype(tr->real, dtype);
+    int rc = gpaw_check_dtype(tr->real, dtype);
     if (rc) return rc;
     if (nin < 0 || nin > 65535) return gpaw_fail(GPAW_EINVAL, "transformer: nin = %ld", nin);
     if (nin == 0) return GPAW_OK;

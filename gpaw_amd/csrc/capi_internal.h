@@ -23,6 +23,7 @@ struct gpaw_transformer {
 int gpaw_fail(int code, const char* fmt, ...);
 int gpaw_check(hipError_t e, const char* what);
 void gpaw_phases(const double* p, gpaw::Phases& ph);
+int gpaw_check_dtype(int real, int dtype);
 int gpaw_op_ensure_partials(gpaw_operator* op);
 int gpaw_op_ensure_scratch(gpaw_operator* op);
 int gpaw_op_jacobi(gpaw_operator* op, double* f, const double* s, int nrelax, double w,

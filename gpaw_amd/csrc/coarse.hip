@@ -180,14 +180,14 @@ hipError_t coarse_set_lds_limit(int bytes)
 
 hipError_t launch_coarse_vcycle(const CoarsePlan* d_plan, int lds_bytes, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_coarse_vcycle<double>, dim3(1), dim3(1024), lds_bytes, st, d_plan);
-    return hipGetLastError();
+    launch_kernel(k_coarse_vcycle<double>, dim3(1), dim3(1024), lds_bytes, st, d_plan);
+    return launch_error();
 }
 
 hipError_t launch_coarse_vcycle_f32(const CoarsePlan* d_plan, int lds_bytes, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_coarse_vcycle<float>, dim3(1), dim3(1024), lds_bytes, st, d_plan);
-    return hipGetLastError();
+    launch_kernel(k_coarse_vcycle<float>, dim3(1), dim3(1024), lds_bytes, st, d_plan);
+    return launch_error();
 }
 
 }  // namespace gpaw

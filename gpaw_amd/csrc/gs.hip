@@ -94,9 +94,9 @@ int gpaw_op_gauss_seidel(gpaw_operator* op, double* f, const double* s, int nrel
             if (e != hipSuccess) return gpaw_check(e, "relax: snapshot");
         }
         for (int t = 0; t <= tmax; t++)
-            hipLaunchKernelGGL(k_gs_plane, dim3(nb), dim3(256), 0, st, f, snap, s, g,
+            launch_kernel(k_gs_plane, dim3(nb), dim3(256), 0, st, f, snap, s, g,
                                op->st.d_coefs, op->st.d_off, op->st.nc, w0, w1, t, inv0);
-        hipError_t e = hipGetLastError();
+        hipError_t e = launch_error();
         if (e != hipSuccess) return gpaw_check(e, "relax: Gauss-Seidel sweep");
     }
     return GPAW_OK;

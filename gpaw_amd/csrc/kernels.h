@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include "device_common.h"
+#include "launch.h"
 
 namespace gpaw {
 
@@ -14,7 +15,43 @@ enum StencilMode {
     ST_APPLY = 0,    // out = sum_c coef_c * in[g + off_c]
     ST_RESID = 1,    // out = apply(in) - aux
     ST_JACOBI = 2,   // out = (1-w) in + w (aux - sum_{c>=1} ...) / coef_0
+    ST_PCUPD = 3,    // out = in - w (apply(in) - aux)     (FP64 types only)
+    ST_PCFIN = 4,    // out = in - w (apply(in) + aux)     (FP64 types only)
 };
+
+// Per-point result of a stencil pass from the stencil sum x (over c >= 1 for
+// ST_JACOBI, over all c otherwise), the centre input value f, the aux value
+// s, the weight w and the centre coefficient c0.  Each operation is rounded
+// on its own, in the order the reference evaluates it:
+//   ST_JACOBI  c/bmgs/relax.c:60;
+//   ST_PCUPD   q = L d; q -= r; d -= w * q         (gpaw/preconditioner.py:56-66);
+//   ST_PCFIN   q = L d; q -= res; d += -w * q; d *= -1 evaluated on e = -d
+//              (gpaw/preconditioner.py:67-70: negation commutes exactly with
+//              the linear stencil and with rounding, so the value equals
+//              e - w * (L e + res)).
+template <int MODE, class V, class RT>
+__device__ __forceinline__ V st_epilogue(V x, V f, V s, RT w, RT c0)
+{
+    if (MODE == ST_APPLY) return x;
+    if (MODE == ST_RESID) return vsub(x, s);
+    if (MODE == ST_JACOBI) return vadd(vmul(f, (RT)1 - w), vdiv_r(vmul(vsub(s, x), w), c0));
+    if (MODE == ST_PCUPD) return vsub(f, vmul(vsub(x, s), w));
+    return vsub(f, vmul(vadd(x, s), w));
+}
+
+// Which (type, mode, norm) combinations are compiled.
+template <class V, int MODE>
+constexpr bool st_mode_ok()
+{
+    if (MODE == ST_PCUPD || MODE == ST_PCFIN) return sizeof(typename VT<V>::R) == 8;
+    if (MODE == ST_JACOBI) return !VT<V>::cplx;
+    return true;
+}
+template <class V, int MODE>
+constexpr bool st_norm_ok()
+{
+    return !VT<V>::cplx && (MODE == ST_APPLY || MODE == ST_RESID || MODE == ST_JACOBI);
+}
 
 #define GPAW_MAXC 512
 #define GPAW_MAXR 6

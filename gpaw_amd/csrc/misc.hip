@@ -5,6 +5,12 @@
 
 namespace gpaw {
 
+GraphRecorder*& graph_recorder()
+{
+    static thread_local GraphRecorder* rec = nullptr;
+    return rec;
+}
+
 __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restrict__ p, int n,
                                                          double scale, double* __restrict__ out)
 {
@@ -67,8 +73,8 @@ static dim3 grid1(i64 n) { return dim3((unsigned)((n + 255) / 256)); }
 hipError_t launch_reduce_partials(const double* partial, int n, double scale, double* result,
                                   hipStream_t st)
 {
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, partial, n, scale, result);
-    return hipGetLastError();
+    launch_kernel(k_reduce_partials, dim3(1), dim3(256), 0, st, partial, n, scale, result);
+    return launch_error();
 }
 
 hipError_t launch_sum(const double* a, long long n, double* partial, int* npartial, hipStream_t st)
@@ -77,22 +83,22 @@ hipError_t launch_sum(const double* a, long long n, double* partial, int* nparti
     if (nb > 1024) nb = 1024;
     if (nb < 1) nb = 1;
     *npartial = nb;
-    hipLaunchKernelGGL(k_sum, dim3(nb), dim3(256), 0, st, a, (i64)n, partial);
-    return hipGetLastError();
+    launch_kernel(k_sum, dim3(nb), dim3(256), 0, st, a, (i64)n, partial);
+    return launch_error();
 }
 
 hipError_t launch_axpby(double* y, const double* x, double a, double b, long long n, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_axpby, grid1(n), dim3(256), 0, st, y, x, a, b, (i64)n);
-    return hipGetLastError();
+    launch_kernel(k_axpby, grid1(n), dim3(256), 0, st, y, x, a, b, (i64)n);
+    return launch_error();
 }
 
 hipError_t launch_sub_scalar(double* y, const double* x, double b, long long n, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sub_scalar, grid1(n), dim3(256), 0, st, y, x, b, (i64)n);
-    return hipGetLastError();
+    launch_kernel(k_sub_scalar, grid1(n), dim3(256), 0, st, y, x, b, (i64)n);
+    return launch_error();
 }
 
 hipError_t launch_fill(void* y, int dtype, long long n, hipStream_t st)
@@ -101,10 +107,10 @@ hipError_t launch_fill(void* y, int dtype, long long n, hipStream_t st)
     i64 m = (i64)n * (dtype_complex(dtype) ? 2 : 1);
     if (m <= 0) return hipSuccess;
     if (dtype == DT_F64 || dtype == DT_C128)
-        hipLaunchKernelGGL(k_fill<double>, grid1(m), dim3(256), 0, st, (double*)y, m);
+        launch_kernel(k_fill<double>, grid1(m), dim3(256), 0, st, (double*)y, m);
     else
-        hipLaunchKernelGGL(k_fill<float>, grid1(m), dim3(256), 0, st, (float*)y, m);
-    return hipGetLastError();
+        launch_kernel(k_fill<float>, grid1(m), dim3(256), 0, st, (float*)y, m);
+    return launch_error();
 }
 
 hipError_t launch_copy(void* y, const void* x, int dtype, long long n, hipStream_t st)
@@ -112,12 +118,12 @@ hipError_t launch_copy(void* y, const void* x, int dtype, long long n, hipStream
     i64 m = (i64)n * (dtype_complex(dtype) ? 2 : 1);
     if (m <= 0) return hipSuccess;
     if (dtype == DT_F64 || dtype == DT_C128)
-        hipLaunchKernelGGL(k_copy<double>, grid1(m), dim3(256), 0, st, (double*)y,
+        launch_kernel(k_copy<double>, grid1(m), dim3(256), 0, st, (double*)y,
                            (const double*)x, m);
     else
-        hipLaunchKernelGGL(k_copy<float>, grid1(m), dim3(256), 0, st, (float*)y,
+        launch_kernel(k_copy<float>, grid1(m), dim3(256), 0, st, (float*)y,
                            (const float*)x, m);
-    return hipGetLastError();
+    return launch_error();
 }
 
 hipError_t launch_cast(void* y, int dty, const void* x, int dtx, long long n, hipStream_t st)
@@ -127,14 +133,14 @@ hipError_t launch_cast(void* y, int dty, const void* x, int dtx, long long n, hi
     if (m <= 0) return hipSuccess;
     const bool yd = (dty == DT_F64 || dty == DT_C128), xd = (dtx == DT_F64 || dtx == DT_C128);
     if (yd && !xd)
-        hipLaunchKernelGGL((k_cast<double, float>), grid1(m), dim3(256), 0, st, (double*)y,
+        launch_kernel((k_cast<double, float>), grid1(m), dim3(256), 0, st, (double*)y,
                            (const float*)x, m);
     else if (!yd && xd)
-        hipLaunchKernelGGL((k_cast<float, double>), grid1(m), dim3(256), 0, st, (float*)y,
+        launch_kernel((k_cast<float, double>), grid1(m), dim3(256), 0, st, (float*)y,
                            (const double*)x, m);
     else
         return launch_copy(y, x, dty, n, st);
-    return hipGetLastError();
+    return launch_error();
 }
 
 }  // namespace gpaw

@@ -40,7 +40,6 @@ struct MixedState {
     double* h_S = nullptr;              // pinned readback
     double* partial = nullptr;
     int nblk = 0;
-    hipStream_t cap = nullptr;
     hipGraphExec_t g_vcycle = nullptr;  // one FP32 V-cycle from zero
     hipGraphExec_t g_vwarm = nullptr;   // one FP32 V-cycle from the last result
     hipGraphExec_t g_a = nullptr;       // PCG: q = Lp, alpha, update, |d|^2
@@ -213,21 +212,9 @@ hipError_t emit32(gpaw_poisson* ps, MixedState* m, int l, bool zero0, hipStream_
 float* z_of(MixedState* m) { return m->phiA[0]; }   // even level-0 sweep count
 
 template <class F>
-int capture_graph(hipStream_t cap, hipGraphExec_t* out, F&& body)
+int capture_graph(hipGraphExec_t* out, F&& body)
 {
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) return gpaw_check(e, "mixed: begin capture");
-    hipError_t e1 = body(cap);
-    e = hipStreamEndCapture(cap, &g);
-    if (e1 != hipSuccess) {
-        if (g) hipGraphDestroy(g);
-        return gpaw_check(e1, "mixed: captured launch");
-    }
-    if (e != hipSuccess) return gpaw_check(e, "mixed: end capture");
-    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
-    return gpaw_check(e, "mixed: instantiate");
+    return gpaw_check(build_graph(out, body), "mixed: graph");
 }
 
 int setup(gpaw_poisson* ps)
@@ -263,7 +250,6 @@ int setup(gpaw_poisson* ps)
     if (!rc) rc = gpaw_check(hipMalloc((void**)&m->S, sizeof(double) * S_N), "mixed: scalars");
     if (!rc) rc = gpaw_check(hipHostMalloc((void**)&m->h_S, sizeof(double) * S_N), "mixed: pinned");
     if (!rc) rc = gpaw_check(hipMalloc((void**)&m->partial, sizeof(double) * 2 * NB), "mixed: partials");
-    if (!rc) rc = gpaw_check(hipStreamCreateWithFlags(&m->cap, hipStreamNonBlocking), "mixed: stream");
     if (!rc) rc = gpaw_check(hipMemset(m->S, 0, sizeof(double) * S_N), "mixed: scalars");
     // the fused FP64 coarse solve needs its arrays in LDS (FP32 boundary)
     if (!rc && ps->lc > 0 && ps->h_coarse.lds) {
@@ -279,44 +265,53 @@ int setup(gpaw_poisson* ps)
     return rc;
 }
 
+void destroy_graphs(MixedState* m)
+{
+    for (hipGraphExec_t* g : {&m->g_vcycle, &m->g_vwarm, &m->g_a, &m->g_b}) {
+        if (*g) hipGraphExecDestroy(*g);
+        *g = nullptr;
+    }
+}
+
 int capture_all(gpaw_poisson* ps)
 {
     MixedState* m = ps->mixed;
-    if (m->g_vcycle) return GPAW_OK;
+    if (m->g_b) return GPAW_OK;   // the last one built: all four exist
     const i64 n = ps->npts[0];
     gpaw_operator* op = ps->ops[0];
-    int rc = capture_graph(m->cap, &m->g_vcycle,
+    int rc = capture_graph(&m->g_vcycle,
                            [&](hipStream_t st) { return emit32(ps, m, 0, true, st); });
     if (!rc)
-        rc = capture_graph(m->cap, &m->g_vwarm,
+        rc = capture_graph(&m->g_vwarm,
                            [&](hipStream_t st) { return emit32(ps, m, 0, false, st); });
     if (!rc)
-        rc = capture_graph(m->cap, &m->g_a, [&](hipStream_t st) -> hipError_t {
+        rc = capture_graph(&m->g_a, [&](hipStream_t st) -> hipError_t {
             Phases ph;
             gpaw_phases(nullptr, ph);
             hipError_t e = launch_stencil(op->st, DT_F64, ST_APPLY, m->p, m->q, nullptr, op->g,
                                           ph, 1, 0.0, nullptr, nullptr, true, st);
             if (e != hipSuccess) return e;
-            hipLaunchKernelGGL(k_dot, gridn(n), dim3(256), 0, st, m->p, m->q, n, m->partial);
-            hipLaunchKernelGGL(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x,
+            launch_kernel(k_dot, gridn(n), dim3(256), 0, st, m->p, m->q, n, m->partial);
+            launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x,
                                m->S, (int)S_PQ);
-            hipLaunchKernelGGL(k_pcg_update, gridn(n), dim3(256), 0, st, ps->phiA[0], ps->res[0],
+            launch_kernel(k_pcg_update, gridn(n), dim3(256), 0, st, ps->phiA[0], ps->res[0],
                                m->p, m->q, m->rho[0], m->S, n, m->partial);
-            hipLaunchKernelGGL(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x,
+            launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x,
                                m->S, (int)S_DD);
-            return hipGetLastError();
+            return launch_error();
         });
     if (!rc)
-        rc = capture_graph(m->cap, &m->g_b, [&](hipStream_t st) -> hipError_t {
+        rc = capture_graph(&m->g_b, [&](hipStream_t st) -> hipError_t {
             hipError_t e = emit32(ps, m, 0, true, st);
             if (e != hipSuccess) return e;
-            hipLaunchKernelGGL(k_dot_dz_zq, gridn(n), dim3(256), 0, st, ps->res[0], z_of(m), m->q,
+            launch_kernel(k_dot_dz_zq, gridn(n), dim3(256), 0, st, ps->res[0], z_of(m), m->q,
                                n, m->partial);
-            hipLaunchKernelGGL(k_red2, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x,
+            launch_kernel(k_red2, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x,
                                m->S);
-            hipLaunchKernelGGL(k_p_update, gridn(n), dim3(256), 0, st, m->p, z_of(m), m->S, n, 0);
-            return hipGetLastError();
+            launch_kernel(k_p_update, gridn(n), dim3(256), 0, st, m->p, z_of(m), m->S, n, 0);
+            return launch_error();
         });
+    if (rc) destroy_graphs(m);   // retried from scratch on the next call
     return rc;
 }
 
@@ -357,7 +352,6 @@ void gpaw_mixed_free(MixedState* m)
         if (p) hipFree(p);
     if (m->h_S) hipHostFree(m->h_S);
     if (m->d_coarse32) hipFree(m->d_coarse32);
-    if (m->cap) hipStreamDestroy(m->cap);
     delete m;
 }
 
@@ -377,10 +371,10 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
     hipError_t e = launch_copy(ps->rho[0], rho, DT_F64, n, st);
     if (e == hipSuccess) e = launch_copy(ps->phiA[0], phi, DT_F64, n, st);
     if (e == hipSuccess) {   // |rho|^2 for the relative criterion
-        hipLaunchKernelGGL(k_dot, gridn(n), dim3(256), 0, st, ps->rho[0], ps->rho[0], n, m->partial);
-        hipLaunchKernelGGL(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x, m->S,
+        launch_kernel(k_dot, gridn(n), dim3(256), 0, st, ps->rho[0], ps->rho[0], n, m->partial);
+        launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x, m->S,
                            (int)S_DZOLD);
-        e = hipGetLastError();
+        e = launch_error();
     }
     if (e != hipSuccess) return gpaw_check(e, "mixed: load");
     double rr = 0.0, dd = 0.0;
@@ -394,12 +388,12 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
         // z = V32(d); dz = d.z; p = z
         rc = gpaw_check(hipGraphLaunch(m->g_vcycle, st), "mixed: V-cycle");
         if (!rc) {
-            hipLaunchKernelGGL(k_dot_dz_zq, gridn(n), dim3(256), 0, st, ps->res[0], z_of(m),
+            launch_kernel(k_dot_dz_zq, gridn(n), dim3(256), 0, st, ps->res[0], z_of(m),
                                ps->res[0], n, m->partial);
-            hipLaunchKernelGGL(k_red2, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x,
+            launch_kernel(k_red2, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x,
                                m->S);
-            hipLaunchKernelGGL(k_p_update, gridn(n), dim3(256), 0, st, m->p, z_of(m), m->S, n, 1);
-            rc = gpaw_check(hipGetLastError(), "mixed: start");
+            launch_kernel(k_p_update, gridn(n), dim3(256), 0, st, m->p, z_of(m), m->S, n, 1);
+            rc = gpaw_check(launch_error(), "mixed: start");
         }
         while (!rc && dd > target && it < maxiter) {
             rc = gpaw_check(hipGraphLaunch(m->g_a, st), "mixed: iteration");
@@ -414,8 +408,8 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
             for (int k = 1; k < inner && !rc; k++)
                 rc = gpaw_check(hipGraphLaunch(m->g_vwarm, st), "mixed: V-cycle");
             if (!rc) {
-                hipLaunchKernelGGL(k_sub_f32, gridn(n), dim3(256), 0, st, ps->phiA[0], z_of(m), n);
-                rc = gpaw_check(hipGetLastError(), "mixed: correction");
+                launch_kernel(k_sub_f32, gridn(n), dim3(256), 0, st, ps->phiA[0], z_of(m), n);
+                rc = gpaw_check(launch_error(), "mixed: correction");
             }
             if (!rc) rc = residual64(ps, m, st);
             if (!rc) rc = read_scalar(m, S_DD, st, &dd);

@@ -115,19 +115,15 @@ int capture(gpaw_poisson* ps)
 {
     if (ps->exec[0]) return GPAW_OK;
     for (int s = 0; s < 1 + ps->swaps; s++) {
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamBeginCapture(ps->cap, hipStreamCaptureModeThreadLocal);
-        if (e != hipSuccess) return gpaw_check(e, "poisson: begin capture");
-        hipError_t e1 = emit_cycle(ps, s, ps->cap);
-        e = hipStreamEndCapture(ps->cap, &g);
-        if (e1 != hipSuccess) {
-            if (g) hipGraphDestroy(g);
-            return gpaw_check(e1, "poisson: captured launch");
+        hipError_t e = build_graph(&ps->exec[s],
+                                   [&](hipStream_t st) { return emit_cycle(ps, s, st); });
+        if (e != hipSuccess) {
+            for (int t = 0; t < s; t++) {
+                hipGraphExecDestroy(ps->exec[t]);
+                ps->exec[t] = nullptr;
+            }
+            return gpaw_check(e, "poisson: V-cycle graph");
         }
-        if (e != hipSuccess) return gpaw_check(e, "poisson: end capture");
-        e = hipGraphInstantiate(&ps->exec[s], g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
-        if (e != hipSuccess) return gpaw_check(e, "poisson: instantiate");
     }
     return GPAW_OK;
 }
@@ -230,8 +226,6 @@ extern "C" int gpaw_poisson_new(int levels, gpaw_operator* const* ops,
     }
     if (!rc) rc = alloc(&ps->d_err, 1);
     if (!rc) rc = gpaw_check(hipHostMalloc((void**)&ps->h_err, sizeof(double)), "poisson: pinned");
-    if (!rc) rc = gpaw_check(hipStreamCreateWithFlags(&ps->cap, hipStreamNonBlocking),
-                             "poisson: capture stream");
     if (rc) {
         gpaw_poisson_free(ps);
         return rc;
@@ -371,6 +365,5 @@ extern "C" void gpaw_poisson_free(gpaw_poisson* ps)
     if (ps->d_coarse) hipFree(ps->d_coarse);
     if (ps->mixed) gpaw_mixed_free(ps->mixed);
     if (ps->h_err) hipHostFree(ps->h_err);
-    if (ps->cap) hipStreamDestroy(ps->cap);
     delete ps;
 }

@@ -20,7 +20,6 @@ struct gpaw_poisson {
     std::vector<long long> npts;
     double* d_err = nullptr;
     double* h_err = nullptr;
-    hipStream_t cap = nullptr;
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     int use_graph = 1;
     int swaps = 0;             // 1 if level 0 does an odd number of sweeps per cycle

@@ -123,16 +123,8 @@ __global__ __launch_bounds__(256) void k_star(const V* __restrict__ in, V* __res
             for (int k = 1; k <= R; k++)      // axis 0: -k
                 x = vadd(x, vmul(qv[R - k], (RT)sc.c[5 * R + k]));
             const i64 idx = boff + (i64)i0 * plane + row + i2;
-            V r;
-            if (MODE == ST_APPLY) {
-                r = x;
-            } else if (MODE == ST_RESID) {
-                r = vsub(x, aux[idx]);
-            } else {
-                // (1 - w) * f + w * (s - x) / coef_0, c/bmgs/relax.c:60
-                const RT wr = (RT)w;
-                r = vadd(vmul(qv[R], (RT)1 - wr), vdiv_r(vmul(vsub(aux[idx], x), wr), (RT)sc.c[0]));
-            }
+            const V r = st_epilogue<MODE>(x, qv[R], MODE == ST_APPLY ? vzero<V>() : aux[idx],
+                                          (RT)w, (RT)sc.c[0]);
             if (NORM) acc += norm2(r);
             if (WRITE) out[idx] = r;
 #pragma unroll
@@ -169,15 +161,9 @@ __global__ __launch_bounds__(256) void k_generic(const V* __restrict__ in, V* __
             x = vadd(x, vmul(ghost_fetch(a, i0 + off[3 * c], i1 + off[3 * c + 1],
                                          i2 + off[3 * c + 2], g, ph),
                              (RT)coefs[c]));
-        V r;
-        if (MODE == ST_APPLY) {
-            r = x;
-        } else if (MODE == ST_RESID) {
-            r = vsub(x, aux[boff + t]);
-        } else {
-            const RT wr = (RT)w;
-            r = vadd(vmul(a[t], (RT)1 - wr), vdiv_r(vmul(vsub(aux[boff + t], x), wr), (RT)coefs[0]));
-        }
+        const V r = st_epilogue<MODE>(x, MODE == ST_APPLY || MODE == ST_RESID ? vzero<V>() : a[t],
+                                      MODE == ST_APPLY ? vzero<V>() : aux[boff + t], (RT)w,
+                                      (RT)coefs[0]);
         if (NORM) acc = norm2(r);
         if (WRITE) out[boff + t] = r;
     }
@@ -238,18 +224,20 @@ hipError_t star_launch(const Stencil& s, const void* in, void* out, const void* 
     const V* i = (const V*)in;
     V* o = (V*)out;
     const V* x = (const V*)aux;
-    if (partial) {
-        if (write)
-            hipLaunchKernelGGL((k_star<V, R, MODE, true, true>), sh.grid, sh.block, 0, st, i, o, x,
-                               g, ph, sc, w, sh.q, sh.ntz, ng, partial);
-        else
-            hipLaunchKernelGGL((k_star<V, R, MODE, true, false>), sh.grid, sh.block, 0, st, i, o,
-                               x, g, ph, sc, w, sh.q, sh.ntz, ng, partial);
-    } else {
-        hipLaunchKernelGGL((k_star<V, R, MODE, false, true>), sh.grid, sh.block, 0, st, i, o, x,
-                           g, ph, sc, w, sh.q, sh.ntz, ng, partial);
+    if constexpr (st_norm_ok<V, MODE>()) {
+        if (partial) {
+            if (write)
+                launch_kernel((k_star<V, R, MODE, true, true>), sh.grid, sh.block, 0, st, i, o,
+                                   x, g, ph, sc, w, sh.q, sh.ntz, ng, partial);
+            else
+                launch_kernel((k_star<V, R, MODE, true, false>), sh.grid, sh.block, 0, st, i,
+                                   o, x, g, ph, sc, w, sh.q, sh.ntz, ng, partial);
+            return launch_error();
+        }
     }
-    return hipGetLastError();
+    launch_kernel((k_star<V, R, MODE, false, true>), sh.grid, sh.block, 0, st, i, o, x, g, ph,
+                       sc, w, sh.q, sh.ntz, ng, partial);
+    return launch_error();
 }
 
 template <class V, int MODE>
@@ -278,18 +266,34 @@ hipError_t generic_launch(const Stencil& s, const void* in, void* out, const voi
     const V* i = (const V*)in;
     V* o = (V*)out;
     const V* x = (const V*)aux;
-    if (partial) {
-        if (write)
-            hipLaunchKernelGGL((k_generic<V, MODE, true, true>), grid, dim3(256), 0, st, i, o, x,
-                               g, ph, s.d_coefs, s.d_off, s.nc, w, ng, partial);
-        else
-            hipLaunchKernelGGL((k_generic<V, MODE, true, false>), grid, dim3(256), 0, st, i, o,
-                               x, g, ph, s.d_coefs, s.d_off, s.nc, w, ng, partial);
-    } else {
-        hipLaunchKernelGGL((k_generic<V, MODE, false, true>), grid, dim3(256), 0, st, i, o, x, g,
-                           ph, s.d_coefs, s.d_off, s.nc, w, ng, partial);
+    if constexpr (st_norm_ok<V, MODE>()) {
+        if (partial) {
+            if (write)
+                launch_kernel((k_generic<V, MODE, true, true>), grid, dim3(256), 0, st, i, o,
+                                   x, g, ph, s.d_coefs, s.d_off, s.nc, w, ng, partial);
+            else
+                launch_kernel((k_generic<V, MODE, true, false>), grid, dim3(256), 0, st, i, o,
+                                   x, g, ph, s.d_coefs, s.d_off, s.nc, w, ng, partial);
+            return launch_error();
+        }
     }
-    return hipGetLastError();
+    launch_kernel((k_generic<V, MODE, false, true>), grid, dim3(256), 0, st, i, o, x, g, ph,
+                       s.d_coefs, s.d_off, s.nc, w, ng, partial);
+    return launch_error();
+}
+
+template <class V, int MODE>
+hipError_t kind_launch(const Stencil& s, const void* in, void* out, const void* aux,
+                       const Geom& g, const Phases& ph, int nin, double w, double* partial,
+                       bool write, hipStream_t st)
+{
+    if constexpr (!st_mode_ok<V, MODE>()) {
+        return hipErrorInvalidValue;
+    } else {
+        if (partial && !st_norm_ok<V, MODE>()) return hipErrorInvalidValue;
+        if (s.star_r > 0) return star_dispatch_r<V, MODE>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+        return generic_launch<V, MODE>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+    }
 }
 
 template <class V>
@@ -297,21 +301,14 @@ hipError_t dispatch_mode(const Stencil& s, int mode, const void* in, void* out, 
                          const Geom& g, const Phases& ph, int nin, double w, double* partial,
                          bool write, hipStream_t st)
 {
-    if constexpr (VT<V>::cplx) {
-        // complex data: plain apply only (relax and residuals are real-only
-        // in the reference, c/operators.c:49-91)
-        if (mode != ST_APPLY || partial) return hipErrorInvalidValue;
-        if (s.star_r > 0) return star_dispatch_r<V, ST_APPLY>(s, in, out, aux, g, ph, nin, w, nullptr, true, st);
-        return generic_launch<V, ST_APPLY>(s, in, out, aux, g, ph, nin, w, nullptr, true, st);
+    switch (mode) {
+    case ST_APPLY: return kind_launch<V, ST_APPLY>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+    case ST_RESID: return kind_launch<V, ST_RESID>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+    case ST_JACOBI: return kind_launch<V, ST_JACOBI>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+    case ST_PCUPD: return kind_launch<V, ST_PCUPD>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+    case ST_PCFIN: return kind_launch<V, ST_PCFIN>(s, in, out, aux, g, ph, nin, w, partial, write, st);
     }
-    if (s.star_r > 0) {
-        if (mode == ST_APPLY) return star_dispatch_r<V, ST_APPLY>(s, in, out, aux, g, ph, nin, w, partial, write, st);
-        if (mode == ST_RESID) return star_dispatch_r<V, ST_RESID>(s, in, out, aux, g, ph, nin, w, partial, write, st);
-        return star_dispatch_r<V, ST_JACOBI>(s, in, out, aux, g, ph, nin, w, partial, write, st);
-    }
-    if (mode == ST_APPLY) return generic_launch<V, ST_APPLY>(s, in, out, aux, g, ph, nin, w, partial, write, st);
-    if (mode == ST_RESID) return generic_launch<V, ST_RESID>(s, in, out, aux, g, ph, nin, w, partial, write, st);
-    return generic_launch<V, ST_JACOBI>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace

@@ -187,15 +187,8 @@ __global__ __launch_bounds__(256) void k_star_tile(const V* __restrict__ in, V* 
             for (int k = 1; k <= R; k++) x = vadd(x, vmul(q[j][R + k], (RT)sc.c[4 * R + k]));
 #pragma unroll
             for (int k = 1; k <= R; k++) x = vadd(x, vmul(q[j][R - k], (RT)sc.c[5 * R + k]));
-            V r;
-            if (MODE == ST_APPLY) {
-                r = x;
-            } else if (MODE == ST_RESID) {
-                r = vsub(x, s[j]);
-            } else {
-                const RT wr = (RT)w;
-                r = vadd(vmul(q[j][R], (RT)1 - wr), vdiv_r(vmul(vsub(s[j], x), wr), (RT)sc.c[0]));
-            }
+            const V r = st_epilogue<MODE>(x, q[j][R], MODE == ST_APPLY ? vzero<V>() : s[j], (RT)w,
+                                          (RT)sc.c[0]);
             if (inr[j]) {
                 if (NORM) acc += tnorm2(r);
                 if (WRITE) out[boff + (i64)i0 * plane + (i64)i1r[j] * n2 + i2] = r;

@@ -14,10 +14,29 @@ hipError_t tile_go(const Stencil& s, const void* in, void* out, const void* aux,
     TileCoefs sc;
     for (int c = 0; c < 6 * R + 1; c++) sc.c[c] = s.coefs[c];
     const i64 ng = (i64)g.n[0] * g.n[1] * g.n[2];
-    hipLaunchKernelGGL((k_star_tile<V, R, TZ, ROWS, MODE, NORM, WRITE>), dim3(sh.nblocks),
+    launch_kernel((k_star_tile<V, R, TZ, ROWS, MODE, NORM, WRITE>), dim3(sh.nblocks),
                        dim3(TZ, 256 / TZ), 0, st, (const V*)in, (V*)out, (const V*)aux, g, ph, sc,
                        w, sh, ng, partial);
-    return hipGetLastError();
+    return launch_error();
+}
+
+template <class V, int R, int TZ, int MODE>
+hipError_t tile_mode(const Stencil& s, const void* in, void* out, const void* aux, const Geom& g,
+                     const Phases& ph, double w, const TileShape& sh, double* partial, bool write,
+                     hipStream_t st)
+{
+    if constexpr (!st_mode_ok<V, MODE>()) {
+        return hipErrorInvalidValue;
+    } else {
+        if constexpr (st_norm_ok<V, MODE>()) {
+            if (partial && write)
+                return tile_go<V, R, TZ, MODE, true, true>(s, in, out, aux, g, ph, w, sh, partial, st);
+            if (partial)
+                return tile_go<V, R, TZ, MODE, true, false>(s, in, out, aux, g, ph, w, sh, partial, st);
+        }
+        if (partial) return hipErrorInvalidValue;
+        return tile_go<V, R, TZ, MODE, false, true>(s, in, out, aux, g, ph, w, sh, nullptr, st);
+    }
 }
 
 template <class V, int R, int TZ>
@@ -25,20 +44,17 @@ hipError_t tile_modes(const Stencil& s, int mode, const void* in, void* out, con
                       const Geom& g, const Phases& ph, double w, const TileShape& sh,
                       double* partial, bool write, hipStream_t st)
 {
-    if constexpr (VT<V>::cplx) {
-        if (mode != ST_APPLY || partial) return hipErrorInvalidValue;
-        return tile_go<V, R, TZ, ST_APPLY, false, true>(s, in, out, aux, g, ph, w, sh, nullptr, st);
-    } else {
-        if (mode == ST_APPLY)
-            return tile_go<V, R, TZ, ST_APPLY, false, true>(s, in, out, aux, g, ph, w, sh, nullptr, st);
-        if (mode == ST_JACOBI)
-            return tile_go<V, R, TZ, ST_JACOBI, false, true>(s, in, out, aux, g, ph, w, sh, nullptr, st);
-        if (!partial)
-            return tile_go<V, R, TZ, ST_RESID, false, true>(s, in, out, aux, g, ph, w, sh, nullptr, st);
-        if (write)
-            return tile_go<V, R, TZ, ST_RESID, true, true>(s, in, out, aux, g, ph, w, sh, partial, st);
-        return tile_go<V, R, TZ, ST_RESID, true, false>(s, in, out, aux, g, ph, w, sh, partial, st);
+    // residual norms are only taken in the Poisson loop (RESID); other modes
+    // never carry partials there
+    if (partial && mode != ST_RESID) return hipErrorInvalidValue;
+    switch (mode) {
+    case ST_APPLY: return tile_mode<V, R, TZ, ST_APPLY>(s, in, out, aux, g, ph, w, sh, nullptr, true, st);
+    case ST_RESID: return tile_mode<V, R, TZ, ST_RESID>(s, in, out, aux, g, ph, w, sh, partial, write, st);
+    case ST_JACOBI: return tile_mode<V, R, TZ, ST_JACOBI>(s, in, out, aux, g, ph, w, sh, nullptr, true, st);
+    case ST_PCUPD: return tile_mode<V, R, TZ, ST_PCUPD>(s, in, out, aux, g, ph, w, sh, nullptr, true, st);
+    case ST_PCFIN: return tile_mode<V, R, TZ, ST_PCFIN>(s, in, out, aux, g, ph, w, sh, nullptr, true, st);
     }
+    return hipErrorInvalidValue;
 }
 
 template <class V>

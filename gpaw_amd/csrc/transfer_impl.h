@@ -137,12 +137,12 @@ hipError_t tr_launch(const Transfer& tr, int mode, const void* in, void* out, vo
         const i64 ncell = (i64)nc[0] * nc[1] * nc[2];
         dim3 g2((unsigned)((ncell + 255) / 256), nin, 1);
         if (mode == TR_SUBTRACT)
-            hipLaunchKernelGGL((k_interp2<V, TR_SUBTRACT>), g2, dim3(256), 0, st, (const V*)in,
+            launch_kernel((k_interp2<V, TR_SUBTRACT>), g2, dim3(256), 0, st, (const V*)in,
                                (V*)out, t, nc[0], nc[1], nc[2], ph);
         else
-            hipLaunchKernelGGL((k_interp2<V, TR_STORE>), g2, dim3(256), 0, st, (const V*)in,
+            launch_kernel((k_interp2<V, TR_STORE>), g2, dim3(256), 0, st, (const V*)in,
                                (V*)out, t, nc[0], nc[1], nc[2], ph);
-        return hipGetLastError();
+        return launch_error();
     }
     // wide transfers on large planes: LDS-tiled plane sweep (8 x 32 tiles)
     if (K >= 4 && t.nout[1] >= 4 && t.nout[2] >= 16 && !getenv("GPAW_AMD_POINTWISE_TRANSFER")) {
@@ -151,28 +151,28 @@ hipError_t tr_launch(const Transfer& tr, int mode, const void* in, void* out, vo
         tt.nt2 = (t.nout[2] + 31) / 32;
         dim3 gt((unsigned)(tt.nt1 * tt.nt2 * nin));
         if (!tr.interpolate)
-            hipLaunchKernelGGL((k_restrict_tile<V, K, 8, 32>), gt, dim3(256), 0, st, (const V*)in,
+            launch_kernel((k_restrict_tile<V, K, 8, 32>), gt, dim3(256), 0, st, (const V*)in,
                                (V*)out, t, tt, ph);
         else if (mode == TR_SUBTRACT)
-            hipLaunchKernelGGL((k_interp_tile<V, K, 8, 32, TR_SUBTRACT>), gt, dim3(256), 0, st,
+            launch_kernel((k_interp_tile<V, K, 8, 32, TR_SUBTRACT>), gt, dim3(256), 0, st,
                                (const V*)in, (V*)out, t, tt, ph);
         else
-            hipLaunchKernelGGL((k_interp_tile<V, K, 8, 32, TR_STORE>), gt, dim3(256), 0, st,
+            launch_kernel((k_interp_tile<V, K, 8, 32, TR_STORE>), gt, dim3(256), 0, st,
                                (const V*)in, (V*)out, t, tt, ph);
         if (!tr.interpolate && zero_out) return launch_fill(zero_out, VT<V>::cplx ? (sizeof(typename VT<V>::R) == 8 ? DT_C128 : DT_C64) : (sizeof(typename VT<V>::R) == 8 ? DT_F64 : DT_F32), t.ngout * nin, st);
-        return hipGetLastError();
+        return launch_error();
     }
     if (!tr.interpolate) {
-        hipLaunchKernelGGL((k_restrict<V, K>), grid, dim3(256), 0, st, (const V*)in, (V*)out,
+        launch_kernel((k_restrict<V, K>), grid, dim3(256), 0, st, (const V*)in, (V*)out,
                            (V*)zero_out, t, ph);
     } else if (mode == TR_SUBTRACT) {
-        hipLaunchKernelGGL((k_interpolate<V, K, TR_SUBTRACT>), grid, dim3(256), 0, st,
+        launch_kernel((k_interpolate<V, K, TR_SUBTRACT>), grid, dim3(256), 0, st,
                            (const V*)in, (V*)out, t, ph);
     } else {
-        hipLaunchKernelGGL((k_interpolate<V, K, TR_STORE>), grid, dim3(256), 0, st, (const V*)in,
+        launch_kernel((k_interpolate<V, K, TR_STORE>), grid, dim3(256), 0, st, (const V*)in,
                            (V*)out, t, ph);
     }
-    return hipGetLastError();
+    return launch_error();
 }
 
 template <class V>

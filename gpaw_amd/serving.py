@@ -353,7 +353,8 @@ def synthetic_stream(n=2000, rate=200.0, seed=0, edges=(48, 64, 80, 96, 128, 160
     """Seeded Poisson-arrival stream: 80 % Poisson solves, 20 % Laplace
     applies on 1-64 bands; cubic edges from `edges`, half periodic.
     Densities are sums of 4 neutral grid-normalised Gaussians (widths
-    0.5-1.5 A, >= 2 A from the walls); bands are normal(0, 1).  Each
+    0.5-1.5 A, >= 2 A from the walls, or a quarter of the cell on grids
+    smaller than 16 A); bands are normal(0, 1).  Each
     (edge, boundary) key draws from `variants` seeded densities and one
     64-band block (requests use its first nb bands), generated once and
     kept in HBM when device=True -- the served data are resident, the
@@ -371,8 +372,10 @@ def synthetic_stream(n=2000, rate=200.0, seed=0, edges=(48, 64, 80, 96, 128, 160
         if k not in dens:
             gd = GridDescriptor((N,) * 3, (N * 0.2 / BOHR,) * 3, pbc)
             r2 = np.random.default_rng(hash(k) % 2 ** 32)
+            # 2 A from the walls, or a quarter of the cell on small grids
+            margin = min(2.0, 0.25 * N * 0.2) / BOHR
             c, w, q = random_gaussian_charges(gd, r2, 4, 0.5 / BOHR, 1.5 / BOHR,
-                                              2.0 / BOHR)
+                                              margin)
             d = gaussian_density(gd, c, w, q)
             dens[k] = torch.from_numpy(d).to('cuda') if device else d
         return dens[k]

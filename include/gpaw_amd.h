@@ -143,6 +143,45 @@ int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi,
 
 void gpaw_poisson_free(gpaw_poisson* ps);
 
+/* ---------------------------------------------------------- preconditioner
+ * Replaces the Python class Preconditioner (gpaw/preconditioner.py:13-70),
+ * whose __call__ drives _gpaw.Operator / _gpaw.Transformer objects and
+ * numpy updates band-block by band-block; here the whole sequence runs on
+ * the device for all nin bands in ten launches.  kin0 is the fine-grid
+ * kinetic operator, kin1 / kin2 the nn=1 Laplacians (scale -0.5) of the two
+ * coarser grids, restrictor0/1 and interpolator2/1 the nn=1 transformers
+ * 0->1, 1->2, 2->1, 1->0 (gpaw/preconditioner.py:14-31).  All are borrowed
+ * and must outlive the preconditioner.  step = 0.66666666 / kin0's
+ * diagonal element (gpaw/preconditioner.py:24). */
+typedef struct gpaw_preconditioner gpaw_preconditioner;
+
+int gpaw_preconditioner_new(gpaw_operator* kin0, gpaw_operator* kin1,
+                            gpaw_operator* kin2, gpaw_transformer* restrictor0,
+                            gpaw_transformer* restrictor1,
+                            gpaw_transformer* interpolator2,
+                            gpaw_transformer* interpolator1,
+                            gpaw_preconditioner** pc);
+
+double gpaw_preconditioner_step(const gpaw_preconditioner* pc);
+
+/* Device workspace (bytes) that gpaw_preconditioner_apply needs for nin
+ * bands of dtype (GPAW_F64 or GPAW_C128): one fine and six coarse grid
+ * functions per band (the reference's scratch0/1/2 arrays). */
+int gpaw_preconditioner_workspace(const gpaw_preconditioner* pc, long nin,
+                                  int dtype, long* bytes);
+
+/* Preconditioner.__call__(residuals, kpt): out = the preconditioned
+ * residuals for nin bands with the k-point's phase_cd (12 doubles in host
+ * memory, NULL = Gamma).  out must not alias residuals; work is a device
+ * buffer of at least gpaw_preconditioner_workspace bytes, 256-byte
+ * aligned, owned by the caller (no allocation happens in this call). */
+int gpaw_preconditioner_apply(gpaw_preconditioner* pc, const void* residuals,
+                              void* out, long nin, int dtype,
+                              const double* phases, void* work,
+                              long work_bytes, void* stream);
+
+void gpaw_preconditioner_free(gpaw_preconditioner* pc);
+
 /* -------------------------------------------------------------- helpers */
 /* Deterministic FP64 sum of n doubles times scale into *result (device):
  * gd.integrate / comm.sum of the reference (gpaw/grid_descriptor.py:197-208). */

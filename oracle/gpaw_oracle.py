@@ -607,3 +607,69 @@ class PoissonSolver:
             self.operators[level].apply(self.phis[level], residual)
             residual -= self.rhos[level]
             return np.dot(residual.ravel(), residual.ravel()) * self.dv
+
+
+# ---------------------------------------------------------------------------
+# Eigensolver preconditioner
+# ---------------------------------------------------------------------------
+class Preconditioner:
+    """Three-grid preconditioner of gpaw/preconditioner.py:13-70 over the
+    oracle kernels, with numpy doing the element-wise updates.
+
+    The reference's final BLAS axpy (y <- y + a x, c/blas.c:197-218) is
+    evaluated here as y + (a x) with two roundings; a BLAS built with fused
+    multiply-add may differ from it in the last bit of that one step.
+    """
+
+    def __init__(self, grid0, kin0, dtype=float, kernels=None):
+        grid1 = grid0.coarsen()
+        grid2 = grid1.coarsen()
+        self.kin0 = kin0
+        self.kin1 = laplace(grid1, -0.5, 1, dtype, kernels)
+        self.kin2 = laplace(grid2, -0.5, 1, dtype, kernels)
+        self.grids = (grid0, grid1, grid2)
+        self.dtype = dtype
+        self.step = 0.66666666 / kin0.get_diagonal_element()
+        self.down = (Transformer(grid0, grid1, 1, dtype, kernels),
+                     Transformer(grid1, grid2, 1, dtype, kernels))
+        self.up = (Transformer(grid2, grid1, 1, dtype, kernels),
+                   Transformer(grid1, grid0, 1, dtype, kernels))
+
+    def __call__(self, residuals, phase_cd=None):
+        """residuals: (nbands, n0, n1, n2); returns the preconditioned
+        residuals as a new array."""
+        s = self.step
+        nb = len(residuals)
+        dt = residuals.dtype
+        shape1 = (nb,) + tuple(self.grids[1].n_c)
+        shape2 = (nb,) + tuple(self.grids[2].n_c)
+        # level 1: restrict the negated residuals
+        r1 = np.empty(shape1, dt)
+        self.down[0].apply(np.ascontiguousarray(-residuals), r1, phase_cd)
+        d1 = (4 * s) * r1
+        q1 = np.empty(shape1, dt)
+        self.kin1.apply(d1, q1, phase_cd)
+        q1 -= r1
+        # level 2: one damped step on the restricted defect
+        r2 = np.empty(shape2, dt)
+        self.down[1].apply(q1, r2, phase_cd)
+        d2 = (16 * s) * r2
+        q2 = np.empty(shape2, dt)
+        self.kin2.apply(d2, q2, phase_cd)
+        q2 -= r2
+        d2 -= (16 * s) * q2
+        # back to level 1: correct, then one more damped step
+        self.up[0].apply(d2, q1, phase_cd)
+        d1 -= q1
+        self.kin1.apply(d1, q1, phase_cd)
+        q1 -= r1
+        d1 -= (4 * s) * q1
+        # level 0
+        d0 = np.empty(residuals.shape, dt)
+        self.up[1].apply(np.ascontiguousarray(-d1), d0, phase_cd)
+        q0 = np.empty(residuals.shape, dt)
+        self.kin0.apply(d0, q0, phase_cd)
+        q0 -= residuals
+        d0 += (-s) * q0
+        d0 *= -1.0
+        return d0

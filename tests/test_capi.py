@@ -132,3 +132,44 @@ def test_poisson_hierarchy_shape():
     assert ps.presmooths == [2, 4, 4, 4, 4] and ps.postsmooths == [1, 4, 4, 4, 4]
     assert ps.weights == [2.0 / 3.0, 1.0, 1.0, 1.0, 1.0]
     assert [o.npoints for o in ps.operators] == [19, 7, 7, 7, 7]
+
+
+def test_preconditioner_plan_and_workspace():
+    """Levels chain, the step is the reference's, the workspace is one fine
+    and six coarse grid functions per band, and bad calls fail before any
+    device work."""
+    import ctypes
+    from gpaw_amd import GpawAmdError, lib
+    from gpaw_amd._lib import check, GPAW_C128, GPAW_F32
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.fd_operators import Laplace
+    from gpaw_amd.preconditioner import Preconditioner
+    gd = GridDescriptor((32, 24, 16), (6.0, 4.5, 3.0), True)
+    kin = Laplace(gd, -0.5, 3, complex)
+    pc = Preconditioner(gd, kin, complex, block=4)
+    pc.allocate()
+    h = pc._plan()
+    L = lib()
+    assert L.gpaw_preconditioner_step(h) == 0.66666666 / kin.get_diagonal_element()
+    nb = ctypes.c_long()
+    check(L.gpaw_preconditioner_workspace(h, 3, GPAW_C128, ctypes.byref(nb)))
+
+    def sl(n):
+        return (n * 3 * 16 + 255) // 256 * 256
+    assert nb.value == sl(32 * 24 * 16) + 3 * sl(16 * 12 * 8) + 3 * sl(8 * 6 * 4)
+    with pytest.raises(GpawAmdError):    # FP32 data
+        check(L.gpaw_preconditioner_workspace(h, 3, GPAW_F32, ctypes.byref(nb)))
+    with pytest.raises(GpawAmdError):    # NULL arrays
+        check(L.gpaw_preconditioner_apply(h, None, None, 1, GPAW_C128, None,
+                                          None, 0, None))
+    # levels that do not chain
+    other = Laplace(GridDescriptor((16, 24, 16), (3.0, 4.5, 3.0)), -0.5, 1,
+                    complex)
+    from gpaw_amd.preconditioner import _handle
+    out = ctypes.c_void_p()
+    rc = L.gpaw_preconditioner_new(
+        _handle(kin), _handle(other), _handle(pc.kin2),
+        _handle(pc.restrictor_object0), _handle(pc.restrictor_object1),
+        _handle(pc.interpolator_object2), _handle(pc.interpolator_object1),
+        ctypes.byref(out))
+    assert rc != 0 and b'chain' in L.gpaw_last_error()

@@ -168,8 +168,16 @@ def test_solver_variants(relax, nn, pbc):
         assert np.array_equal(got, ref)
 
 
-def test_reference_gs_symmetry():
-    """gpaw/test/poisson.py with relax='GS' (the reference's own bound)."""
+@pytest.mark.parametrize('relax', ['J', 'GS'])
+def test_reference_symmetry_case(relax):
+    """The setting of gpaw/test/poisson.py: two equal charges at (0,0,0) and
+    (1/2,1/2,1/2) of a periodic 16^3 cell, nn=1.  The reference test builds
+    them from a spline (LFC, not part of this path), here from Gaussians.
+    Jacobi smoothing is mirror-symmetric, so the reference's bound
+    |phi(0) - phi(centre)| < 6e-16 holds for any such density; Gauss-Seidel
+    sweeps break the symmetry by an amount that depends on the density, so
+    GS is compared with the oracle on this one instead."""
+    print('case symmetry', relax, flush=True)
     from gpaw_amd.grid_descriptor import GridDescriptor
     from gpaw_amd.poisson import PoissonSolver
     N = 16
@@ -179,10 +187,17 @@ def test_reference_gs_symmetry():
     f = O.gaussian_charges(og, [(0.0, 0.0, 0.0)], [0.8], [1.0])
     rho = f + np.roll(f, (8, 8, 8), (0, 1, 2))
     rho -= og.integrate(rho) / L ** 3
-    ps = PoissonSolver(nn=1, relax='GS')
+    ps = PoissonSolver(nn=1, relax=relax)
     ps.set_grid_descriptor(gd)
     ps.initialize()
     phi = dev(np.zeros((N, N, N)))
-    ps.solve(phi, dev(rho), charge=0)
+    n = ps.solve(phi, dev(rho), charge=0)
     b = phi.cpu().numpy()
-    assert abs(b[0, 0, 0] - b[8, 8, 8]) < 3e-6
+    if relax == 'J':
+        assert abs(b[0, 0, 0] - b[8, 8, 8]) < 6e-16
+    ops = O.PoissonSolver(nn=1, relax=relax)
+    ops.set_grid_descriptor(og)
+    ops.initialize()
+    ref = og.zeros()
+    assert n == ops.solve(ref, rho.copy(), charge=0)
+    assert rel_maxnorm(b, ref) < 1e-13

@@ -126,7 +126,7 @@ def test_results_identical_alone_and_in_mix():
         pytest.skip('no GPU')
     from gpaw_amd.serving import synthetic_stream
     print('case serving mix vs alone', flush=True)
-    reqs = synthetic_stream(n=40, rate=2000.0, seed=3, edges=(16, 24, 32),
+    reqs = synthetic_stream(n=40, rate=2000.0, seed=3, edges=(24, 32, 48),
                             max_bands=8)
     mix = Server(slots=4, hbm_budget=8 << 30)
     rep = mix.run(reqs, realtime=True)
