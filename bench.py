@@ -11,11 +11,17 @@ Workload (a "step" = one full Poisson solve from phi = 0):
 Also measured (same process, not the headline value):
   * roofline of the dominant kernel of the solve (level-0 Jacobi sweep),
     timed with HIP events on the launch stream;
-  * FD Laplace (nn=3, complex128, 256 bands x 64^3, Bloch phases) HBM GB/s;
+  * FD Laplace (nn=3, complex128, 256 bands x 64^3, Bloch phases) HBM GB/s,
+    and the K=6 restriction / interpolation of the same bands;
+  * the eigensolver preconditioner on the same bands;
+  * mixed-precision vs FP64 multigrid on 160^3 (159^3 arrays), h = 0.18 A;
+  * a serving run: Poisson-process stream of mixed Poisson / Laplace
+    requests, p50 / p99 latency;
   * the reference's own C kernels (oracle/_ref, built from the reference
     sources) driven by the restated V-cycle, on one host core.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
+                [--only fd_laplace|preconditioner|mixed_precision|serving|copy]
 """
 import argparse
 import json
@@ -40,6 +46,9 @@ def parse():
     p.add_argument('--cpu-seconds', type=float, default=12.0)
     p.add_argument('--serve-n', type=int, default=2000)
     p.add_argument('--serve-rate', type=float, default=300.0)
+    p.add_argument('--only', choices=sorted(SECTIONS),
+                   help='run one measurement section alone and print its '
+                        'JSON (for profiling a single workload)')
     return p.parse_args()
 
 
@@ -148,6 +157,37 @@ def fd_laplace_c3(torch, np):
     res['transfer_bytes'] = tb
     del a, b, c
     return res
+
+
+def preconditioner_c3(torch, np):
+    """Eigensolver preconditioner (gpaw/preconditioner.py) on the C3 bands:
+    256 complex128 bands of 64^3, periodic, Bloch phases, nn=3 kinetic
+    operator on the fine grid.  Algorithmic bytes per call: every array a
+    launch reads or writes counted once per launch, 16 B per point:
+    5 fine + 14 coarse-1 + 7 coarse-2 grid functions per band."""
+    from gpaw_amd.fd_operators import Laplace
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.preconditioner import Preconditioner
+    N, nb = 64, 256
+    gd = GridDescriptor((N,) * 3, (N * 0.2 / BOHR,) * 3, True)
+    k_c = np.random.default_rng(5).uniform(-0.5, 0.5, 3)
+
+    class Kpt:
+        phase_cd = gd.phase_cd(k_c)
+    pc = Preconditioner(gd, Laplace(gd, -0.5, 3, complex), complex, nb)
+    pc.allocate()
+    g = torch.Generator(device='cuda').manual_seed(13)
+    r = torch.randn((nb, N, N, N), dtype=torch.complex128, device='cuda',
+                    generator=g)
+    t = event_time(torch, lambda: pc(r, Kpt), 5)
+    n0 = N ** 3
+    algo = 16.0 * nb * (5 * n0 + 14 * n0 / 8 + 7 * n0 / 64)
+    del r
+    return {'config': 'Preconditioner(nn=3 kin), complex128, 256 bands x '
+                      '64^3, Bloch phases',
+            'avg_call_ms': round(t * 1e3, 3),
+            'GBps': round(algo / t / 1e9, 1),
+            'bytes_per_call': algo}
 
 
 def mixed_vs_fp64(torch, np, N=160, h=0.18, count=32, reps=3):
@@ -273,6 +313,16 @@ def cpu_baseline(np, gd_shape_args, rho, eps, budget):
                       (n, niter, dt)}
 
 
+SECTIONS = {
+    'fd_laplace': lambda torch, np, args: fd_laplace_c3(torch, np),
+    'preconditioner': lambda torch, np, args: preconditioner_c3(torch, np),
+    'mixed_precision': lambda torch, np, args: mixed_vs_fp64(torch, np),
+    'serving': lambda torch, np, args: serving(torch, np, n=args.serve_n,
+                                               rate=args.serve_rate),
+    'copy': lambda torch, np, args: {'copy_GBps': copy_bandwidth(torch)},
+}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -284,6 +334,10 @@ def main():
     import torch
     from gpaw_amd.poisson import PoissonSolver
 
+    if args.only:
+        print(json.dumps({args.only: SECTIONS[args.only](torch, np, args)}),
+              flush=True)
+        return
     gd, rho = c2_inputs(np)
     eps, rho_norm = eps_for(np, gd, rho)
     ps = PoissonSolver(nn=3, relax='J', eps=eps)
@@ -338,6 +392,7 @@ def main():
     out['roofline']['copy_GBps_measured'] = copy_bandwidth(torch)
     if not args.no_extra:
         out['fd_laplace'] = fd_laplace_c3(torch, np)
+        out['preconditioner'] = preconditioner_c3(torch, np)
         out['mixed_precision'] = mixed_vs_fp64(torch, np)
         out['serving'] = serving(torch, np, n=args.serve_n, rate=args.serve_rate)
     if not args.no_cpu:

@@ -328,7 +328,7 @@ static int forced_kind()
     return f;
 }
 
-bool tile_shape(const Geom& g, int R, int nin, int& TZ, TileShape& sh)
+bool tile_shape(const Geom& g, int R, int esize, int nin, int& TZ, TileShape& sh)
 {
     if (R < 1 || R > 4) return false;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
@@ -337,7 +337,7 @@ bool tile_shape(const Geom& g, int R, int nin, int& TZ, TileShape& sh)
         const int w64 = ((n2 + 63) / 64) * 64 - n2, w32 = ((n2 + 31) / 32) * 32 - n2;
         if (w32 * 100 < w64 * 100 - 5 * n2) TZ = 32;
     }
-    const int rows = R <= 3 ? 4 : 2;
+    const int rows = tile_rows(R, esize);
     const int TY = (256 / TZ) * rows;
     sh.ntz = (n2 + TZ - 1) / TZ;
     sh.nty = (n1 + TY - 1) / TY;
@@ -358,10 +358,11 @@ bool tile_shape(const Geom& g, int R, int nin, int& TZ, TileShape& sh)
     return true;
 }
 
-static int stencil_kind(const Stencil& s, const Geom& g, int nin, int& TZ, TileShape& sh)
+static int stencil_kind(const Stencil& s, const Geom& g, int esize, int nin, int& TZ,
+                        TileShape& sh)
 {
     const int f = forced_kind();
-    const bool tile_ok = s.star_r > 0 && tile_shape(g, s.star_r, nin, TZ, sh);
+    const bool tile_ok = s.star_r > 0 && tile_shape(g, s.star_r, esize, nin, TZ, sh);
     if (f == K_GENERIC && s.d_coefs) return K_GENERIC;
     if (f == K_QUEUE && s.star_r > 0) return K_QUEUE;
     if (tile_ok && f != K_QUEUE) return K_TILE;
@@ -370,9 +371,11 @@ static int stencil_kind(const Stencil& s, const Geom& g, int nin, int& TZ, TileS
 
 int stencil_partials(const Stencil& s, const Geom& g, int nin)
 {
+    // norms are taken of real data only, whose tiles all have the same
+    // shape (4- and 8-byte elements)
     int TZ;
     TileShape sh;
-    const int kind = stencil_kind(s, g, nin, TZ, sh);
+    const int kind = stencil_kind(s, g, 8, nin, TZ, sh);
     if (kind == K_TILE) return sh.nblocks;
     if (kind == K_QUEUE) {
         StarShape q = star_shape(g, nin);
@@ -388,7 +391,7 @@ hipError_t launch_stencil(const Stencil& s, int dtype, int mode, const void* in,
 {
     int TZ = 64;
     TileShape sh;
-    const int kind = stencil_kind(s, g, nin, TZ, sh);
+    const int kind = stencil_kind(s, g, dtype_size(dtype), nin, TZ, sh);
     if (npartial) *npartial = stencil_partials(s, g, nin);
     if (kind == K_TILE) {
         switch (dtype) {

@@ -206,8 +206,12 @@ __global__ __launch_bounds__(256) void k_star_tile(const V* __restrict__ in, V* 
     }
 }
 
+// Rows per thread: 4, or 2 for 16-byte elements and wide stencils, which
+// keeps the register queue within 3 waves per SIMD.
+constexpr int tile_rows(int R, int esize) { return (R <= 3 && esize <= 8) ? 4 : 2; }
+
 // host-side shape selection and launch (stencil_tile_*.hip)
-bool tile_shape(const Geom& g, int R, int nin, int& TZ, TileShape& sh);
+bool tile_shape(const Geom& g, int R, int esize, int nin, int& TZ, TileShape& sh);
 hipError_t launch_stencil_tile(const Stencil& s, int dtype, int mode, const void* in, void* out,
                                const void* aux, const Geom& g, const Phases& ph, int nin,
                                double w, double* partial, int* npartial, bool write,

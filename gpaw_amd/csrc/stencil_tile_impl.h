@@ -10,7 +10,7 @@ hipError_t tile_go(const Stencil& s, const void* in, void* out, const void* aux,
                    const Phases& ph, double w, const TileShape& sh, double* partial,
                    hipStream_t st)
 {
-    constexpr int ROWS = R <= 3 ? 4 : 2;
+    constexpr int ROWS = tile_rows(R, (int)sizeof(V));
     TileCoefs sc;
     for (int c = 0; c < 6 * R + 1; c++) sc.c[c] = s.coefs[c];
     const i64 ng = (i64)g.n[0] * g.n[1] * g.n[2];
