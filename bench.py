@@ -109,12 +109,14 @@ def jacobi_roofline(torch, np, ps, rho_d):
 
 
 def copy_bandwidth(torch):
-    """Device-to-device copy of 2 GiB (read + write 4 GiB) timed with events:
-    the attainable HBM bandwidth on this box, reported beside the peak."""
+    """Device-to-device copy of 2 GiB (read + write 4 GiB) timed with events
+    with the library's 16-byte copy kernel (k_copy16): the attainable HBM
+    bandwidth on this box, reported beside the peak."""
+    from gpaw_amd.ops import copy
     n = 1 << 28
     a = torch.empty(n, dtype=torch.float64, device='cuda').fill_(1.0)
     b = torch.empty_like(a)
-    t = event_time(torch, lambda: b.copy_(a), 10)
+    t = event_time(torch, lambda: copy(b, a), 10)
     del a, b
     return round(2 * 8 * n / t / 1e9, 1)
 

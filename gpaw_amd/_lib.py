@@ -54,6 +54,7 @@ PROTOTYPES = {
     'gpaw_axpby': (_i, [_vp, _vp, _d, _d, _l, _vp]),
     'gpaw_sub_scalar': (_i, [_vp, _vp, _d, _l, _vp]),
     'gpaw_cast': (_i, [_vp, _i, _vp, _i, _l, _vp]),
+    'gpaw_copy': (_i, [_vp, _vp, _l, _vp]),
 }
 
 

@@ -417,3 +417,12 @@ extern "C" int gpaw_cast(void* y, int dty, const void* x, int dtx, long n, void*
     if (!y || !x || n < 0) return gpaw_fail(GPAW_EINVAL, "cast: bad argument");
     return gpaw_check(launch_cast(y, dty, x, dtx, n, (hipStream_t)stream), "cast");
 }
+
+extern "C" int gpaw_copy(void* y, const void* x, long nbytes, void* stream)
+{
+    if (!y || !x || nbytes < 0) return gpaw_fail(GPAW_EINVAL, "copy: bad argument");
+    if ((nbytes | (long)(uintptr_t)y | (long)(uintptr_t)x) & 15)
+        return gpaw_fail(GPAW_EINVAL, "copy: sizes and addresses must be multiples of 16 bytes");
+    if (nbytes == 0) return GPAW_OK;
+    return gpaw_check(launch_copy_bytes(y, x, nbytes, (hipStream_t)stream), "copy");
+}

@@ -82,6 +82,14 @@ __device__ __forceinline__ float2 pmul(float2 v, double pr, double pi)
     return make_float2(r * v.x - i * v.y, r * v.y + i * v.x);
 }
 
+// Multiply by the phase of one side (f = 1: high halo, 2: low halo) of one
+// axis.
+template <class V>
+__device__ __forceinline__ V pside(V v, const Phases& ph, int axis, int f)
+{
+    return pmul(v, ph.re[axis][f - 1], ph.im[axis][f - 1]);
+}
+
 // Wrap one index.  Returns 0 = inside, 1 = high halo (phase side 0),
 // 2 = low halo (phase side 1), 3 = outside a zero-boundary axis.
 __device__ __forceinline__ int wrap_index(int& j, int n, int per)
@@ -110,9 +118,9 @@ __device__ __forceinline__ V ghost_fetch(const V* __restrict__ a, int j0, int j1
     if (f0 == 3 || f1 == 3 || f2 == 3) return vzero<V>();
     V v = a[((i64)j0 * g.n[1] + j1) * g.n[2] + j2];
     if (VT<V>::cplx) {
-        if (f0) v = pmul(v, ph.re[0][f0 - 1], ph.im[0][f0 - 1]);
-        if (f1) v = pmul(v, ph.re[1][f1 - 1], ph.im[1][f1 - 1]);
-        if (f2) v = pmul(v, ph.re[2][f2 - 1], ph.im[2][f2 - 1]);
+        if (f0) v = pside(v, ph, 0, f0);
+        if (f1) v = pside(v, ph, 1, f1);
+        if (f2) v = pside(v, ph, 2, f2);
     }
     return v;
 }

@@ -110,5 +110,7 @@ hipError_t launch_sub_scalar(double* y, const double* x, double b, long long n,
 hipError_t launch_fill(void* y, int dtype, long long n, hipStream_t st);
 hipError_t launch_copy(void* y, const void* x, int dtype, long long n, hipStream_t st);
 hipError_t launch_cast(void* y, int dty, const void* x, int dtx, long long n, hipStream_t st);
+// nbytes a multiple of 16, both pointers 16-byte aligned
+hipError_t launch_copy_bytes(void* y, const void* x, long long nbytes, hipStream_t st);
 
 }  // namespace gpaw

@@ -61,6 +61,23 @@ __global__ __launch_bounds__(256) void k_copy(T* __restrict__ y, const T* __rest
     if (i < n) y[i] = x[i];
 }
 
+// 16-byte copy, four loads in flight per thread, grid-stride over blocks of
+// 4 x 256 vectors (the attainable HBM read + write rate)
+__global__ __launch_bounds__(256) void k_copy16(uint4* __restrict__ y, const uint4* __restrict__ x,
+                                                i64 n)
+{
+    const i64 stride = (i64)gridDim.x * 1024;
+    for (i64 base = (i64)blockIdx.x * 1024 + threadIdx.x; base < n; base += stride) {
+        uint4 v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (base + 256 * k < n) v[k] = x[base + 256 * k];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (base + 256 * k < n) y[base + 256 * k] = v[k];
+    }
+}
+
 template <class TO, class TI>
 __global__ __launch_bounds__(256) void k_cast(TO* __restrict__ y, const TI* __restrict__ x, i64 n)
 {
@@ -140,6 +157,16 @@ hipError_t launch_cast(void* y, int dty, const void* x, int dtx, long long n, hi
                            (const double*)x, m);
     else
         return launch_copy(y, x, dty, n, st);
+    return launch_error();
+}
+
+hipError_t launch_copy_bytes(void* y, const void* x, long long nbytes, hipStream_t st)
+{
+    const i64 n = nbytes / 16;
+    i64 blocks = (n + 1023) / 1024;
+    if (blocks > 8192) blocks = 8192;   // 32 blocks per CU, then grid-stride
+    if (blocks < 1) blocks = 1;
+    launch_kernel(k_copy16, dim3((unsigned)blocks), dim3(256), 0, st, (uint4*)y, (const uint4*)x, n);
     return launch_error();
 }
 

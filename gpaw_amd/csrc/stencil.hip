@@ -53,7 +53,7 @@ __device__ __forceinline__ V flagged_load(const V* __restrict__ p, int flag, con
 {
     if (flag == 3) return vzero<V>();
     V v = *p;
-    if (VT<V>::cplx && flag) v = pmul(v, ph.re[axis][flag - 1], ph.im[axis][flag - 1]);
+    if (VT<V>::cplx && flag) v = pside(v, ph, axis, flag);
     return v;
 }
 

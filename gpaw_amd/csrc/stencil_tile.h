@@ -43,44 +43,37 @@ template <> __device__ __forceinline__ double tnorm2<float2>(float2 v)
     return (double)v.x * (double)v.x + (double)v.y * (double)v.y;
 }
 
+template <class V, int R, int TZ, int ROWS>
+struct TileGeom {
+    static constexpr int NTY = 256 / TZ;
+    static constexpr int TY = NTY * ROWS;
+    static constexpr int LW = TZ + 2 * R;              // LDS row length
+    static constexpr int LH = TY + 2 * R;              // LDS rows
+    static constexpr int NHALO = 2 * R * (TZ + TY);    // halo cells per plane
+    static constexpr int NH = (NHALO + 255) / 256;     // halo cells per thread
+    static constexpr int LDS = 2 * LH * LW;            // two plane buffers
+};
+
+// One tile of one band: columns z0 .. z0+TZ-1, rows y0 .. y0+TY-1, planes
+// i0s .. i0e-1.  a / o / sa point at the band's input, output and aux
+// arrays; lds holds TileGeom::LDS elements.  Every thread of the block
+// calls it with the same tile.  Returns the thread's sum of |result|^2
+// (NORM).
 template <class V, int R, int TZ, int ROWS, int MODE, bool NORM, bool WRITE>
-__global__ __launch_bounds__(256) void k_star_tile(const V* __restrict__ in, V* __restrict__ out,
-                                                   const V* __restrict__ aux, Geom g, Phases ph,
-                                                   TileCoefs sc, double w, TileShape sh, i64 ng,
-                                                   double* __restrict__ partial)
+__device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __restrict__ o,
+                                                  const V* __restrict__ sa, const Geom g,
+                                                  const Phases ph, const TileCoefs sc, double w,
+                                                  int z0, int y0, int i0s, int i0e, V* lds)
 {
     typedef typename VT<V>::R RT;
-    constexpr int NTY = 256 / TZ;
-    constexpr int TY = NTY * ROWS;
-    constexpr int LW = TZ + 2 * R;              // LDS row length
-    constexpr int LH = TY + 2 * R;              // LDS rows
-    constexpr int NHALO = 2 * R * (TZ + TY);    // halo cells per plane
-    constexpr int NH = (NHALO + 255) / 256;     // halo cells per thread
-    __shared__ V lds[2][LH * LW];
-
-    // XCD-aware tile order: block b runs on XCD b % 8; give each XCD a
-    // contiguous range of tiles
-    const int nb = sh.nblocks;
-    const int b = blockIdx.x;
-    int t = b;
-    if ((nb & 7) == 0) t = (b & 7) * (nb >> 3) + (b >> 3);
-    const int tz = t % sh.ntz;
-    int rest = t / sh.ntz;
-    const int ty = rest % sh.nty;
-    rest /= sh.nty;
-    const int ch = rest % sh.nchunk;
-    const int band = rest / sh.nchunk;
+    typedef TileGeom<V, R, TZ, ROWS> TG;
+    constexpr int TY = TG::TY, LW = TG::LW, LH = TG::LH, NHALO = TG::NHALO, NH = TG::NH;
 
     const int tx = threadIdx.x, tyi = threadIdx.y;
     const int tid = tx + TZ * tyi;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     const i64 plane = (i64)n1 * n2;
-    const i64 boff = (i64)band * ng;
-    const V* __restrict__ a = in + boff;
-    const int z0 = tz * TZ, y0 = ty * TY;
     const int i2 = z0 + tx;
-    const int i0s = ch * sh.q;
-    const int i0e = min(i0s + sh.q, n0);
 
     // owned rows
     int i1r[ROWS];
@@ -117,7 +110,7 @@ __global__ __launch_bounds__(256) void k_star_tile(const V* __restrict__ in, V* 
             const int f = wrap_index(pp, n0, g.per[0]);
             if (f == 3) return vzero<V>();
             V v = a[(i64)pp * plane + (i64)i1r[j] * n2 + i2];
-            if (VT<V>::cplx && f) v = pmul(v, ph.re[0][f - 1], ph.im[0][f - 1]);
+            if (VT<V>::cplx && f) v = pside(v, ph, 0, f);
             return v;
         }
         return ghost_fetch(a, p, i1r[j], i2, g, ph);
@@ -138,7 +131,7 @@ __global__ __launch_bounds__(256) void k_star_tile(const V* __restrict__ in, V* 
 
     double acc = 0.0;
     for (int i0 = i0s; i0 < i0e; i0++) {
-        V* B = lds[(i0 - i0s) & 1];
+        V* B = lds + ((i0 - i0s) & 1) * (LH * LW);
 #pragma unroll
         for (int j = 0; j < ROWS; j++) {
             q[j][2 * R] = qn[j];
@@ -151,7 +144,7 @@ __global__ __launch_bounds__(256) void k_star_tile(const V* __restrict__ in, V* 
         if (MODE != ST_APPLY) {
 #pragma unroll
             for (int j = 0; j < ROWS; j++)
-                s[j] = inr[j] ? aux[boff + (i64)i0 * plane + (i64)i1r[j] * n2 + i2] : vzero<V>();
+                s[j] = inr[j] ? sa[(i64)i0 * plane + (i64)i1r[j] * n2 + i2] : vzero<V>();
         }
         if (i0 + 1 < i0e) {
 #pragma unroll
@@ -191,7 +184,7 @@ __global__ __launch_bounds__(256) void k_star_tile(const V* __restrict__ in, V* 
                                           (RT)sc.c[0]);
             if (inr[j]) {
                 if (NORM) acc += tnorm2(r);
-                if (WRITE) out[boff + (i64)i0 * plane + (i64)i1r[j] * n2 + i2] = r;
+                if (WRITE) o[(i64)i0 * plane + (i64)i1r[j] * n2 + i2] = r;
             }
         }
 #pragma unroll
@@ -199,10 +192,38 @@ __global__ __launch_bounds__(256) void k_star_tile(const V* __restrict__ in, V* 
 #pragma unroll
             for (int k = 0; k < 2 * R; k++) q[j][k] = q[j][k + 1];
     }
+    return acc;
+}
+
+template <class V, int R, int TZ, int ROWS, int MODE, bool NORM, bool WRITE>
+__global__ __launch_bounds__(256) void k_star_tile(const V* __restrict__ in, V* __restrict__ out,
+                                                   const V* __restrict__ aux, Geom g, Phases ph,
+                                                   TileCoefs sc, double w, TileShape sh, i64 ng,
+                                                   double* __restrict__ partial)
+{
+    typedef TileGeom<V, R, TZ, ROWS> TG;
+    __shared__ V lds[TG::LDS];
+    // XCD-aware tile order: block b runs on XCD b % 8; give each XCD a
+    // contiguous range of tiles
+    const int nb = sh.nblocks;
+    const int b = blockIdx.x;
+    int t = b;
+    if ((nb & 7) == 0) t = (b & 7) * (nb >> 3) + (b >> 3);
+    const int tz = t % sh.ntz;
+    int rest = t / sh.ntz;
+    const int ty = rest % sh.nty;
+    rest /= sh.nty;
+    const int ch = rest % sh.nchunk;
+    const int band = rest / sh.nchunk;
+    const i64 boff = (i64)band * ng;
+    const int i0s = ch * sh.q;
+    const double acc = star_tile_sweep<V, R, TZ, ROWS, MODE, NORM, WRITE>(
+        in + boff, out + boff, aux ? aux + boff : nullptr, g, ph, sc, w, tz * TZ, ty * TG::TY,
+        i0s, min(i0s + sh.q, g.n[0]), lds);
     if (NORM) {
         __shared__ double red[256];
         const double sum = block_sum<256>(acc, red);
-        if (tid == 0) partial[b] = sum;
+        if (threadIdx.x == 0 && threadIdx.y == 0) partial[b] = sum;
     }
 }
 

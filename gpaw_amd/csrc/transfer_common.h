@@ -87,7 +87,7 @@ __device__ __forceinline__ int ip_tap(int m)   // 0, 1, -1, 2, -2, ...
 // restriction: z-pass at input row (j0, j1); the K+1 taps are unrolled
 template <class V, int K>
 __device__ __forceinline__ V rst_z(const V* __restrict__ a, int j0, int j1, int c2,
-                                   const TrArgs& t, const Phases& ph)
+                                   const TrArgs& t, const Phases ph)
 {
     V s[K + 1];
 #pragma unroll
@@ -99,7 +99,7 @@ __device__ __forceinline__ V rst_z(const V* __restrict__ a, int j0, int j1, int 
 // accumulated pair by pair in the reference's order
 template <class V, int K>
 __device__ __forceinline__ V rst_yz(const V* __restrict__ a, int j0, int c1, int c2,
-                                    const TrArgs& t, const Phases& ph)
+                                    const TrArgs& t, const Phases ph)
 {
     typedef typename VT<V>::R RT;
     V x = rst_z<V, K>(a, j0, c1, c2, t, ph);
@@ -115,7 +115,7 @@ __device__ __forceinline__ V rst_yz(const V* __restrict__ a, int j0, int c1, int
 
 template <class V, int K>
 __device__ __forceinline__ V restrict_point(const V* __restrict__ a, int o0, int o1, int o2,
-                                            const TrArgs& t, const Phases& ph)
+                                            const TrArgs& t, const Phases ph)
 {
     typedef typename VT<V>::R RT;
     const int c0 = K - 1 + 2 * o0 - t.lo[0];
@@ -135,7 +135,7 @@ __device__ __forceinline__ V restrict_point(const V* __restrict__ a, int o0, int
 // interpolation: the odd combination w1 (a0 + a1) + w2 (a-1 + a2) + ...
 template <class V, int K>
 __device__ __forceinline__ V interp_z(const V* __restrict__ a, int j0, int j1, int c2, bool odd,
-                                      const TrArgs& t, const Phases& ph)
+                                      const TrArgs& t, const Phases ph)
 {
     if (!odd) return ghost_fetch(a, j0, j1, c2, t.gin, ph);
     V s[K];
@@ -146,7 +146,7 @@ __device__ __forceinline__ V interp_z(const V* __restrict__ a, int j0, int j1, i
 
 template <class V, int K>
 __device__ __forceinline__ V interp_yz(const V* __restrict__ a, int j0, int c1, int c2, bool oy,
-                                       bool oz, const TrArgs& t, const Phases& ph)
+                                       bool oz, const TrArgs& t, const Phases ph)
 {
     typedef typename VT<V>::R RT;
     if (!oy) return interp_z<V, K>(a, j0, c1, c2, oz, t, ph);
@@ -163,7 +163,7 @@ __device__ __forceinline__ V interp_yz(const V* __restrict__ a, int j0, int c1, 
 
 template <class V, int K>
 __device__ __forceinline__ V interp_point(const V* __restrict__ a, int o0, int o1, int o2,
-                                          const TrArgs& t, const Phases& ph)
+                                          const TrArgs& t, const Phases ph)
 {
     typedef typename VT<V>::R RT;
     const int p0 = o0 + t.skip[0][0], p1 = o1 + t.skip[1][0], p2 = o2 + t.skip[2][0];

@@ -208,3 +208,15 @@ def cast(y, x):
     _lib.check(_lib.lib().gpaw_cast(yy.ptr, yy.code, xx.ptr, xx.code,
                                     int(np.prod(yy.shape)), _arrays.stream()))
     yy.finish()
+
+
+def copy(dst, src):
+    """dst[...] = src on the device with the library's 16-byte copy kernel
+    (both C-contiguous tensors of the same size in bytes)."""
+    d = Arg(dst, write=True, name='dst')
+    a = Arg(src, name='src')
+    nb = a.tensor.numel() * a.tensor.element_size()
+    if d.tensor.numel() * d.tensor.element_size() != nb:
+        raise ValueError('copy: sizes differ')
+    _lib.check(_lib.lib().gpaw_copy(d.ptr, a.ptr, nb, _arrays.stream()))
+    d.finish()

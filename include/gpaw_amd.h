@@ -198,6 +198,11 @@ int gpaw_axpby(double* y, const double* x, double a, double b, long n,
 int gpaw_sub_scalar(double* y, const double* x, double b, long n,
                     void* stream);
 
+/* Device copy of nbytes (numpy's b[:] = a between grid functions, e.g.
+ * gpaw/poisson.py:197 rhos[0][:] = rho); sizes and addresses multiples of 16.
+ * Also the kernel that calibrates the attainable HBM read + write rate. */
+int gpaw_copy(void* y, const void* x, long nbytes, void* stream);
+
 /* Precision conversion between GPAW_F64 <-> GPAW_F32 or GPAW_C128 <->
  * GPAW_C64 (n elements). */
 int gpaw_cast(void* y, int dty, const void* x, int dtx, long n, void* stream);
