@@ -125,6 +125,14 @@ __device__ __forceinline__ V ghost_fetch(const V* __restrict__ a, int j0, int j1
     return v;
 }
 
+// Tile index of block b of nb: block b runs on XCD b % 8, so each XCD gets
+// a contiguous range of tiles and neighbouring tiles, which share halo
+// rows, meet in one L2.  A bijection on [0, nb).
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+}
+
 // Deterministic block reduction of one double per thread (fixed tree order).
 // Every thread of the block must call it.
 template <int NT>

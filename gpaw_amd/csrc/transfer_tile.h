@@ -36,9 +36,10 @@ __global__ __launch_bounds__(256) void k_restrict_tile(const V* __restrict__ in,
     constexpr int W = 2 * K - 1;             // x window
     __shared__ V A[SR * SC];
     __shared__ V Z[SR * T2];
-    const int tz = blockIdx.x % tt.nt2;
-    const int ty = (blockIdx.x / tt.nt2) % tt.nt1;
-    const int band = blockIdx.x / (tt.nt2 * tt.nt1);
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    const int tz = b % tt.nt2;
+    const int ty = (b / tt.nt2) % tt.nt1;
+    const int band = b / (tt.nt2 * tt.nt1);
     const int tid = threadIdx.x;
     const int tzi = tid % T2, tyi = tid / T2;
     const V* __restrict__ a = in + (i64)band * t.ngin;
@@ -104,9 +105,10 @@ __global__ __launch_bounds__(256) void k_interp_tile(const V* __restrict__ in, V
     constexpr int SC = T2 / 2 + K + 1;
     __shared__ V A[SR * SC];
     __shared__ V Z[SR * T2];
-    const int tz = blockIdx.x % tt.nt2;
-    const int ty = (blockIdx.x / tt.nt2) % tt.nt1;
-    const int band = blockIdx.x / (tt.nt2 * tt.nt1);
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    const int tz = b % tt.nt2;
+    const int ty = (b / tt.nt2) % tt.nt1;
+    const int band = b / (tt.nt2 * tt.nt1);
     const int tid = threadIdx.x;
     const int tzi = tid % T2, tyi = tid / T2;
     const V* __restrict__ a = in + (i64)band * t.ngin;
