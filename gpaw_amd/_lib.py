@@ -50,6 +50,10 @@ PROTOTYPES = {
     'gpaw_preconditioner_apply': (_i, [_vp, _vp, _vp, _l, _i, _dp, _vp, _l,
                                        _vp]),
     'gpaw_preconditioner_free': (None, [_vp]),
+    'gpaw_batch_new': (_i, [_i, ctypes.POINTER(_vp)]),
+    'gpaw_batch_apply': (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                              ctypes.POINTER(_vp), _lp, _dp, _i, _vp]),
+    'gpaw_batch_free': (None, [_vp]),
     'gpaw_sum': (_i, [_vp, _l, _d, _vp, _vp]),
     'gpaw_axpby': (_i, [_vp, _vp, _d, _d, _l, _vp]),
     'gpaw_sub_scalar': (_i, [_vp, _vp, _d, _l, _vp]),

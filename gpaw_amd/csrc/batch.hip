@@ -1,0 +1,179 @@
+// C ABI of the batched apply (gpaw_batch_*): groups of (operator, bands,
+// output, Bloch phases) applied in one persistent launch per stencil range.
+//
+// Descriptors are written into a pinned host slot, copied to the device on
+// the caller's stream and read by the kernel; a ring of slots with one
+// event each lets the host fill the next call's descriptors while earlier
+// launches still run.  Groups whose grid cannot be cut into tiles (tiny or
+// very narrow periodic grids) and non-star stencils are applied by the
+// single-operator kernels on the same stream.
+#include <vector>
+
+#include "batch.h"
+#include "capi_internal.h"
+
+namespace gpaw {
+
+hipError_t batch_launch_f64(const BatchGroup*, int, int, int, int, hipStream_t);
+hipError_t batch_launch_c128(const BatchGroup*, int, int, int, int, hipStream_t);
+hipError_t batch_launch_f32(const BatchGroup*, int, int, int, int, hipStream_t);
+hipError_t batch_launch_c64(const BatchGroup*, int, int, int, int, hipStream_t);
+
+hipError_t batch_launch(int dtype, const BatchGroup* d_groups, int ngroups, int ntiles, int R,
+                        int TZ, hipStream_t st)
+{
+    switch (dtype) {
+    case DT_F64: return batch_launch_f64(d_groups, ngroups, ntiles, R, TZ, st);
+    case DT_C128: return batch_launch_c128(d_groups, ngroups, ntiles, R, TZ, st);
+    case DT_F32: return batch_launch_f32(d_groups, ngroups, ntiles, R, TZ, st);
+    case DT_C64: return batch_launch_c64(d_groups, ngroups, ntiles, R, TZ, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace gpaw
+
+using namespace gpaw;
+
+namespace {
+constexpr int SLOTS = 4;
+constexpr int BATCH_Q = 16;   // planes per tile: similar tile costs across groups
+}
+
+struct gpaw_batch {
+    int max_groups = 0;
+    BatchGroup* h[SLOTS] = {};
+    BatchGroup* d[SLOTS] = {};
+    hipEvent_t ev[SLOTS] = {};
+    bool used[SLOTS] = {};
+    int next = 0;
+};
+
+extern "C" void gpaw_batch_free(gpaw_batch* b)
+{
+    if (!b) return;
+    for (int s = 0; s < SLOTS; s++) {
+        if (b->used[s]) hipEventSynchronize(b->ev[s]);
+        if (b->ev[s]) hipEventDestroy(b->ev[s]);
+        if (b->h[s]) hipHostFree(b->h[s]);
+        if (b->d[s]) hipFree(b->d[s]);
+    }
+    delete b;
+}
+
+extern "C" int gpaw_batch_new(int max_groups, gpaw_batch** out)
+{
+    if (!out) return gpaw_fail(GPAW_EINVAL, "batch: NULL argument");
+    *out = nullptr;
+    if (max_groups < 1 || max_groups > 65536)
+        return gpaw_fail(GPAW_EINVAL, "batch: max_groups = %d", max_groups);
+    gpaw_batch* b = new gpaw_batch();
+    b->max_groups = max_groups;
+    const size_t bytes = sizeof(BatchGroup) * (size_t)max_groups;
+    hipError_t e = hipSuccess;
+    for (int s = 0; s < SLOTS && e == hipSuccess; s++) {
+        e = hipHostMalloc((void**)&b->h[s], bytes, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipMalloc((void**)&b->d[s], bytes);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev[s], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+        gpaw_batch_free(b);
+        return gpaw_check(e, "batch: buffers");
+    }
+    *out = b;
+    return GPAW_OK;
+}
+
+extern "C" int gpaw_batch_apply(gpaw_batch* b, int ngroups, gpaw_operator* const* ops,
+                                const void* const* ins, void* const* outs, const long* nins,
+                                const double* phases, int dtype, void* stream)
+{
+    if (!b) return gpaw_fail(GPAW_EINVAL, "batch: NULL handle");
+    if (ngroups < 0 || ngroups > b->max_groups)
+        return gpaw_fail(GPAW_EINVAL, "batch: %d groups, the batch holds %d", ngroups,
+                         b->max_groups);
+    if (ngroups == 0) return GPAW_OK;
+    if (!ops || !ins || !outs || !nins) return gpaw_fail(GPAW_EINVAL, "batch: NULL array");
+    hipStream_t st = (hipStream_t)stream;
+    const int esize = dtype_size(dtype);
+    // validate every group before any work is queued
+    for (int i = 0; i < ngroups; i++) {
+        if (!ops[i]) return gpaw_fail(GPAW_EINVAL, "batch: group %d has no operator", i);
+        int rc = gpaw_check_dtype(ops[i]->real, dtype);
+        if (rc) return rc;
+        if (nins[i] < 0 || nins[i] > 65535)
+            return gpaw_fail(GPAW_EINVAL, "batch: group %d nin = %ld", i, nins[i]);
+        if (nins[i] && (!ins[i] || !outs[i]))
+            return gpaw_fail(GPAW_EINVAL, "batch: group %d has a NULL array", i);
+        if (nins[i] && ins[i] == outs[i])
+            return gpaw_fail(GPAW_EINVAL, "batch: group %d input and output must differ", i);
+    }
+    // one slot of descriptors; wait until the launch that last used it ended
+    const int s = b->next;
+    b->next = (b->next + 1) % SLOTS;
+    if (b->used[s]) {
+        hipError_t e = hipEventSynchronize(b->ev[s]);
+        if (e != hipSuccess) return gpaw_check(e, "batch: slot reuse");
+    }
+    BatchGroup* h = b->h[s];
+    // per stencil range: tile-able groups first, the rest applied singly
+    int count[GPAW_MAXR + 1] = {0};
+    int tz_vote[GPAW_MAXR + 1] = {0};
+    for (int i = 0; i < ngroups; i++) {
+        const int R = ops[i]->st.star_r;
+        if (R >= 1 && R <= 4 && nins[i]) tz_vote[R] += tile_tz(ops[i]->g.n[2]) == 64 ? 1 : -1;
+    }
+    int n = 0;
+    int start[GPAW_MAXR + 2] = {0};
+    int tiles[GPAW_MAXR + 1] = {0};
+    std::vector<char> fallback(ngroups, 0);
+    for (int R = 1; R <= 4; R++) {
+        start[R] = n;
+        const int TZ = tz_vote[R] >= 0 ? 64 : 32;
+        for (int i = 0; i < ngroups; i++) {
+            gpaw_operator* op = ops[i];
+            if (op->st.star_r != R || !nins[i]) continue;
+            TileShape sh;
+            if (!tile_geometry(op->g, R, esize, (int)nins[i], TZ, BATCH_Q, sh) ||
+                (long)tiles[R] + sh.nblocks > 0x7fffffffL) {
+                fallback[i] = 1;
+                continue;
+            }
+            BatchGroup& gr = h[n++];
+            gr.in = ins[i];
+            gr.out = outs[i];
+            gr.g = op->g;
+            gpaw_phases(phases ? phases + 12 * i : nullptr, gr.ph);
+            for (int c = 0; c < 6 * R + 1; c++) gr.sc.c[c] = op->st.coefs[c];
+            gr.sh = sh;
+            gr.ng = (i64)op->g.n[0] * op->g.n[1] * op->g.n[2];
+            gr.first = tiles[R];
+            gr.pad = 0;
+            tiles[R] += sh.nblocks;
+        }
+        count[R] = n - start[R];
+    }
+    hipError_t e = hipSuccess;
+    if (n) {
+        e = hipMemcpyAsync(b->d[s], h, sizeof(BatchGroup) * n, hipMemcpyHostToDevice, st);
+        for (int R = 1; R <= 4 && e == hipSuccess; R++) {
+            if (!count[R]) continue;
+            const int TZ = tz_vote[R] >= 0 ? 64 : 32;
+            e = batch_launch(dtype, b->d[s] + start[R], count[R], tiles[R], R, TZ, st);
+        }
+        // the slot is busy until the copy and the launches reading it are done
+        const hipError_t er = hipEventRecord(b->ev[s], st);
+        b->used[s] = (er == hipSuccess);
+        if (e == hipSuccess) e = er;
+    }
+    for (int i = 0; i < ngroups && e == hipSuccess; i++) {
+        gpaw_operator* op = ops[i];
+        const int R = op->st.star_r;
+        if (!nins[i] || (R >= 1 && R <= 4 && !fallback[i])) continue;
+        Phases ph;
+        gpaw_phases(phases ? phases + 12 * i : nullptr, ph);
+        e = launch_stencil(op->st, dtype, ST_APPLY, ins[i], outs[i], nullptr, op->g, ph,
+                           (int)nins[i], 0.0, nullptr, nullptr, true, st);
+    }
+    return gpaw_check(e, "batch apply");
+}

@@ -83,11 +83,20 @@ __device__ __forceinline__ float2 pmul(float2 v, double pr, double pi)
 }
 
 // Multiply by the phase of one side (f = 1: high halo, 2: low halo) of one
-// axis.
-template <class V>
+// axis.  SEL chooses the side by a select instead of an index: phases held
+// in a copy of a struct read from global memory would otherwise be indexed
+// at run time and placed in private (scratch) memory; phases passed as
+// kernel arguments are indexed in place, which needs fewer registers.
+template <bool SEL = false, class V>
 __device__ __forceinline__ V pside(V v, const Phases& ph, int axis, int f)
 {
-    return pmul(v, ph.re[axis][f - 1], ph.im[axis][f - 1]);
+    if constexpr (SEL) {
+        const double pr = f == 1 ? ph.re[axis][0] : ph.re[axis][1];
+        const double pi = f == 1 ? ph.im[axis][0] : ph.im[axis][1];
+        return pmul(v, pr, pi);
+    } else {
+        return pmul(v, ph.re[axis][f - 1], ph.im[axis][f - 1]);
+    }
 }
 
 // Wrap one index.  Returns 0 = inside, 1 = high halo (phase side 0),
@@ -108,7 +117,7 @@ __device__ __forceinline__ int wrap_index(int& j, int n, int per)
 }
 
 // Value of the padded field at (j0, j1, j2) of one band.
-template <class V>
+template <class V, bool SEL = false>
 __device__ __forceinline__ V ghost_fetch(const V* __restrict__ a, int j0, int j1, int j2,
                                          const Geom& g, const Phases& ph)
 {
@@ -118,9 +127,9 @@ __device__ __forceinline__ V ghost_fetch(const V* __restrict__ a, int j0, int j1
     if (f0 == 3 || f1 == 3 || f2 == 3) return vzero<V>();
     V v = a[((i64)j0 * g.n[1] + j1) * g.n[2] + j2];
     if (VT<V>::cplx) {
-        if (f0) v = pside(v, ph, 0, f0);
-        if (f1) v = pside(v, ph, 1, f1);
-        if (f2) v = pside(v, ph, 2, f2);
+        if (f0) v = pside<SEL>(v, ph, 0, f0);
+        if (f1) v = pside<SEL>(v, ph, 1, f1);
+        if (f2) v = pside<SEL>(v, ph, 2, f2);
     }
     return v;
 }

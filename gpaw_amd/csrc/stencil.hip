@@ -328,34 +328,52 @@ static int forced_kind()
     return f;
 }
 
-bool tile_shape(const Geom& g, int R, int esize, int nin, int& TZ, TileShape& sh)
+int tile_tz(int n2)
 {
-    if (R < 1 || R > 4) return false;
+    // 64 lanes along i2 unless 32 wastes clearly fewer lanes
+    const int w64 = ((n2 + 63) / 64) * 64 - n2, w32 = ((n2 + 31) / 32) * 32 - n2;
+    return (w32 * 100 < w64 * 100 - 5 * n2) ? 32 : 64;
+}
+
+bool tile_geometry(const Geom& g, int R, int esize, int nin, int TZ, int q, TileShape& sh)
+{
+    if (R < 1 || R > 4 || (TZ != 32 && TZ != 64) || q < 1 || nin < 1) return false;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
-    TZ = 64;
-    {
-        const int w64 = ((n2 + 63) / 64) * 64 - n2, w32 = ((n2 + 31) / 32) * 32 - n2;
-        if (w32 * 100 < w64 * 100 - 5 * n2) TZ = 32;
-    }
-    const int rows = tile_rows(R, esize);
-    const int TY = (256 / TZ) * rows;
+    const int TY = (256 / TZ) * tile_rows(R, esize);
     sh.ntz = (n2 + TZ - 1) / TZ;
     sh.nty = (n1 + TY - 1) / TY;
     // every coordinate a tile touches (tile + halo) must lie within one
     // period on either side of the grid: ghost indices wrap only once
     if (sh.ntz * TZ + R > 2 * n2 || sh.nty * TY + R > 2 * n1 || n0 < R || n1 < R || n2 < R)
         return false;
-    const long nt = (long)sh.ntz * sh.nty * nin;
+    sh.q = q < n0 ? q : n0;
+    sh.nchunk = (n0 + sh.q - 1) / sh.q;
+    const long nb = (long)sh.ntz * sh.nty * nin * sh.nchunk;
+    if (nb > 0x7fffffffL) return false;
+    sh.nblocks = (int)nb;
+    return true;
+}
+
+int tile_default_q(const Geom& g, int R, int esize, int nin, int TZ)
+{
+    // enough tiles to fill the chip several times, runs of >= 16 planes
+    const int n0 = g.n[0];
+    const int TY = (256 / TZ) * tile_rows(R, esize);
+    const long nt = (long)((g.n[2] + TZ - 1) / TZ) * ((g.n[1] + TY - 1) / TY) * nin;
     long nch = (2048 + nt - 1) / nt;
     const long maxch = n0 / 16 > 1 ? n0 / 16 : 1;
     if (nch > maxch) nch = maxch;
     if (nch < 1) nch = 1;
-    sh.q = (int)((n0 + nch - 1) / nch);
-    sh.nchunk = (n0 + sh.q - 1) / sh.q;
-    const long nb = nt * sh.nchunk;
-    if (nb < 256 || nb > 0x7fffffffL) return false;
-    sh.nblocks = (int)nb;
-    return true;
+    return (int)((n0 + nch - 1) / nch);
+}
+
+bool tile_shape(const Geom& g, int R, int esize, int nin, int& TZ, TileShape& sh)
+{
+    if (R < 1 || R > 4) return false;
+    TZ = tile_tz(g.n[2]);
+    if (!tile_geometry(g, R, esize, nin, TZ, tile_default_q(g, R, esize, nin, TZ), sh))
+        return false;
+    return sh.nblocks >= 256;
 }
 
 static int stencil_kind(const Stencil& s, const Geom& g, int esize, int nin, int& TZ,

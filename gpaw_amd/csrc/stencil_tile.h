@@ -60,7 +60,8 @@ struct TileGeom {
 // arrays; lds holds TileGeom::LDS elements.  Every thread of the block
 // calls it with the same tile.  Returns the thread's sum of |result|^2
 // (NORM).
-template <class V, int R, int TZ, int ROWS, int MODE, bool NORM, bool WRITE, int PF>
+template <class V, int R, int TZ, int ROWS, int MODE, bool NORM, bool WRITE, int PF,
+          bool SEL = false>
 __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __restrict__ o,
                                                   const V* __restrict__ sa, const Geom g,
                                                   const Phases ph, const TileCoefs sc, double w,
@@ -111,10 +112,10 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
             const int f = wrap_index(pp, n0, g.per[0]);
             if (f == 3) return vzero<V>();
             V v = a[(i64)pp * plane + (i64)i1r[j] * n2 + i2];
-            if (VT<V>::cplx && f) v = pside(v, ph, 0, f);
+            if (VT<V>::cplx && f) v = pside<SEL>(v, ph, 0, f);
             return v;
         }
-        return ghost_fetch(a, p, i1r[j], i2, g, ph);
+        return ghost_fetch<V, SEL>(a, p, i1r[j], i2, g, ph);
     };
 
     // planes i0-R .. i0+R of the owned rows; the loads of the queue plane
@@ -133,9 +134,9 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
     }
 #pragma unroll
     for (int k = 0; k < NH; k++) {
-        ha[k] = hoff[k] >= 0 ? ghost_fetch(a, i0s, hy[k], hz[k], g, ph) : vzero<V>();
+        ha[k] = hoff[k] >= 0 ? ghost_fetch<V, SEL>(a, i0s, hy[k], hz[k], g, ph) : vzero<V>();
         hb[k] = (PF == 2 && hoff[k] >= 0 && i0s + 1 < i0e)
-                    ? ghost_fetch(a, i0s + 1, hy[k], hz[k], g, ph)
+                    ? ghost_fetch<V, SEL>(a, i0s + 1, hy[k], hz[k], g, ph)
                     : vzero<V>();
     }
 
@@ -161,7 +162,7 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
             for (int j = 0; j < ROWS; j++) qn[j] = qload(i0 + D + R, j);
 #pragma unroll
             for (int k = 0; k < NH; k++)
-                if (hoff[k] >= 0) hn[k] = ghost_fetch(a, i0 + D, hy[k], hz[k], g, ph);
+                if (hoff[k] >= 0) hn[k] = ghost_fetch<V, SEL>(a, i0 + D, hy[k], hz[k], g, ph);
         }
         __syncthreads();
         // register window along axis 1: rows tyi*ROWS - R .. tyi*ROWS + ROWS + R - 1
@@ -231,7 +232,7 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
                 for (int j = 0; j < ROWS; j++) qa[j] = qload(i0 + 1 + R, j);
 #pragma unroll
                 for (int k = 0; k < NH; k++)
-                    if (hoff[k] >= 0) ha[k] = ghost_fetch(a, i0 + 1, hy[k], hz[k], g, ph);
+                    if (hoff[k] >= 0) ha[k] = ghost_fetch<V, SEL>(a, i0 + 1, hy[k], hz[k], g, ph);
             }
             __syncthreads();
             V win[ROWS + 2 * R];
@@ -308,6 +309,10 @@ __global__ __launch_bounds__(256) void k_star_tile(const V* __restrict__ in, V* 
 constexpr int tile_rows(int R, int esize) { return (R <= 3 && esize <= 8) ? 4 : 2; }
 
 // host-side shape selection and launch (stencil_tile_*.hip)
+int tile_tz(int n2);
+// tiles of TZ columns and q planes; false if a tile's halo would wrap twice
+bool tile_geometry(const Geom& g, int R, int esize, int nin, int TZ, int q, TileShape& sh);
+int tile_default_q(const Geom& g, int R, int esize, int nin, int TZ);
 bool tile_shape(const Geom& g, int R, int esize, int nin, int& TZ, TileShape& sh);
 hipError_t launch_stencil_tile(const Stencil& s, int dtype, int mode, const void* in, void* out,
                                const void* aux, const Geom& g, const Phases& ph, int nin,

@@ -182,6 +182,27 @@ int gpaw_preconditioner_apply(gpaw_preconditioner* pc, const void* residuals,
 
 void gpaw_preconditioner_free(gpaw_preconditioner* pc);
 
+/* ----------------------------------------------------------- batched apply
+ * No reference counterpart: the reference applies an operator to the bands
+ * of one grid and one k-point per call (Operator_apply, c/operators.c:
+ * 316-405).  Here several such calls -- bands of several k-points, grids of
+ * uneven size, different coefficients -- run as one persistent launch per
+ * stencil range.  Group i applies ops[i] to nins[i] bands at ins[i] into
+ * outs[i] with the phase_cd of phases[12*i .. 12*i+11] (host memory; NULL
+ * = all phases 1).  Results are bit-identical to gpaw_operator_apply of
+ * each group.  A batch object holds the descriptor buffers for up to
+ * max_groups groups; it may be reused, one call at a time. */
+typedef struct gpaw_batch gpaw_batch;
+
+int gpaw_batch_new(int max_groups, gpaw_batch** batch);
+
+int gpaw_batch_apply(gpaw_batch* batch, int ngroups, gpaw_operator* const* ops,
+                     const void* const* ins, void* const* outs,
+                     const long* nins, const double* phases, int dtype,
+                     void* stream);
+
+void gpaw_batch_free(gpaw_batch* batch);
+
 /* -------------------------------------------------------------- helpers */
 /* Deterministic FP64 sum of n doubles times scale into *result (device):
  * gd.integrate / comm.sum of the reference (gpaw/grid_descriptor.py:197-208). */

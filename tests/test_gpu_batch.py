@@ -1,0 +1,84 @@
+"""Batched apply (one persistent launch over many grids): every group's
+result is bit-identical to its own FDOperator.apply, for bands of several
+k-points, grids of uneven size and boundary, different grid spacings (so
+different coefficients), stencil ranges 1..3 in one call, and groups that
+cannot be tiled (applied by the single-operator kernels)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip('no GPU')
+    return torch
+
+
+def _groups(torch, dtype, rng):
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.fd_operators import Laplace
+    specs = [  # N, pbc, h, nn, nbands, k
+        ((48, 48, 48), True, 0.20, 3, 3, (0.1, 0.2, -0.3)),
+        ((48, 48, 48), True, 0.20, 3, 2, (0.5, 0.0, 0.25)),
+        ((48, 48, 48), False, 0.22, 3, 1, None),
+        ((80, 64, 96), True, 0.18, 3, 2, (-0.2, 0.4, 0.1)),
+        ((40, 24, 56), (True, False, True), 0.25, 2, 4, (0.3, 0.0, -0.1)),
+        ((64, 64, 64), True, 0.20, 1, 5, (0.0, 0.25, 0.25)),
+        ((6, 6, 6), True, 0.30, 3, 2, (0.2, 0.2, 0.2)),      # too small to tile
+        ((32, 32, 32), False, 0.20, 3, 0, None),             # no bands
+    ]
+    out = []
+    tdt = torch.complex128 if dtype == complex else torch.float64
+    for N, pbc, h, nn, nb, k in specs:
+        gd = GridDescriptor(N, np.array(N) * h, pbc)
+        op = Laplace(gd, -0.5, nn, dtype)
+        shape = (nb,) + tuple(gd.n_c)
+        a = rng.standard_normal(shape)
+        if dtype == complex:
+            a = a + 1j * rng.standard_normal(shape)
+        ph = gd.phase_cd(k) if (dtype == complex and k is not None) else None
+        ta = torch.from_numpy(a).cuda()
+        out.append((op, ta, torch.empty(shape, dtype=tdt, device='cuda'), ph))
+    return out
+
+
+@pytest.mark.parametrize('dtype', [float, complex])
+def test_batch_equals_single_applies(dtype):
+    print('case batch', dtype.__name__, flush=True)
+    torch = _torch()
+    from gpaw_amd.batch import BatchApply
+    rng = np.random.default_rng(17)
+    groups = _groups(torch, dtype, rng)
+    batch = BatchApply(max_groups=16)
+    for rep in range(3):      # the descriptor slots are reused
+        for _, _, b, _ in groups:
+            b.fill_(float('nan'))
+        batch(groups)
+        torch.cuda.synchronize()
+        for i, (op, a, b, ph) in enumerate(groups):
+            ref = torch.empty_like(a)
+            op.apply(a, ref, ph)
+            torch.cuda.synchronize()
+            assert torch.equal(b, ref), (rep, i)
+
+
+def test_batch_single_band_3d_and_errors():
+    torch = _torch()
+    from gpaw_amd.batch import BatchApply
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.fd_operators import Laplace
+    gd = GridDescriptor((32, 48, 64), (6.0, 9.0, 12.0))
+    op = Laplace(gd, 1.0, 3)
+    a = torch.randn(tuple(gd.n_c), dtype=torch.float64, device='cuda')
+    b = torch.empty_like(a)
+    BatchApply(4)([(op, a, b, None)])
+    ref = torch.empty_like(a)
+    op.apply(a, ref)
+    torch.cuda.synchronize()
+    assert torch.equal(b, ref)
+    with pytest.raises(ValueError):
+        BatchApply(1)([(op, a, b, None), (op, a, b, None)])
+    with pytest.raises(TypeError):    # mixed dtypes
+        BatchApply(4)([(op, a, b, None), (op, a.float(), b.float(), None)])
