@@ -54,6 +54,8 @@ PROTOTYPES = {
     'gpaw_batch_apply': (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                               ctypes.POINTER(_vp), _lp, _dp, _i, _vp]),
     'gpaw_batch_free': (None, [_vp]),
+    'gpaw_autotune': (_i, [_i]),
+    'gpaw_autotune_report': (_l, [ctypes.c_char_p, _l]),
     'gpaw_sum': (_i, [_vp, _l, _d, _vp, _vp]),
     'gpaw_axpby': (_i, [_vp, _vp, _d, _d, _l, _vp]),
     'gpaw_sub_scalar': (_i, [_vp, _vp, _d, _l, _vp]),

@@ -426,3 +426,7 @@ extern "C" int gpaw_copy(void* y, const void* x, long nbytes, void* stream)
     if (nbytes == 0) return GPAW_OK;
     return gpaw_check(launch_copy_bytes(y, x, nbytes, (hipStream_t)stream), "copy");
 }
+
+extern "C" int gpaw_autotune(int enable) { return autotune_set(enable); }
+
+extern "C" long gpaw_autotune_report(char* buf, long len) { return autotune_report(buf, len); }

@@ -70,6 +70,11 @@ struct Stencil {
     int* d_off = nullptr;
 };
 
+// Stencil launch-shape autotuner (stencil.hip): switch on / off (returns
+// the previous state) and write a text table of the choices made.
+int autotune_set(int on);
+long autotune_report(char* buf, long len);
+
 // Work estimate used to size partial-sum buffers.
 int stencil_partials(const Stencil& s, const Geom& g, int nin);
 

@@ -17,7 +17,15 @@
 //    one thread per point, coefficients read through the scalar cache.
 // Both fuse the residual (apply - rho) and an FP64 sum of |result|^2 per
 // block (deterministic tree order) for the multigrid convergence test.
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
 
 #include "kernels.h"
 #include "stencil_tile.h"
@@ -283,7 +291,7 @@ hipError_t generic_launch(const Stencil& s, const void* in, void* out, const voi
 }
 
 template <class V, int MODE>
-hipError_t kind_launch(const Stencil& s, const void* in, void* out, const void* aux,
+hipError_t kind_launch(bool star, const Stencil& s, const void* in, void* out, const void* aux,
                        const Geom& g, const Phases& ph, int nin, double w, double* partial,
                        bool write, hipStream_t st)
 {
@@ -291,22 +299,22 @@ hipError_t kind_launch(const Stencil& s, const void* in, void* out, const void* 
         return hipErrorInvalidValue;
     } else {
         if (partial && !st_norm_ok<V, MODE>()) return hipErrorInvalidValue;
-        if (s.star_r > 0) return star_dispatch_r<V, MODE>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+        if (star) return star_dispatch_r<V, MODE>(s, in, out, aux, g, ph, nin, w, partial, write, st);
         return generic_launch<V, MODE>(s, in, out, aux, g, ph, nin, w, partial, write, st);
     }
 }
 
 template <class V>
-hipError_t dispatch_mode(const Stencil& s, int mode, const void* in, void* out, const void* aux,
-                         const Geom& g, const Phases& ph, int nin, double w, double* partial,
-                         bool write, hipStream_t st)
+hipError_t dispatch_mode(bool star, const Stencil& s, int mode, const void* in, void* out,
+                         const void* aux, const Geom& g, const Phases& ph, int nin, double w,
+                         double* partial, bool write, hipStream_t st)
 {
     switch (mode) {
-    case ST_APPLY: return kind_launch<V, ST_APPLY>(s, in, out, aux, g, ph, nin, w, partial, write, st);
-    case ST_RESID: return kind_launch<V, ST_RESID>(s, in, out, aux, g, ph, nin, w, partial, write, st);
-    case ST_JACOBI: return kind_launch<V, ST_JACOBI>(s, in, out, aux, g, ph, nin, w, partial, write, st);
-    case ST_PCUPD: return kind_launch<V, ST_PCUPD>(s, in, out, aux, g, ph, nin, w, partial, write, st);
-    case ST_PCFIN: return kind_launch<V, ST_PCFIN>(s, in, out, aux, g, ph, nin, w, partial, write, st);
+    case ST_APPLY: return kind_launch<V, ST_APPLY>(star, s, in, out, aux, g, ph, nin, w, partial, write, st);
+    case ST_RESID: return kind_launch<V, ST_RESID>(star, s, in, out, aux, g, ph, nin, w, partial, write, st);
+    case ST_JACOBI: return kind_launch<V, ST_JACOBI>(star, s, in, out, aux, g, ph, nin, w, partial, write, st);
+    case ST_PCUPD: return kind_launch<V, ST_PCUPD>(star, s, in, out, aux, g, ph, nin, w, partial, write, st);
+    case ST_PCFIN: return kind_launch<V, ST_PCFIN>(star, s, in, out, aux, g, ph, nin, w, partial, write, st);
     }
     return hipErrorInvalidValue;
 }
@@ -403,32 +411,200 @@ int stencil_partials(const Stencil& s, const Geom& g, int nin)
     return (int)((ng + 255) / 256) * nin;
 }
 
+// One launch configuration: the kernel kind and, for tiles, the shape.
+struct LaunchCfg {
+    int kind;
+    int TZ;
+    TileShape sh;
+};
+
+static hipError_t launch_cfg(const LaunchCfg& c, const Stencil& s, int dtype, int mode,
+                             const void* in, void* out, const void* aux, const Geom& g,
+                             const Phases& ph, int nin, double w, double* partial, bool write,
+                             hipStream_t st)
+{
+    if (c.kind == K_TILE) {
+        switch (dtype) {
+        case DT_F64: return tile_launch_f64(s, mode, in, out, aux, g, ph, w, c.TZ, c.sh, partial, write, st);
+        case DT_C128: return tile_launch_c128(s, mode, in, out, aux, g, ph, w, c.TZ, c.sh, partial, write, st);
+        case DT_F32: return tile_launch_f32(s, mode, in, out, aux, g, ph, w, c.TZ, c.sh, partial, write, st);
+        case DT_C64: return tile_launch_c64(s, mode, in, out, aux, g, ph, w, c.TZ, c.sh, partial, write, st);
+        }
+        return hipErrorInvalidValue;
+    }
+    const bool star = c.kind == K_QUEUE;
+    switch (dtype) {
+    case DT_F64: return dispatch_mode<double>(star, s, mode, in, out, aux, g, ph, nin, w, partial, write, st);
+    case DT_C128: return dispatch_mode<double2>(star, s, mode, in, out, aux, g, ph, nin, w, partial, write, st);
+    case DT_F32: return dispatch_mode<float>(star, s, mode, in, out, aux, g, ph, nin, w, partial, write, st);
+    case DT_C64: return dispatch_mode<float2>(star, s, mode, in, out, aux, g, ph, nin, w, partial, write, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner.  For a star stencil of range 1..4 the launch shape -- the
+// register-queue kernel, or the tile kernel with TZ = 32 / 64 columns and
+// q = 8 .. n0 planes per tile -- is chosen by timing every candidate on the
+// caller's own launch the first time a (type, mode, range, grid, bands)
+// combination is seen.  Every shape computes identical bits and input and
+// output never alias, so running the candidates repeats the same result.
+// Off by default (heuristic shapes); gpaw_autotune(1) or GPAW_AMD_AUTOTUNE=1
+// turns it on.  Launches that produce norm partials (whose count is fixed
+// by the heuristic shape) and launches recorded into a graph are not tuned;
+// the latter use an existing entry.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct TuneKey {
+    int dtype, mode, R, n0, n1, n2, per, nin;
+    bool operator<(const TuneKey& o) const
+    {
+        const int a[8] = {dtype, mode, R, n0, n1, n2, per, nin};
+        const int b[8] = {o.dtype, o.mode, o.R, o.n0, o.n1, o.n2, o.per, o.nin};
+        return std::lexicographical_compare(a, a + 8, b, b + 8);
+    }
+};
+
+struct TuneVal {
+    LaunchCfg cfg;
+    float us;
+    int candidates;
+};
+
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune;
+std::atomic<int> g_tune_on{-1};
+
+bool tune_enabled()
+{
+    int v = g_tune_on.load();
+    if (v < 0) {
+        const char* e = getenv("GPAW_AMD_AUTOTUNE");
+        v = (e && atoi(e) > 0) ? 1 : 0;
+        g_tune_on.store(v);
+    }
+    return v > 0;
+}
+
+// time `reps` launches of one candidate (after one untimed launch)
+float time_cfg(const LaunchCfg& c, const Stencil& s, int dtype, int mode, const void* in,
+               void* out, const void* aux, const Geom& g, const Phases& ph, int nin, double w,
+               bool write, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
+{
+    const int reps = 3;
+    if (launch_cfg(c, s, dtype, mode, in, out, aux, g, ph, nin, w, nullptr, write, st) != hipSuccess)
+        return -1.0f;
+    hipEventRecord(e0, st);
+    for (int r = 0; r < reps; r++)
+        launch_cfg(c, s, dtype, mode, in, out, aux, g, ph, nin, w, nullptr, write, st);
+    hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess || hipGetLastError() != hipSuccess) return -1.0f;
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, e0, e1);
+    return 1e3f * ms / reps;
+}
+
+TuneVal tune(const TuneKey& key, const LaunchCfg& heuristic, const Stencil& s, int dtype,
+             int mode, const void* in, void* out, const void* aux, const Geom& g,
+             const Phases& ph, int nin, double w, bool write, hipStream_t st)
+{
+    std::vector<LaunchCfg> cand;
+    LaunchCfg qc{K_QUEUE, 64, TileShape{}};
+    cand.push_back(qc);
+    for (int TZ : {64, 32})
+        for (int q : {8, 16, 32, 64, 1 << 20}) {
+            LaunchCfg c{K_TILE, TZ, TileShape{}};
+            if (!tile_geometry(g, s.star_r, dtype_size(dtype), nin, TZ, q, c.sh)) continue;
+            bool dup = false;
+            for (const LaunchCfg& o : cand)
+                dup |= o.kind == K_TILE && o.TZ == TZ && o.sh.q == c.sh.q;
+            if (!dup) cand.push_back(c);
+        }
+    TuneVal best{heuristic, -1.0f, 0};
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return best;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return best;
+    }
+    for (const LaunchCfg& c : cand) {
+        const float us = time_cfg(c, s, dtype, mode, in, out, aux, g, ph, nin, w, write, st, e0, e1);
+        if (us < 0) continue;
+        best.candidates++;
+        if (best.us < 0 || us < best.us) {
+            best.cfg = c;
+            best.us = us;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    (void)key;
+    return best;
+}
+
+}  // namespace
+
+int autotune_set(int on)
+{
+    const int prev = tune_enabled() ? 1 : 0;
+    g_tune_on.store(on > 0 ? 1 : 0);
+    return prev;
+}
+
+long autotune_report(char* buf, long len)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::string r;
+    char line[256];
+    for (const auto& kv : g_tune) {
+        const TuneKey& k = kv.first;
+        const TuneVal& v = kv.second;
+        snprintf(line, sizeof line,
+                 "dtype=%d mode=%d R=%d n=%dx%dx%d periodic=%d nin=%d -> %s TZ=%d q=%d "
+                 "%.2f us (%d candidates)\n",
+                 k.dtype, k.mode, k.R, k.n0, k.n1, k.n2, k.per, k.nin,
+                 v.cfg.kind == K_TILE ? "tile" : "queue", v.cfg.kind == K_TILE ? v.cfg.TZ : 0,
+                 v.cfg.kind == K_TILE ? v.cfg.sh.q : 0, v.us, v.candidates);
+        r += line;
+    }
+    if (buf && len > 0) {
+        const long n = (long)r.size() < len - 1 ? (long)r.size() : len - 1;
+        memcpy(buf, r.data(), n);
+        buf[n] = 0;
+    }
+    return (long)r.size();
+}
+
 hipError_t launch_stencil(const Stencil& s, int dtype, int mode, const void* in, void* out,
                           const void* aux, const Geom& g, const Phases& ph, int nin, double w,
                           double* partial, int* npartial, bool write, hipStream_t st)
 {
-    int TZ = 64;
-    TileShape sh;
-    const int kind = stencil_kind(s, g, dtype_size(dtype), nin, TZ, sh);
+    LaunchCfg c{K_GENERIC, 64, TileShape{}};
+    c.kind = stencil_kind(s, g, dtype_size(dtype), nin, c.TZ, c.sh);
     if (npartial) *npartial = stencil_partials(s, g, nin);
-    if (kind == K_TILE) {
-        switch (dtype) {
-        case DT_F64: return tile_launch_f64(s, mode, in, out, aux, g, ph, w, TZ, sh, partial, write, st);
-        case DT_C128: return tile_launch_c128(s, mode, in, out, aux, g, ph, w, TZ, sh, partial, write, st);
-        case DT_F32: return tile_launch_f32(s, mode, in, out, aux, g, ph, w, TZ, sh, partial, write, st);
-        case DT_C64: return tile_launch_c64(s, mode, in, out, aux, g, ph, w, TZ, sh, partial, write, st);
+    if (!partial && s.star_r >= 1 && s.star_r <= 4 && forced_kind() < 0 && tune_enabled()) {
+        const TuneKey key{dtype, mode, s.star_r, g.n[0], g.n[1], g.n[2],
+                          g.per[0] | g.per[1] << 1 | g.per[2] << 2, nin};
+        bool found = false;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tune.find(key);
+            if (it != g_tune.end()) {
+                c = it->second.cfg;
+                found = true;
+            }
         }
-        return hipErrorInvalidValue;
+        if (!found && !graph_recorder() && in != out) {
+            const TuneVal v = tune(key, c, s, dtype, mode, in, out, aux, g, ph, nin, w, write, st);
+            if (v.us >= 0) {
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                g_tune[key] = v;
+                c = v.cfg;
+            }
+        }
     }
-    Stencil sq = s;
-    if (kind == K_GENERIC) sq.star_r = 0;
-    switch (dtype) {
-    case DT_F64: return dispatch_mode<double>(sq, mode, in, out, aux, g, ph, nin, w, partial, write, st);
-    case DT_C128: return dispatch_mode<double2>(sq, mode, in, out, aux, g, ph, nin, w, partial, write, st);
-    case DT_F32: return dispatch_mode<float>(sq, mode, in, out, aux, g, ph, nin, w, partial, write, st);
-    case DT_C64: return dispatch_mode<float2>(sq, mode, in, out, aux, g, ph, nin, w, partial, write, st);
-    }
-    return hipErrorInvalidValue;
+    return launch_cfg(c, s, dtype, mode, in, out, aux, g, ph, nin, w, partial, write, st);
 }
 
 }  // namespace gpaw

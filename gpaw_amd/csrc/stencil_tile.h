@@ -9,9 +9,10 @@
 //     (double-buffered, one barrier per plane);
 //   * axis-1 neighbours use a register window of ROWS + 2R values per
 //     column, so only the 2R window ends are LDS reads;
-//   * the queue values and halo cells of plane i0 + 2 are loaded before
-//     the barrier of plane i0 (two register sets alternate), so two planes'
-//     work overlaps their latency.
+//   * the next plane's queue values and halo cells are loaded before the
+//     barrier, overlapping their latency with the current plane's work
+//     (a distance of two planes was measured slower: the extra registers
+//     cost a wave per SIMD).
 // Waves are TZ lanes wide along the contiguous axis, so every HBM access is
 // a full coalesced row segment.  Tiles are ordered so that one XCD (one L2)
 // processes a contiguous range of tiles: the halo rows a block re-reads
@@ -60,8 +61,7 @@ struct TileGeom {
 // arrays; lds holds TileGeom::LDS elements.  Every thread of the block
 // calls it with the same tile.  Returns the thread's sum of |result|^2
 // (NORM).
-template <class V, int R, int TZ, int ROWS, int MODE, bool NORM, bool WRITE, int PF,
-          bool SEL = false>
+template <class V, int R, int TZ, int ROWS, int MODE, bool NORM, bool WRITE, bool SEL = false>
 __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __restrict__ o,
                                                   const V* __restrict__ sa, const Geom g,
                                                   const Phases ph, const TileCoefs sc, double w,
@@ -118,54 +118,46 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
         return ghost_fetch<V, SEL>(a, p, i1r[j], i2, g, ph);
     };
 
-    // planes i0-R .. i0+R of the owned rows; the loads of the queue plane
-    // and the halo of plane i0 + 2 are issued two steps ahead (register sets
-    // a / b alternate with the LDS buffers), so two compute phases cover
-    // their latency
+    // planes i0-R .. i0+R of the owned rows; the queue plane and the halo of
+    // plane i0 + 1 are loaded during plane i0
     V q[ROWS][2 * R + 1];
-    V qa[ROWS], qb[ROWS];
-    V ha[NH], hb[NH];
+    V qa[ROWS];
+    V ha[NH];
 #pragma unroll
     for (int j = 0; j < ROWS; j++) {
 #pragma unroll
         for (int k = 0; k < 2 * R; k++) q[j][k] = qload(i0s - R + k, j);
         qa[j] = qload(i0s + R, j);
-        qb[j] = (PF == 2 && i0s + 1 < i0e) ? qload(i0s + 1 + R, j) : vzero<V>();
     }
 #pragma unroll
-    for (int k = 0; k < NH; k++) {
+    for (int k = 0; k < NH; k++)
         ha[k] = hoff[k] >= 0 ? ghost_fetch<V, SEL>(a, i0s, hy[k], hz[k], g, ph) : vzero<V>();
-        hb[k] = (PF == 2 && hoff[k] >= 0 && i0s + 1 < i0e)
-                    ? ghost_fetch<V, SEL>(a, i0s + 1, hy[k], hz[k], g, ph)
-                    : vzero<V>();
-    }
 
     double acc = 0.0;
-    auto step = [&](int i0, V(&qn)[ROWS], V(&hn)[NH], V* B) __attribute__((always_inline)) {
-        constexpr int D = PF;   // prefetch distance in planes
+    for (int i0 = i0s; i0 < i0e; i0++) {
+        V* B = lds + ((i0 - i0s) & 1) * (LH * LW);
 #pragma unroll
         for (int j = 0; j < ROWS; j++) {
-            q[j][2 * R] = qn[j];
+            q[j][2 * R] = qa[j];
             B[(R + tyi * ROWS + j) * LW + R + tx] = q[j][R];
         }
 #pragma unroll
         for (int k = 0; k < NH; k++)
-            if (hoff[k] >= 0) B[hoff[k]] = hn[k];
+            if (hoff[k] >= 0) B[hoff[k]] = ha[k];
         V s[ROWS];
         if (MODE != ST_APPLY) {
 #pragma unroll
             for (int j = 0; j < ROWS; j++)
                 s[j] = inr[j] ? sa[(i64)i0 * plane + (i64)i1r[j] * n2 + i2] : vzero<V>();
         }
-        if (i0 + D < i0e) {
+        if (i0 + 1 < i0e) {
 #pragma unroll
-            for (int j = 0; j < ROWS; j++) qn[j] = qload(i0 + D + R, j);
+            for (int j = 0; j < ROWS; j++) qa[j] = qload(i0 + 1 + R, j);
 #pragma unroll
             for (int k = 0; k < NH; k++)
-                if (hoff[k] >= 0) hn[k] = ghost_fetch<V, SEL>(a, i0 + D, hy[k], hz[k], g, ph);
+                if (hoff[k] >= 0) ha[k] = ghost_fetch<V, SEL>(a, i0 + 1, hy[k], hz[k], g, ph);
         }
         __syncthreads();
-        // register window along axis 1: rows tyi*ROWS - R .. tyi*ROWS + ROWS + R - 1
         V win[ROWS + 2 * R];
 #pragma unroll
         for (int k = 0; k < R; k++) {
@@ -191,8 +183,8 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
             for (int k = 1; k <= R; k++) x = vadd(x, vmul(q[j][R + k], (RT)sc.c[4 * R + k]));
 #pragma unroll
             for (int k = 1; k <= R; k++) x = vadd(x, vmul(q[j][R - k], (RT)sc.c[5 * R + k]));
-            const V r = st_epilogue<MODE>(x, q[j][R], MODE == ST_APPLY ? vzero<V>() : s[j], (RT)w,
-                                          (RT)sc.c[0]);
+            const V r = st_epilogue<MODE>(x, q[j][R], MODE == ST_APPLY ? vzero<V>() : s[j],
+                                          (RT)w, (RT)sc.c[0]);
             if (inr[j]) {
                 if (NORM) acc += tnorm2(r);
                 if (WRITE) o[(i64)i0 * plane + (i64)i1r[j] * n2 + i2] = r;
@@ -202,81 +194,11 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
         for (int j = 0; j < ROWS; j++)
 #pragma unroll
             for (int k = 0; k < 2 * R; k++) q[j][k] = q[j][k + 1];
-    };
-    if constexpr (PF == 2) {
-        // even steps use set a and LDS buffer 0, odd steps set b and buffer 1
-        for (int i0 = i0s; i0 < i0e; i0 += 2) {
-            step(i0, qa, ha, lds);
-            if (i0 + 1 < i0e) step(i0 + 1, qb, hb, lds + LH * LW);
-        }
-    } else {
-        // one set: plane i0 + 1 is loaded during plane i0
-        for (int i0 = i0s; i0 < i0e; i0++) {
-            V* B = lds + ((i0 - i0s) & 1) * (LH * LW);
-#pragma unroll
-            for (int j = 0; j < ROWS; j++) {
-                q[j][2 * R] = qa[j];
-                B[(R + tyi * ROWS + j) * LW + R + tx] = q[j][R];
-            }
-#pragma unroll
-            for (int k = 0; k < NH; k++)
-                if (hoff[k] >= 0) B[hoff[k]] = ha[k];
-            V s[ROWS];
-            if (MODE != ST_APPLY) {
-#pragma unroll
-                for (int j = 0; j < ROWS; j++)
-                    s[j] = inr[j] ? sa[(i64)i0 * plane + (i64)i1r[j] * n2 + i2] : vzero<V>();
-            }
-            if (i0 + 1 < i0e) {
-#pragma unroll
-                for (int j = 0; j < ROWS; j++) qa[j] = qload(i0 + 1 + R, j);
-#pragma unroll
-                for (int k = 0; k < NH; k++)
-                    if (hoff[k] >= 0) ha[k] = ghost_fetch<V, SEL>(a, i0 + 1, hy[k], hz[k], g, ph);
-            }
-            __syncthreads();
-            V win[ROWS + 2 * R];
-#pragma unroll
-            for (int k = 0; k < R; k++) {
-                win[k] = B[(tyi * ROWS + k) * LW + R + tx];
-                win[ROWS + R + k] = B[(tyi * ROWS + ROWS + R + k) * LW + R + tx];
-            }
-#pragma unroll
-            for (int j = 0; j < ROWS; j++) win[R + j] = q[j][R];
-#pragma unroll
-            for (int j = 0; j < ROWS; j++) {
-                const V* row = B + (R + tyi * ROWS + j) * LW + R + tx;
-                V x = vzero<V>();
-                if (MODE != ST_JACOBI) x = vadd(x, vmul(q[j][R], (RT)sc.c[0]));
-#pragma unroll
-                for (int k = 1; k <= R; k++) x = vadd(x, vmul(row[k], (RT)sc.c[k]));
-#pragma unroll
-                for (int k = 1; k <= R; k++) x = vadd(x, vmul(row[-k], (RT)sc.c[R + k]));
-#pragma unroll
-                for (int k = 1; k <= R; k++) x = vadd(x, vmul(win[R + j + k], (RT)sc.c[2 * R + k]));
-#pragma unroll
-                for (int k = 1; k <= R; k++) x = vadd(x, vmul(win[R + j - k], (RT)sc.c[3 * R + k]));
-#pragma unroll
-                for (int k = 1; k <= R; k++) x = vadd(x, vmul(q[j][R + k], (RT)sc.c[4 * R + k]));
-#pragma unroll
-                for (int k = 1; k <= R; k++) x = vadd(x, vmul(q[j][R - k], (RT)sc.c[5 * R + k]));
-                const V r = st_epilogue<MODE>(x, q[j][R], MODE == ST_APPLY ? vzero<V>() : s[j],
-                                              (RT)w, (RT)sc.c[0]);
-                if (inr[j]) {
-                    if (NORM) acc += tnorm2(r);
-                    if (WRITE) o[(i64)i0 * plane + (i64)i1r[j] * n2 + i2] = r;
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < ROWS; j++)
-#pragma unroll
-                for (int k = 0; k < 2 * R; k++) q[j][k] = q[j][k + 1];
-        }
     }
     return acc;
 }
 
-template <class V, int R, int TZ, int ROWS, int MODE, bool NORM, bool WRITE, int PF>
+template <class V, int R, int TZ, int ROWS, int MODE, bool NORM, bool WRITE>
 __global__ __launch_bounds__(256) void k_star_tile(const V* __restrict__ in, V* __restrict__ out,
                                                    const V* __restrict__ aux, Geom g, Phases ph,
                                                    TileCoefs sc, double w, TileShape sh, i64 ng,
@@ -294,7 +216,7 @@ __global__ __launch_bounds__(256) void k_star_tile(const V* __restrict__ in, V* 
     const int band = rest / sh.nchunk;
     const i64 boff = (i64)band * ng;
     const int i0s = ch * sh.q;
-    const double acc = star_tile_sweep<V, R, TZ, ROWS, MODE, NORM, WRITE, PF>(
+    const double acc = star_tile_sweep<V, R, TZ, ROWS, MODE, NORM, WRITE>(
         in + boff, out + boff, aux ? aux + boff : nullptr, g, ph, sc, w, tz * TZ, ty * TG::TY,
         i0s, min(i0s + sh.q, g.n[0]), lds);
     if (NORM) {

@@ -1,23 +1,9 @@
 // Host launcher of k_star_tile for one storage type (instantiated once per
 // type in stencil_tile_<type>.hip so the variants compile in parallel).
 #pragma once
-#include <cstdlib>
-
 #include "stencil_tile.h"
 
 namespace gpaw {
-
-// prefetch distance of the APPLY kernels: GPAW_AMD_TILE_PF=2 selects two
-// planes ahead (timing experiments), the default is one
-inline int tile_prefetch()
-{
-    static int pf = -1;
-    if (pf < 0) {
-        const char* e = getenv("GPAW_AMD_TILE_PF");
-        pf = (e && e[0] == '2') ? 2 : 1;
-    }
-    return pf;
-}
 
 template <class V, int R, int TZ, int MODE, bool NORM, bool WRITE>
 hipError_t tile_go(const Stencil& s, const void* in, void* out, const void* aux, const Geom& g,
@@ -28,13 +14,7 @@ hipError_t tile_go(const Stencil& s, const void* in, void* out, const void* aux,
     TileCoefs sc;
     for (int c = 0; c < 6 * R + 1; c++) sc.c[c] = s.coefs[c];
     const i64 ng = (i64)g.n[0] * g.n[1] * g.n[2];
-    if constexpr (MODE == ST_APPLY) {
-        if (tile_prefetch() == 2)
-            return launch_kernel((k_star_tile<V, R, TZ, ROWS, MODE, NORM, WRITE, 2>),
-                                 dim3(sh.nblocks), dim3(TZ, 256 / TZ), 0, st, (const V*)in,
-                                 (V*)out, (const V*)aux, g, ph, sc, w, sh, ng, partial);
-    }
-    launch_kernel((k_star_tile<V, R, TZ, ROWS, MODE, NORM, WRITE, 1>), dim3(sh.nblocks),
+    launch_kernel((k_star_tile<V, R, TZ, ROWS, MODE, NORM, WRITE>), dim3(sh.nblocks),
                   dim3(TZ, 256 / TZ), 0, st, (const V*)in, (V*)out, (const V*)aux, g, ph, sc, w,
                   sh, ng, partial);
     return launch_error();

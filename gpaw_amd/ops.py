@@ -220,3 +220,19 @@ def copy(dst, src):
         raise ValueError('copy: sizes differ')
     _lib.check(_lib.lib().gpaw_copy(d.ptr, a.ptr, nb, _arrays.stream()))
     d.finish()
+
+
+def autotune(enable=True):
+    """Switch the launch-shape autotuner of the stencil kernels on or off;
+    returns the previous setting."""
+    return bool(_lib.lib().gpaw_autotune(int(bool(enable))))
+
+
+def autotune_report():
+    """The tuner's choices so far, one line per (dtype, mode, range, grid,
+    bands) combination."""
+    L = _lib.lib()
+    n = L.gpaw_autotune_report(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    L.gpaw_autotune_report(buf, n + 1)
+    return buf.value.decode()

@@ -203,6 +203,19 @@ int gpaw_batch_apply(gpaw_batch* batch, int ngroups, gpaw_operator* const* ops,
 
 void gpaw_batch_free(gpaw_batch* batch);
 
+/* --------------------------------------------------------------- autotune
+ * No reference counterpart (the reference's stencil loops have no launch
+ * shape).  With the tuner on, the first apply / relax / residual launch of
+ * each (dtype, mode, stencil range, grid, band count) times the candidate
+ * launch shapes of the star-stencil kernels on that very launch and keeps
+ * the fastest; every shape gives bit-identical results.  Returns the
+ * previous setting.  Off by default; GPAW_AMD_AUTOTUNE=1 also turns it on. */
+int gpaw_autotune(int enable);
+
+/* The tuner's choices as text, one line per combination; returns the full
+ * length (buf may be NULL to query it). */
+long gpaw_autotune_report(char* buf, long len);
+
 /* -------------------------------------------------------------- helpers */
 /* Deterministic FP64 sum of n doubles times scale into *result (device):
  * gd.integrate / comm.sum of the reference (gpaw/grid_descriptor.py:197-208). */

@@ -82,3 +82,30 @@ def test_batch_single_band_3d_and_errors():
         BatchApply(1)([(op, a, b, None), (op, a, b, None)])
     with pytest.raises(TypeError):    # mixed dtypes
         BatchApply(4)([(op, a, b, None), (op, a.float(), b.float(), None)])
+
+
+def test_autotuned_shapes_give_identical_results():
+    """The launch-shape tuner times its candidates on the real launch; every
+    shape computes the same bits, so tuned and heuristic results agree."""
+    torch = _torch()
+    from gpaw_amd import ops
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.fd_operators import Laplace
+    gd = GridDescriptor((64, 48, 40), (12.0, 9.0, 7.5))
+    ph = gd.phase_cd((0.1, -0.2, 0.3))
+    lap = Laplace(gd, -0.5, 3, complex)
+    a = torch.randn((16,) + tuple(gd.n_c), dtype=torch.complex128, device='cuda')
+    ref = torch.empty_like(a)
+    lap.apply(a, ref, ph)
+    prev = ops.autotune(True)
+    try:
+        for _ in range(2):       # first call tunes, second reuses the entry
+            b = torch.empty_like(a)
+            lap.apply(a, b, ph)
+            torch.cuda.synchronize()
+            assert torch.equal(b, ref)
+        rep = ops.autotune_report()
+        assert 'n=64x48x40' in rep and 'nin=16' in rep, rep
+        print(rep, flush=True)
+    finally:
+        ops.autotune(prev)
