@@ -157,7 +157,18 @@ def fd_laplace_c3(torch, np):
     res['interpolate_K6_ms'] = round(ti * 1e3, 3)
     res['interpolate_K6_GBps'] = round(tb / ti / 1e9, 1)
     res['transfer_bytes'] = tb
-    del a, b, c
+    # FP32 variant: complex64 storage and arithmetic on the same bands,
+    # error against the FP64 result (which is bit-identical to the reference)
+    a32 = a.to(torch.complex64)
+    b32 = torch.empty_like(a32)
+    t32 = event_time(torch, lambda: lap.apply(a32, b32, ph), 10)
+    lap.apply(a, b, ph)
+    err = float((b32.to(torch.complex128) - b).abs().max() / b.abs().max())
+    res['fp32'] = {'avg_apply_ms': round(t32 * 1e3, 3),
+                   'GBps': round(algo / 2 / t32 / 1e9, 1),
+                   'pct_peak': round(100 * algo / 2 / t32 / 1e9 / PEAK_HBM_GBS, 1),
+                   'rel_maxnorm_err_vs_fp64': err}
+    del a, b, c, a32, b32
     return res
 
 

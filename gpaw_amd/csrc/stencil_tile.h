@@ -226,9 +226,12 @@ __global__ __launch_bounds__(256) void k_star_tile(const V* __restrict__ in, V* 
     }
 }
 
-// Rows per thread: 4, or 2 for 16-byte elements and wide stencils, which
-// keeps the register queue within 3 waves per SIMD.
-constexpr int tile_rows(int R, int esize) { return (R <= 3 && esize <= 8) ? 4 : 2; }
+// Rows per thread.  One row keeps the register queue small (R=3: 58 VGPRs
+// for float64, 118 for complex128, 8 and 4 waves per SIMD); on MI355X the
+// occupancy gained outweighs the extra L2 reads of the taller halo: 1 row
+// measured 4.1 / 3.6 TB/s on 64^3 float64 / complex128 bands against
+// 2.7-3.3 / 3.0-3.2 TB/s for 2-4 rows.
+constexpr int tile_rows(int, int) { return 1; }
 
 // host-side shape selection and launch (stencil_tile_*.hip)
 int tile_tz(int n2);

@@ -4,8 +4,10 @@
 //
 // Both kernels own a tile of the output (i1, i2) plane (256 threads, one
 // output column each) and sweep the input planes along i0 in order:
-//   1. the tile's slab of the current input plane (with its halo) is loaded
-//      once into LDS, ghost-indexed (periodic + Bloch phase or zero);
+//   1. the tile's slab of the current input plane (with its halo) is stored
+//      into LDS from registers, ghost-indexed (periodic + Bloch phase or
+//      zero); the next plane's slab is loaded into those registers while
+//      the current one is combined (restriction);
 //   2. the z-pass is evaluated from LDS into a second LDS array;
 //   3. the y-pass is evaluated from that, one value per thread, and pushed
 //      into a per-thread register ring of the last K (interpolation) or
@@ -54,15 +56,29 @@ __global__ __launch_bounds__(256) void k_restrict_tile(const V* __restrict__ in,
     const int hi2 = 2 * t.nout[2] + 2 * K - 3 - t.lo[2];
     const int p0s = -t.lo[0];
     const int np = 2 * t.nout[0] + 2 * K - 3;   // input planes swept
+    // slab cells of this thread: c = tid + 256 k; the next plane's cells are
+    // loaded into registers while the current plane is combined
+    constexpr int NS = (SR * SC + 255) / 256;
+    auto slab = [&](int p0, int k) -> V {
+        const int c = tid + 256 * k;
+        const int r = c / SC, s = c % SC;
+        const int j1 = r0 + r, j2 = s0 + s;
+        return (c >= SR * SC || j1 < lo1 || j1 >= hi1 || j2 < lo2 || j2 >= hi2 || p0 < lo0 ||
+                p0 >= hi0)
+                   ? vzero<V>()
+                   : ghost_fetch(a, p0, j1, j2, t.gin, ph);
+    };
+    V pre[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) pre[k] = slab(p0s, k);
     V ring[W];
     for (int j = 0; j < np; j++) {
-        const int p0 = p0s + j;
-        for (int c = tid; c < SR * SC; c += 256) {
-            const int r = c / SC, s = c % SC;
-            const int j1 = r0 + r, j2 = s0 + s;
-            A[c] = (j1 < lo1 || j1 >= hi1 || j2 < lo2 || j2 >= hi2 || p0 < lo0 || p0 >= hi0)
-                       ? vzero<V>()
-                       : ghost_fetch(a, p0, j1, j2, t.gin, ph);
+#pragma unroll
+        for (int k = 0; k < NS; k++)
+            if (tid + 256 * k < SR * SC) A[tid + 256 * k] = pre[k];
+        if (j + 1 < np) {
+#pragma unroll
+            for (int k = 0; k < NS; k++) pre[k] = slab(p0s + j + 1, k);
         }
         __syncthreads();
         for (int c = tid; c < SR * T2; c += 256) {
