@@ -17,6 +17,7 @@
 // cycle the phi_0 buffers swap roles every cycle, so two graphs (one per
 // starting buffer) are captured and alternated.  The host reads the 8-byte
 // error after each cycle to apply the reference's stopping rule exactly.
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -97,7 +98,8 @@ hipError_t emit_level(gpaw_poisson* ps, int l, double*& cur0, double*& alt0, hip
         e = launch_stencil(op->st, DT_F64, ST_RESID, cur, nullptr, ps->rho[0], op->g, ph, 1, 0.0,
                            op->partial, &np, false, st);
         if (e != hipSuccess) return e;
-        e = launch_reduce_partials(op->partial, np, ps->dv, ps->d_err, st);
+        e = launch_reduce_partials(op->partial, np, ps->dv, ps->err_dst ? ps->err_dst : ps->d_err,
+                                   st);
         cur0 = cur;
         alt0 = alt;
     }
@@ -126,6 +128,55 @@ int capture(gpaw_poisson* ps)
         }
     }
     return GPAW_OK;
+}
+
+// k V-cycles from `state`; before cycle i, phi is copied to snap[i]; the
+// error after cycle i goes to d_errs[i]
+hipError_t emit_chunk(gpaw_poisson* ps, int k, int state, hipStream_t st)
+{
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < k && e == hipSuccess; i++) {
+        e = launch_copy(ps->snap[i], state ? ps->phiB[0] : ps->phiA[0], DT_F64, ps->npts[0], st);
+        ps->err_dst = ps->d_errs + i;
+        if (e == hipSuccess) e = emit_cycle(ps, state, st);
+        ps->err_dst = nullptr;
+        state ^= ps->swaps;
+    }
+    return e;
+}
+
+int ensure_chunks(gpaw_poisson* ps)
+{
+    if (!ps->snap.empty()) return GPAW_OK;
+    int rc = GPAW_OK;
+    ps->snap.assign(GPAW_CHUNK_MAX, nullptr);
+    for (int i = 0; i < GPAW_CHUNK_MAX && !rc; i++) rc = alloc(&ps->snap[i], ps->npts[0]);
+    if (!rc) rc = alloc(&ps->d_errs, GPAW_CHUNK_MAX);
+    if (!rc)
+        rc = gpaw_check(hipHostMalloc((void**)&ps->h_errs, sizeof(double) * GPAW_CHUNK_MAX),
+                        "poisson: pinned errors");
+    if (!rc) ps->chunk.assign(2 * GPAW_CHUNK_MAX, nullptr);
+    if (rc) {   // leave no half-built state behind
+        for (double* p : ps->snap)
+            if (p) hipFree(p);
+        ps->snap.clear();
+        if (ps->d_errs) hipFree(ps->d_errs);
+        ps->d_errs = nullptr;
+    }
+    return rc;
+}
+
+int run_chunk(gpaw_poisson* ps, int k, int state, hipStream_t st)
+{
+    if (ps->use_graph) {
+        hipGraphExec_t& g = ps->chunk[2 * (k - 1) + state];
+        if (!g) {
+            hipError_t e = build_graph(&g, [&](hipStream_t s) { return emit_chunk(ps, k, state, s); });
+            if (e != hipSuccess) return gpaw_check(e, "poisson: chunk graph");
+        }
+        return gpaw_check(hipGraphLaunch(g, st), "poisson: chunk launch");
+    }
+    return gpaw_check(emit_chunk(ps, k, state, st), "poisson: chunk");
 }
 
 int run_cycle(gpaw_poisson* ps, int state, hipStream_t st)
@@ -330,17 +381,56 @@ extern "C" int gpaw_poisson_solve_neutral(gpaw_poisson* ps, double* phi, const d
     hipStream_t st = (hipStream_t)stream;
     int rc = load_inputs(ps, phi, rho, st);
     if (rc) return rc;
+    rc = ensure_chunks(ps);
+    if (rc) return rc;
+    // The reference runs `while iterate2() > eps and niter < maxiter:
+    // niter += 1` (gpaw/poisson.py:203-209), one V-cycle and one norm per
+    // step.  Here V-cycles run in chunks of k with phi saved before each;
+    // the host reads the k norms once, finds the first cycle that meets the
+    // stopping rule and, if later cycles of the chunk ran, restores phi as
+    // it was after that cycle.  niter and phi are exactly the reference's.
+    // k follows the observed convergence rate so that a chunk usually ends
+    // at the stopping cycle.
     int state = 0;
-    int n = 1;
+    int n = 0;         // V-cycles accepted so far
     double err = 0.0;
-    for (;;) {   // while iterate2() > eps and niter < maxiter: niter += 1
-        rc = run_cycle(ps, state, st);
+    double prev = -1.0, rate = -1.0;
+    int k = 2;
+    for (;;) {
+        if (k > maxiter - n) k = maxiter - n;
+        rc = run_chunk(ps, k, state, st);
         if (rc) return rc;
-        state ^= ps->swaps;
-        rc = read_error(ps, st, &err);
-        if (rc) return rc;
-        if (!(err > eps) || n >= maxiter) break;
-        n++;
+        hipError_t e = hipMemcpyAsync(ps->h_errs, ps->d_errs, sizeof(double) * k,
+                                      hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return gpaw_check(e, "poisson: error readback");
+        int stop = -1;
+        for (int i = 0; i < k; i++) {
+            err = ps->h_errs[i];
+            n++;
+            if (prev > 0 && err > 0) rate = err / prev;
+            prev = err;
+            if (!(err > eps) || n >= maxiter) {
+                stop = i;
+                break;
+            }
+        }
+        if (stop >= 0) {
+            const int after = state ^ (ps->swaps & ((stop + 1) & 1));
+            if (stop + 1 < k)   // cycles past the stopping one ran: take phi back
+                e = launch_copy(after ? ps->phiB[0] : ps->phiA[0], ps->snap[stop + 1], DT_F64,
+                                ps->npts[0], st);
+            if (e != hipSuccess) return gpaw_check(e, "poisson: restore");
+            state = after;
+            break;
+        }
+        state ^= ps->swaps & (k & 1);
+        // cycles still needed at the observed rate
+        k = GPAW_CHUNK_MAX;
+        if (rate > 0 && rate < 1 && err > eps) {
+            const double m = std::ceil(std::log(eps / err) / std::log(rate));
+            if (m >= 1 && m < GPAW_CHUNK_MAX) k = (int)m;
+        }
     }
     rc = gpaw_check(launch_copy(phi, state ? ps->phiB[0] : ps->phiA[0], DT_F64, ps->npts[0], st),
                     "poisson: store");
@@ -362,6 +452,12 @@ extern "C" void gpaw_poisson_free(gpaw_poisson* ps)
         for (double* p : *v)
             if (p) hipFree(p);
     if (ps->d_err) hipFree(ps->d_err);
+    for (hipGraphExec_t g : ps->chunk)
+        if (g) hipGraphExecDestroy(g);
+    for (double* p : ps->snap)
+        if (p) hipFree(p);
+    if (ps->d_errs) hipFree(ps->d_errs);
+    if (ps->h_errs) hipHostFree(ps->h_errs);
     if (ps->d_coarse) hipFree(ps->d_coarse);
     if (ps->mixed) gpaw_mixed_free(ps->mixed);
     if (ps->h_err) hipHostFree(ps->h_err);

@@ -3,6 +3,8 @@
 #pragma once
 #include <vector>
 
+#define GPAW_CHUNK_MAX 8   // V-cycles per chunk
+
 #include "capi_internal.h"
 #include "coarse.h"
 
@@ -20,7 +22,14 @@ struct gpaw_poisson {
     std::vector<long long> npts;
     double* d_err = nullptr;
     double* h_err = nullptr;
+    double* err_dst = nullptr;  // where the level-0 norm of the next emitted cycle goes
     hipGraphExec_t exec[2] = {nullptr, nullptr};
+    // chunks of V-cycles (solve_neutral): snapshots of phi before each cycle
+    // of a chunk, one error slot per cycle, graphs by (length, start state)
+    std::vector<double*> snap;
+    double* d_errs = nullptr;
+    double* h_errs = nullptr;
+    std::vector<hipGraphExec_t> chunk;   // index 2 * (k - 1) + state
     int use_graph = 1;
     int swaps = 0;             // 1 if level 0 does an odd number of sweeps per cycle
     int lc = -1;               // first level run by the fused coarse kernel (-1: none)
