@@ -22,21 +22,6 @@ namespace gpaw {
 
 namespace {
 
-__device__ __forceinline__ double star7(const double* __restrict__ a, int i0, int i1, int i2,
-                                        const Geom& g, const Phases& ph, const double* c,
-                                        bool centre)
-{
-    double x = 0.0;
-    if (centre) x = x + a[((i64)i0 * g.n[1] + i1) * g.n[2] + i2] * c[0];
-    x = x + ghost_fetch(a, i0, i1, i2 + 1, g, ph) * c[1];
-    x = x + ghost_fetch(a, i0, i1, i2 - 1, g, ph) * c[2];
-    x = x + ghost_fetch(a, i0, i1 + 1, i2, g, ph) * c[3];
-    x = x + ghost_fetch(a, i0, i1 - 1, i2, g, ph) * c[4];
-    x = x + ghost_fetch(a, i0 + 1, i1, i2, g, ph) * c[5];
-    x = x + ghost_fetch(a, i0 - 1, i1, i2, g, ph) * c[6];
-    return x;
-}
-
 __device__ __forceinline__ void split(int idx, const Geom& g, int& i0, int& i1, int& i2)
 {
     i2 = idx % g.n[2];
@@ -48,19 +33,70 @@ __device__ __forceinline__ void split(int idx, const Geom& g, int& i0, int& i1, 
 // level arrays: phi A/B, rho, residual
 enum { PA = 0, PB = 1, RHO = 2, RES = 3 };
 
-__device__ void jacobi_sweeps(const CoarseLevel& L, double* const* ptr, int k, int& cur,
-                              const Phases& ph)
+// A level has at most 2048 points (two per thread).  Each thread's points
+// and their six neighbours (z+, z-, y+, y-, x+, x-: the summation order of
+// the 7-point operator, centre first) are found once per level visit; a
+// neighbour outside a zero boundary is -1.  The sweeps then need no index
+// arithmetic.
+struct Pts {
+    int idx[2];       // flat index, -1 when the thread has no such point
+    int nb[2][6];
+};
+
+__device__ __forceinline__ int nb_index(int j0, int j1, int j2, const Geom& g)
+{
+    const int f0 = wrap_index(j0, g.n[0], g.per[0]);
+    const int f1 = wrap_index(j1, g.n[1], g.per[1]);
+    const int f2 = wrap_index(j2, g.n[2], g.per[2]);
+    if (f0 == 3 || f1 == 3 || f2 == 3) return -1;
+    return (j0 * g.n[1] + j1) * g.n[2] + j2;
+}
+
+__device__ __forceinline__ void level_points(const CoarseLevel& L, Pts& p)
 {
     const int n = L.g.n[0] * L.g.n[1] * L.g.n[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int idx = threadIdx.x + 1024 * k;
+        p.idx[k] = idx < n ? idx : -1;
+        int i0 = 0, i1 = 0, i2 = 0;
+        if (idx < n) split(idx, L.g, i0, i1, i2);
+        p.nb[k][0] = nb_index(i0, i1, i2 + 1, L.g);
+        p.nb[k][1] = nb_index(i0, i1, i2 - 1, L.g);
+        p.nb[k][2] = nb_index(i0, i1 + 1, i2, L.g);
+        p.nb[k][3] = nb_index(i0, i1 - 1, i2, L.g);
+        p.nb[k][4] = nb_index(i0 + 1, i1, i2, L.g);
+        p.nb[k][5] = nb_index(i0 - 1, i1, i2, L.g);
+    }
+}
+
+// x + the six neighbour terms in summation order (the coarse levels are
+// real, phases 1, so a neighbour is a plain load or zero)
+__device__ __forceinline__ double nb_sum(const double* __restrict__ a, const int* nb,
+                                         const double* c, double x)
+{
+#pragma unroll
+    for (int m = 0; m < 6; m++) x = x + (nb[m] >= 0 ? a[nb[m]] : 0.0) * c[m + 1];
+    return x;
+}
+
+__device__ void jacobi_sweeps(const CoarseLevel& L, const Pts& p, double* const* ptr, int k,
+                              int& cur)
+{
     const double* __restrict__ rho = ptr[RHO];
+    double c[7];
+#pragma unroll
+    for (int m = 0; m < 7; m++) c[m] = L.c[m];
+    const double w = L.w;
     for (int it = 0; it < k; it++) {
         const double* __restrict__ src = ptr[cur];
         double* __restrict__ dst = ptr[cur ^ 1];
-        for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
-            int i0, i1, i2;
-            split(idx, L.g, i0, i1, i2);
-            const double x = star7(src, i0, i1, i2, L.g, ph, L.c, false);
-            dst[idx] = (1.0 - L.w) * src[idx] + L.w * (rho[idx] - x) / L.c[0];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int idx = p.idx[q];
+            if (idx < 0) continue;
+            const double x = nb_sum(src, p.nb[q], c, 0.0);
+            dst[idx] = (1.0 - w) * src[idx] + w * (rho[idx] - x) / c[0];
         }
         __syncthreads();
         cur ^= 1;
@@ -112,16 +148,21 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
     // down: pre-smooth, residual, restrict (phi of the next level zeroed)
     for (int l = 0; l < nl - 1; l++) {
         const CoarseLevel& L = P->lev[l];
+        Pts pts;
+        level_points(L, pts);
         int cur = PA;
-        jacobi_sweeps(L, ptr[l], L.pre, cur, ph);
-        const int n = L.g.n[0] * L.g.n[1] * L.g.n[2];
+        jacobi_sweeps(L, pts, ptr[l], L.pre, cur);
         const double* __restrict__ phi = ptr[l][cur];
         const double* __restrict__ rho = ptr[l][RHO];
         double* __restrict__ res = ptr[l][RES];
-        for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
-            int i0, i1, i2;
-            split(idx, L.g, i0, i1, i2);
-            res[idx] = star7(phi, i0, i1, i2, L.g, ph, L.c, true) - rho[idx];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int idx = pts.idx[q];
+            if (idx < 0) continue;
+            double c[7];
+#pragma unroll
+            for (int m = 0; m < 7; m++) c[m] = L.c[m];
+            res[idx] = nb_sum(phi, pts.nb[q], c, 0.0 + phi[idx] * c[0]) - rho[idx];
         }
         __syncthreads();
         const CoarseLevel& N = P->lev[l + 1];
@@ -138,8 +179,10 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
     }
     {   // bottom: post-smooth only
         const CoarseLevel& L = P->lev[nl - 1];
+        Pts pts;
+        level_points(L, pts);
         int cur = PA;
-        jacobi_sweeps(L, ptr[nl - 1], L.post, cur, ph);
+        jacobi_sweeps(L, pts, ptr[nl - 1], L.post, cur);
     }
     // up: correct from the coarser level, post-smooth
     for (int l = nl - 2; l >= 0; l--) {
@@ -156,7 +199,9 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
             phi[idx] = phi[idx] - interp_point<double, 2>(coarse, i0, i1, i2, L.itp, ph);
         }
         __syncthreads();
-        jacobi_sweeps(L, ptr[l], L.post, cur, ph);
+        Pts pts;
+        level_points(L, pts);
+        jacobi_sweeps(L, pts, ptr[l], L.post, cur);
     }
     if (lds) {   // hand the first level's solution back where the caller expects it
         const CoarseLevel& G = P->lev[0];

@@ -70,8 +70,9 @@ struct Stencil {
     int* d_off = nullptr;
 };
 
-// Stencil launch-shape autotuner (stencil.hip): switch on / off (returns
-// the previous state) and write a text table of the choices made.
+// Stencil launch-shape autotuner (stencil.hip): switch on (1) / off (0), or
+// query (-1); returns the previous state.  autotune_report writes a text
+// table of the choices made.
 int autotune_set(int on);
 long autotune_report(char* buf, long len);
 

@@ -289,8 +289,10 @@ extern "C" int gpaw_poisson_new(int levels, gpaw_operator* const* ops,
 // all 7-point stars with K=2 transfers (the Poisson hierarchy below level 0).
 static int setup_coarse(gpaw_poisson* ps)
 {
+    // the fused kernel holds at most 2048 points per level (two per thread)
     const char* env = getenv("GPAW_AMD_COARSE_MAX");
-    const long long limit = env ? atoll(env) : 2000;
+    long long limit = env ? atoll(env) : 2000;
+    if (limit > 2048) limit = 2048;
     int lc = -1;
     for (int l = 1; l <= ps->L; l++)
         if (ps->npts[l] <= limit) {
@@ -346,8 +348,28 @@ static int setup_coarse(gpaw_poisson* ps)
     return GPAW_OK;
 }
 
+// With the launch-shape tuner on, one V-cycle is first run eagerly on the
+// plan's own buffers (their contents are overwritten by load_inputs), so
+// that the graphs record tuned shapes: launches inside a graph are never
+// timed.
+static int pretune(gpaw_poisson* ps, hipStream_t st)
+{
+    if (ps->tuned || !autotune_set(-1)) return GPAW_OK;
+    ps->tuned = 1;
+    hipError_t e = hipSuccess;
+    for (int l = 0; l <= ps->L && e == hipSuccess; l++) {
+        e = launch_fill(ps->phiA[l], DT_F64, ps->npts[l], st);
+        if (e == hipSuccess) e = launch_fill(ps->phiB[l], DT_F64, ps->npts[l], st);
+        if (e == hipSuccess) e = launch_fill(ps->rho[l], DT_F64, ps->npts[l], st);
+    }
+    if (e == hipSuccess) e = emit_cycle(ps, 0, st);
+    return gpaw_check(e, "poisson: tune");
+}
+
 static int load_inputs(gpaw_poisson* ps, const double* phi, const double* rho, hipStream_t st)
 {
+    int rc = pretune(ps, st);
+    if (rc) return rc;
     hipError_t e = launch_copy(ps->rho[0], rho, DT_F64, ps->npts[0], st);
     if (e == hipSuccess) e = launch_copy(ps->phiA[0], phi, DT_F64, ps->npts[0], st);
     return gpaw_check(e, "poisson: load");

@@ -31,6 +31,7 @@ struct gpaw_poisson {
     double* h_errs = nullptr;
     std::vector<hipGraphExec_t> chunk;   // index 2 * (k - 1) + state
     int use_graph = 1;
+    int tuned = 0;             // launch shapes tuned (when the tuner is on)
     int swaps = 0;             // 1 if level 0 does an odd number of sweeps per cycle
     int lc = -1;               // first level run by the fused coarse kernel (-1: none)
     gpaw::CoarsePlan* d_coarse = nullptr;

@@ -550,7 +550,7 @@ TuneVal tune(const TuneKey& key, const LaunchCfg& heuristic, const Stencil& s, i
 int autotune_set(int on)
 {
     const int prev = tune_enabled() ? 1 : 0;
-    g_tune_on.store(on > 0 ? 1 : 0);
+    if (on >= 0) g_tune_on.store(on > 0 ? 1 : 0);
     return prev;
 }
 

@@ -208,7 +208,9 @@ void gpaw_batch_free(gpaw_batch* batch);
  * shape).  With the tuner on, the first apply / relax / residual launch of
  * each (dtype, mode, stencil range, grid, band count) times the candidate
  * launch shapes of the star-stencil kernels on that very launch and keeps
- * the fastest; every shape gives bit-identical results.  Returns the
+ * the fastest; every shape gives bit-identical results.  A Poisson plan
+ * runs one V-cycle on its own buffers before recording its graphs, so that
+ * they use tuned shapes.  enable: 1 on, 0 off, -1 query.  Returns the
  * previous setting.  Off by default; GPAW_AMD_AUTOTUNE=1 also turns it on. */
 int gpaw_autotune(int enable);
 
