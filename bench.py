@@ -76,11 +76,19 @@ def eps_for(np, gd, rho, rel=1e-10, eps_ref=2e-10):
     return min(eps_ref, gd.dv * (rel * rn) ** 2), rn
 
 
-def event_time(torch, fn, reps):
+def event_time(torch, fn, reps, warm_s=0.03):
+    """Average seconds per call of fn over `reps` calls, HIP events on the
+    current stream, after at least `warm_s` seconds of untimed calls (the
+    GPU clocks ramp up under load; short warm-ups read 10% slow)."""
     st = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
     fn()
+    torch.cuda.synchronize()
+    first = time.perf_counter() - t0
+    for _ in range(min(200, max(2, int(warm_s / max(first, 1e-6))))):
+        fn()
     torch.cuda.synchronize()
     e0.record(st)
     for _ in range(reps):

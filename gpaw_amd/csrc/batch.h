@@ -28,7 +28,7 @@ struct BatchGroup {
 };
 
 template <class V, int R, int TZ>
-__global__ __launch_bounds__(256) void k_star_tile_batch(const BatchGroup* __restrict__ groups,
+__global__ __launch_bounds__(TILE_NT) void k_star_tile_batch(const BatchGroup* __restrict__ groups,
                                                          int ngroups, int ntiles)
 {
     constexpr int ROWS = tile_rows(R, (int)sizeof(V));
@@ -74,7 +74,7 @@ int batch_resident()
         hipGetDevice(&dev);
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, (const void*)k_star_tile_batch<V, R, TZ>, 256, 0) != hipSuccess ||
+                &per_cu, (const void*)k_star_tile_batch<V, R, TZ>, TILE_NT, 0) != hipSuccess ||
             per_cu < 1)
             per_cu = 1;
         n = per_cu * (cus > 0 ? cus : 256);
@@ -90,7 +90,7 @@ hipError_t batch_go(const BatchGroup* d_groups, int ngroups, int ntiles, hipStre
         nb = ntiles;
     else if (nb >= 8)
         nb &= ~7;   // whole rounds of 8 XCDs
-    return launch_kernel(k_star_tile_batch<V, R, TZ>, dim3(nb), dim3(TZ, 256 / TZ), 0, st,
+    return launch_kernel(k_star_tile_batch<V, R, TZ>, dim3(nb), dim3(TZ, TILE_NT / TZ), 0, st,
                          d_groups, ngroups, ntiles);
 }
 

@@ -344,11 +344,14 @@ int tile_tz(int)
     return 32;
 }
 
-bool tile_geometry(const Geom& g, int R, int esize, int nin, int TZ, int q, TileShape& sh)
+bool tile_geometry(const Geom& g, int R, int esize, int nin, int TZ, int q, TileShape& sh,
+                   int nt)
 {
     if (R < 1 || R > 4 || (TZ != 32 && TZ != 64) || q < 1 || nin < 1) return false;
+    if (nt != TILE_NT && nt != TILE_NT_WIDE) return false;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
-    const int TY = (256 / TZ) * tile_rows(R, esize);
+    const int TY = (nt / TZ) * tile_rows(R, esize);
+    sh.nt = nt;
     sh.ntz = (n2 + TZ - 1) / TZ;
     sh.nty = (n1 + TY - 1) / TY;
     // every coordinate a tile touches (tile + halo) must lie within one
@@ -363,34 +366,34 @@ bool tile_geometry(const Geom& g, int R, int esize, int nin, int TZ, int q, Tile
     return true;
 }
 
-int tile_default_q(const Geom& g, int R, int esize, int nin, int TZ)
+int tile_default_q(const Geom& g, int R, int esize, int nin, int TZ, int nt)
 {
     // enough tiles to fill the chip twice at 8 blocks per CU, runs of >= 16
     // planes (each run re-reads 2R planes of queue)
     const int n0 = g.n[0];
-    const int TY = (256 / TZ) * tile_rows(R, esize);
-    const long nt = (long)((g.n[2] + TZ - 1) / TZ) * ((g.n[1] + TY - 1) / TY) * nin;
-    long nch = (4096 + nt - 1) / nt;
+    const int TY = (nt / TZ) * tile_rows(R, esize);
+    const long ntile = (long)((g.n[2] + TZ - 1) / TZ) * ((g.n[1] + TY - 1) / TY) * nin;
+    long nch = (4096 * TILE_NT / nt + ntile - 1) / ntile;
     const long maxch = n0 / 16 > 1 ? n0 / 16 : 1;
     if (nch > maxch) nch = maxch;
     if (nch < 1) nch = 1;
     return (int)((n0 + nch - 1) / nch);
 }
 
-bool tile_shape(const Geom& g, int R, int esize, int nin, int& TZ, TileShape& sh)
+bool tile_shape(const Geom& g, int R, int esize, int nin, int& TZ, TileShape& sh, int nt)
 {
     if (R < 1 || R > 4) return false;
     TZ = tile_tz(g.n[2]);
-    if (!tile_geometry(g, R, esize, nin, TZ, tile_default_q(g, R, esize, nin, TZ), sh))
+    if (!tile_geometry(g, R, esize, nin, TZ, tile_default_q(g, R, esize, nin, TZ, nt), sh, nt))
         return false;
-    return sh.nblocks >= 256;
+    return (long)sh.nblocks * nt >= 256L * TILE_NT;
 }
 
 static int stencil_kind(const Stencil& s, const Geom& g, int esize, int nin, int& TZ,
-                        TileShape& sh)
+                        TileShape& sh, int nt = TILE_NT)
 {
     const int f = forced_kind();
-    const bool tile_ok = s.star_r > 0 && tile_shape(g, s.star_r, esize, nin, TZ, sh);
+    const bool tile_ok = s.star_r > 0 && tile_shape(g, s.star_r, esize, nin, TZ, sh, nt);
     if (f == K_GENERIC && s.d_coefs) return K_GENERIC;
     if (f == K_QUEUE && s.star_r > 0) return K_QUEUE;
     if (tile_ok && f != K_QUEUE) return K_TILE;
@@ -514,15 +517,19 @@ TuneVal tune(const TuneKey& key, const LaunchCfg& heuristic, const Stencil& s, i
     std::vector<LaunchCfg> cand;
     LaunchCfg qc{K_QUEUE, 64, TileShape{}};
     cand.push_back(qc);
-    for (int TZ : {64, 32})
-        for (int q : {8, 16, 32, 64, 1 << 20}) {
-            LaunchCfg c{K_TILE, TZ, TileShape{}};
-            if (!tile_geometry(g, s.star_r, dtype_size(dtype), nin, TZ, q, c.sh)) continue;
-            bool dup = false;
-            for (const LaunchCfg& o : cand)
-                dup |= o.kind == K_TILE && o.TZ == TZ && o.sh.q == c.sh.q;
-            if (!dup) cand.push_back(c);
-        }
+    const bool wide = dtype_complex(dtype) && mode == ST_APPLY && s.star_r <= 3;
+    for (int nt : {TILE_NT, TILE_NT_WIDE}) {
+        if (nt == TILE_NT_WIDE && !wide) continue;
+        for (int TZ : {64, 32})
+            for (int q : {8, 16, 32, 64, 1 << 20}) {
+                LaunchCfg c{K_TILE, TZ, TileShape{}};
+                if (!tile_geometry(g, s.star_r, dtype_size(dtype), nin, TZ, q, c.sh, nt)) continue;
+                bool dup = false;
+                for (const LaunchCfg& o : cand)
+                    dup |= o.kind == K_TILE && o.TZ == TZ && o.sh.q == c.sh.q && o.sh.nt == nt;
+                if (!dup) cand.push_back(c);
+            }
+    }
     TuneVal best{heuristic, -1.0f, 0};
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return best;
@@ -564,10 +571,11 @@ long autotune_report(char* buf, long len)
         const TuneVal& v = kv.second;
         snprintf(line, sizeof line,
                  "dtype=%d mode=%d R=%d n=%dx%dx%d periodic=%d nin=%d -> %s TZ=%d q=%d "
-                 "%.2f us (%d candidates)\n",
+                 "threads=%d %.2f us (%d candidates)\n",
                  k.dtype, k.mode, k.R, k.n0, k.n1, k.n2, k.per, k.nin,
                  v.cfg.kind == K_TILE ? "tile" : "queue", v.cfg.kind == K_TILE ? v.cfg.TZ : 0,
-                 v.cfg.kind == K_TILE ? v.cfg.sh.q : 0, v.us, v.candidates);
+                 v.cfg.kind == K_TILE ? v.cfg.sh.q : 0, v.cfg.kind == K_TILE ? v.cfg.sh.nt : 256,
+                 v.us, v.candidates);
         r += line;
     }
     if (buf && len > 0) {
@@ -583,7 +591,10 @@ hipError_t launch_stencil(const Stencil& s, int dtype, int mode, const void* in,
                           double* partial, int* npartial, bool write, hipStream_t st)
 {
     LaunchCfg c{K_GENERIC, 64, TileShape{}};
-    c.kind = stencil_kind(s, g, dtype_size(dtype), nin, c.TZ, c.sh);
+    // complex applies of range <= 3 use 1024-thread tiles
+    const int nt = (dtype_complex(dtype) && mode == ST_APPLY && s.star_r <= 3) ? TILE_NT_WIDE
+                                                                              : TILE_NT;
+    c.kind = stencil_kind(s, g, dtype_size(dtype), nin, c.TZ, c.sh, nt);
     if (npartial) *npartial = stencil_partials(s, g, nin);
     if (!partial && s.star_r >= 1 && s.star_r <= 4 && forced_kind() < 0 && tune_enabled()) {
         const TuneKey key{dtype, mode, s.star_r, g.n[0], g.n[1], g.n[2],

@@ -26,6 +26,12 @@
 
 namespace gpaw {
 
+// threads per tile block: 256, or 1024 for complex applies of range <= 3
+// (taller tiles, fewer halo rows: measured 7% faster on complex128 bands,
+// slower on float64)
+constexpr int TILE_NT = 256;
+constexpr int TILE_NT_WIDE = 1024;
+
 struct TileCoefs {
     double c[6 * GPAW_MAXR + 1];
 };
@@ -33,6 +39,7 @@ struct TileCoefs {
 struct TileShape {
     int ntz, nty, nchunk, q;   // tiles along i2, i1; chunks along i0 of q planes
     int nblocks;
+    int nt = TILE_NT;          // threads per block
 };
 
 template <class V>
@@ -45,14 +52,14 @@ template <> __device__ __forceinline__ double tnorm2<float2>(float2 v)
     return (double)v.x * (double)v.x + (double)v.y * (double)v.y;
 }
 
-template <class V, int R, int TZ, int ROWS>
+template <class V, int R, int TZ, int ROWS, int NT = TILE_NT>
 struct TileGeom {
-    static constexpr int NTY = 256 / TZ;
+    static constexpr int NTY = NT / TZ;
     static constexpr int TY = NTY * ROWS;
     static constexpr int LW = TZ + 2 * R;              // LDS row length
     static constexpr int LH = TY + 2 * R;              // LDS rows
     static constexpr int NHALO = 2 * R * (TZ + TY);    // halo cells per plane
-    static constexpr int NH = (NHALO + 255) / 256;     // halo cells per thread
+    static constexpr int NH = (NHALO + NT - 1) / NT;   // halo cells per thread
     static constexpr int LDS = 2 * LH * LW;            // two plane buffers
 };
 
@@ -61,14 +68,15 @@ struct TileGeom {
 // arrays; lds holds TileGeom::LDS elements.  Every thread of the block
 // calls it with the same tile.  Returns the thread's sum of |result|^2
 // (NORM).
-template <class V, int R, int TZ, int ROWS, int MODE, bool NORM, bool WRITE, bool SEL = false>
+template <class V, int R, int TZ, int ROWS, int MODE, bool NORM, bool WRITE, bool SEL = false,
+          int NT = TILE_NT>
 __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __restrict__ o,
                                                   const V* __restrict__ sa, const Geom g,
                                                   const Phases ph, const TileCoefs sc, double w,
                                                   int z0, int y0, int i0s, int i0e, V* lds)
 {
     typedef typename VT<V>::R RT;
-    typedef TileGeom<V, R, TZ, ROWS> TG;
+    typedef TileGeom<V, R, TZ, ROWS, NT> TG;
     constexpr int TY = TG::TY, LW = TG::LW, LH = TG::LH, NHALO = TG::NHALO, NH = TG::NH;
 
     const int tx = threadIdx.x, tyi = threadIdx.y;
@@ -89,7 +97,7 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
     int hoff[NH], hy[NH], hz[NH];
 #pragma unroll
     for (int k = 0; k < NH; k++) {
-        const int h = tid + 256 * k;
+        const int h = tid + NT * k;
         int lr, lc;
         if (h < 2 * R * TZ) {
             const int yr = h / TZ;
@@ -198,13 +206,13 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
     return acc;
 }
 
-template <class V, int R, int TZ, int ROWS, int MODE, bool NORM, bool WRITE>
-__global__ __launch_bounds__(256) void k_star_tile(const V* __restrict__ in, V* __restrict__ out,
+template <class V, int R, int TZ, int ROWS, int MODE, bool NORM, bool WRITE, int NT>
+__global__ __launch_bounds__(NT) void k_star_tile(const V* __restrict__ in, V* __restrict__ out,
                                                    const V* __restrict__ aux, Geom g, Phases ph,
                                                    TileCoefs sc, double w, TileShape sh, i64 ng,
                                                    double* __restrict__ partial)
 {
-    typedef TileGeom<V, R, TZ, ROWS> TG;
+    typedef TileGeom<V, R, TZ, ROWS, NT> TG;
     __shared__ V lds[TG::LDS];
     const int b = blockIdx.x;
     const int t = xcd_tile(b, sh.nblocks);
@@ -216,12 +224,12 @@ __global__ __launch_bounds__(256) void k_star_tile(const V* __restrict__ in, V* 
     const int band = rest / sh.nchunk;
     const i64 boff = (i64)band * ng;
     const int i0s = ch * sh.q;
-    const double acc = star_tile_sweep<V, R, TZ, ROWS, MODE, NORM, WRITE>(
+    const double acc = star_tile_sweep<V, R, TZ, ROWS, MODE, NORM, WRITE, false, NT>(
         in + boff, out + boff, aux ? aux + boff : nullptr, g, ph, sc, w, tz * TZ, ty * TG::TY,
         i0s, min(i0s + sh.q, g.n[0]), lds);
     if (NORM) {
-        __shared__ double red[256];
-        const double sum = block_sum<256>(acc, red);
+        __shared__ double red[NT];
+        const double sum = block_sum<NT>(acc, red);
         if (threadIdx.x == 0 && threadIdx.y == 0) partial[b] = sum;
     }
 }
@@ -236,9 +244,11 @@ constexpr int tile_rows(int, int) { return 1; }
 // host-side shape selection and launch (stencil_tile_*.hip)
 int tile_tz(int n2);
 // tiles of TZ columns and q planes; false if a tile's halo would wrap twice
-bool tile_geometry(const Geom& g, int R, int esize, int nin, int TZ, int q, TileShape& sh);
-int tile_default_q(const Geom& g, int R, int esize, int nin, int TZ);
-bool tile_shape(const Geom& g, int R, int esize, int nin, int& TZ, TileShape& sh);
+bool tile_geometry(const Geom& g, int R, int esize, int nin, int TZ, int q, TileShape& sh,
+                   int nt = TILE_NT);
+int tile_default_q(const Geom& g, int R, int esize, int nin, int TZ, int nt = TILE_NT);
+bool tile_shape(const Geom& g, int R, int esize, int nin, int& TZ, TileShape& sh,
+                int nt = TILE_NT);
 hipError_t launch_stencil_tile(const Stencil& s, int dtype, int mode, const void* in, void* out,
                                const void* aux, const Geom& g, const Phases& ph, int nin,
                                double w, double* partial, int* npartial, bool write,

@@ -14,9 +14,17 @@ hipError_t tile_go(const Stencil& s, const void* in, void* out, const void* aux,
     TileCoefs sc;
     for (int c = 0; c < 6 * R + 1; c++) sc.c[c] = s.coefs[c];
     const i64 ng = (i64)g.n[0] * g.n[1] * g.n[2];
-    launch_kernel((k_star_tile<V, R, TZ, ROWS, MODE, NORM, WRITE>), dim3(sh.nblocks),
-                  dim3(TZ, 256 / TZ), 0, st, (const V*)in, (V*)out, (const V*)aux, g, ph, sc, w,
-                  sh, ng, partial);
+    if constexpr (VT<V>::cplx && MODE == ST_APPLY && R <= 3) {
+        if (sh.nt == TILE_NT_WIDE)
+            return launch_kernel((k_star_tile<V, R, TZ, ROWS, MODE, NORM, WRITE, TILE_NT_WIDE>),
+                                 dim3(sh.nblocks), dim3(TZ, TILE_NT_WIDE / TZ), 0, st,
+                                 (const V*)in, (V*)out, (const V*)aux, g, ph, sc, w, sh, ng,
+                                 partial);
+    }
+    if (sh.nt != TILE_NT) return hipErrorInvalidValue;
+    launch_kernel((k_star_tile<V, R, TZ, ROWS, MODE, NORM, WRITE, TILE_NT>), dim3(sh.nblocks),
+                  dim3(TZ, TILE_NT / TZ), 0, st, (const V*)in, (V*)out, (const V*)aux, g, ph, sc,
+                  w, sh, ng, partial);
     return launch_error();
 }
 
