@@ -225,7 +225,7 @@ def preconditioner_c3(torch, np):
             'bytes_per_call': algo}
 
 
-def mixed_vs_fp64(torch, np, N=160, h=0.18, count=32, reps=3):
+def mixed_vs_fp64(torch, np, N=160, h=0.18, count=32, reps=10):
     """Mixed-precision vs plain FP64 multigrid on the 160^3 zero-boundary
     configuration (159^3 arrays, h = 0.18 A, 32 seeded neutral Gaussians of
     widths 0.4-1.2 A), both to ||L phi - rho|| / ||rho|| <= 1e-10."""
@@ -246,7 +246,9 @@ def mixed_vs_fp64(torch, np, N=160, h=0.18, count=32, reps=3):
         solver.set_grid_descriptor(gd)
         solver.initialize()
         phi = torch.zeros(tuple(gd.n_c), dtype=torch.float64, device='cuda')
-        it = solver.solve(phi, rho_d)          # warm-up (graph capture)
+        for _ in range(2):                     # warm-up (tuning, graphs)
+            phi.zero_()
+            it = solver.solve(phi, rho_d)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
@@ -284,7 +286,11 @@ def serving(torch, np, n=2000, rate=300.0, slots=4, budget_gb=64):
     48..160, half periodic), data resident in HBM, served by the
     continuous-batching server; one warm-up request per key first so that
     hierarchies are resident."""
+    from gpaw_amd import ops
     from gpaw_amd.serving import Request, Server, synthetic_stream
+    # requests come with arbitrary band counts: no shape tuning on the
+    # serving path (the heuristic shapes are used)
+    ops.autotune(False)
     t0 = time.perf_counter()
     reqs = synthetic_stream(n=n, rate=rate, seed=5)
     setup = time.perf_counter() - t0

@@ -22,6 +22,8 @@
 // Scalars (alpha, beta, dot products) stay on the device; the host reads
 // one 8-byte norm per outer iteration.  The FP32 V-cycle and the per-
 // iteration kernel sequences are captured as hipGraphs.
+#include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 
@@ -44,7 +46,12 @@ struct MixedState {
     hipGraphExec_t g_vwarm = nullptr;   // one FP32 V-cycle from the last result
     hipGraphExec_t g_a = nullptr;       // PCG: q = Lp, alpha, update, |d|^2
     hipGraphExec_t g_b = nullptr;       // PCG: z = V32(d), beta, p = z + beta p
+    // k PCG steps (b then a) per launch, k = 1 .. MIXED_CHUNK_MAX
+    std::vector<hipGraphExec_t> g_ba;
+    int tuned = 0;                      // launch shapes tuned (tuner on)
 };
+
+#define MIXED_CHUNK_MAX 8
 
 // device scalars
 enum { S_DZ = 0, S_PQ = 1, S_DD = 2, S_ZQ = 3, S_DZOLD = 4, S_ALPHA = 5, S_N = 8 };
@@ -271,48 +278,77 @@ void destroy_graphs(MixedState* m)
         if (*g) hipGraphExecDestroy(*g);
         *g = nullptr;
     }
+    for (hipGraphExec_t& g : m->g_ba) {
+        if (g) hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
+
+// PCG step part a: q = L p, alpha, phi += alpha p, d -= alpha q, |d|^2
+hipError_t emit_a(gpaw_poisson* ps, MixedState* m, hipStream_t st)
+{
+    const i64 n = ps->npts[0];
+    gpaw_operator* op = ps->ops[0];
+    Phases ph;
+    gpaw_phases(nullptr, ph);
+    hipError_t e = launch_stencil(op->st, DT_F64, ST_APPLY, m->p, m->q, nullptr, op->g, ph, 1,
+                                  0.0, nullptr, nullptr, true, st);
+    if (e != hipSuccess) return e;
+    launch_kernel(k_dot, gridn(n), dim3(256), 0, st, m->p, m->q, n, m->partial);
+    launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x, m->S,
+                  (int)S_PQ);
+    launch_kernel(k_pcg_update, gridn(n), dim3(256), 0, st, ps->phiA[0], ps->res[0], m->p, m->q,
+                  m->rho[0], m->S, n, m->partial);
+    launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x, m->S,
+                  (int)S_DD);
+    return launch_error();
+}
+
+// PCG step part b: z = V32(d), beta (flexible), p = z + beta p
+hipError_t emit_b(gpaw_poisson* ps, MixedState* m, hipStream_t st)
+{
+    const i64 n = ps->npts[0];
+    hipError_t e = emit32(ps, m, 0, true, st);
+    if (e != hipSuccess) return e;
+    launch_kernel(k_dot_dz_zq, gridn(n), dim3(256), 0, st, ps->res[0], z_of(m), m->q, n,
+                  m->partial);
+    launch_kernel(k_red2, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x, m->S);
+    launch_kernel(k_p_update, gridn(n), dim3(256), 0, st, m->p, z_of(m), m->S, n, 0);
+    return launch_error();
 }
 
 int capture_all(gpaw_poisson* ps)
 {
     MixedState* m = ps->mixed;
     if (m->g_b) return GPAW_OK;   // the last one built: all four exist
-    const i64 n = ps->npts[0];
-    gpaw_operator* op = ps->ops[0];
     int rc = capture_graph(&m->g_vcycle,
                            [&](hipStream_t st) { return emit32(ps, m, 0, true, st); });
     if (!rc)
         rc = capture_graph(&m->g_vwarm,
                            [&](hipStream_t st) { return emit32(ps, m, 0, false, st); });
-    if (!rc)
-        rc = capture_graph(&m->g_a, [&](hipStream_t st) -> hipError_t {
-            Phases ph;
-            gpaw_phases(nullptr, ph);
-            hipError_t e = launch_stencil(op->st, DT_F64, ST_APPLY, m->p, m->q, nullptr, op->g,
-                                          ph, 1, 0.0, nullptr, nullptr, true, st);
-            if (e != hipSuccess) return e;
-            launch_kernel(k_dot, gridn(n), dim3(256), 0, st, m->p, m->q, n, m->partial);
-            launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x,
-                               m->S, (int)S_PQ);
-            launch_kernel(k_pcg_update, gridn(n), dim3(256), 0, st, ps->phiA[0], ps->res[0],
-                               m->p, m->q, m->rho[0], m->S, n, m->partial);
-            launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x,
-                               m->S, (int)S_DD);
-            return launch_error();
-        });
-    if (!rc)
-        rc = capture_graph(&m->g_b, [&](hipStream_t st) -> hipError_t {
-            hipError_t e = emit32(ps, m, 0, true, st);
-            if (e != hipSuccess) return e;
-            launch_kernel(k_dot_dz_zq, gridn(n), dim3(256), 0, st, ps->res[0], z_of(m), m->q,
-                               n, m->partial);
-            launch_kernel(k_red2, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x,
-                               m->S);
-            launch_kernel(k_p_update, gridn(n), dim3(256), 0, st, m->p, z_of(m), m->S, n, 0);
-            return launch_error();
-        });
+    if (!rc) rc = capture_graph(&m->g_a, [&](hipStream_t st) { return emit_a(ps, m, st); });
+    if (!rc) rc = capture_graph(&m->g_b, [&](hipStream_t st) { return emit_b(ps, m, st); });
+    if (!rc) m->g_ba.assign(MIXED_CHUNK_MAX, nullptr);
     if (rc) destroy_graphs(m);   // retried from scratch on the next call
     return rc;
+}
+
+// k PCG steps (b, a) in one graph
+int run_ba(gpaw_poisson* ps, MixedState* m, int k, hipStream_t st)
+{
+    hipGraphExec_t& g = m->g_ba[k - 1];
+    if (!g) {
+        int rc = capture_graph(&g, [&](hipStream_t s) -> hipError_t {
+            hipError_t e = hipSuccess;
+            for (int i = 0; i < k && e == hipSuccess; i++) {
+                e = emit_b(ps, m, s);
+                if (e == hipSuccess) e = emit_a(ps, m, s);
+            }
+            return e;
+        });
+        if (rc) return rc;
+    }
+    return gpaw_check(hipGraphLaunch(g, st), "mixed: iterations");
 }
 
 int read_scalar(MixedState* m, int slot, hipStream_t st, double* v)
@@ -364,6 +400,14 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
         return gpaw_fail(GPAW_EINVAL, "mixed: bad argument");
     hipStream_t st = (hipStream_t)stream;
     int rc = setup(ps);
+    if (!rc && !ps->mixed->tuned && autotune_set(-1)) {
+        // one eager pass of the graph bodies on the plan's own buffers (all
+        // reloaded below), so that the graphs record tuned launch shapes
+        ps->mixed->tuned = 1;
+        hipError_t e = emit32(ps, ps->mixed, 0, true, st);
+        if (e == hipSuccess) e = emit_a(ps, ps->mixed, st);
+        rc = gpaw_check(e, "mixed: tune");
+    }
     if (!rc) rc = capture_all(ps);
     if (rc) return rc;
     MixedState* m = ps->mixed;
@@ -395,12 +439,30 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
             launch_kernel(k_p_update, gridn(n), dim3(256), 0, st, m->p, z_of(m), m->S, n, 1);
             rc = gpaw_check(launch_error(), "mixed: start");
         }
+        // PCG steps run in chunks of k (b, a) between readbacks of |d|^2; a
+        // chunk may run a few steps past the tolerance, which only makes the
+        // result more accurate.  k follows the observed convergence rate.
+        if (!rc) rc = gpaw_check(hipGraphLaunch(m->g_a, st), "mixed: iteration");
+        if (!rc) rc = read_scalar(m, S_DD, st, &dd);
+        it = 1;
+        double prev = dd;
+        int k = 2;
         while (!rc && dd > target && it < maxiter) {
-            rc = gpaw_check(hipGraphLaunch(m->g_a, st), "mixed: iteration");
+            if (k > maxiter - it) k = maxiter - it;
+            rc = run_ba(ps, m, k, st);
             if (!rc) rc = read_scalar(m, S_DD, st, &dd);
-            it++;
-            if (!rc && dd > target && it < maxiter)
-                rc = gpaw_check(hipGraphLaunch(m->g_b, st), "mixed: preconditioner");
+            it += k;
+            // steps still needed at the rate of this chunk (|d|^2 per step)
+            const int done = k;
+            k = MIXED_CHUNK_MAX;
+            if (!rc && dd > target && dd < prev && prev > 0) {
+                const double rate = std::pow(dd / prev, 1.0 / done);
+                if (rate > 0 && rate < 1) {
+                    const double need = std::ceil(std::log(target / dd) / std::log(rate));
+                    if (need >= 1 && need < MIXED_CHUNK_MAX) k = (int)need;
+                }
+            }
+            prev = dd;
         }
     } else if (!rc && mode == 0) {
         while (!rc && dd > target && it < maxiter) {
