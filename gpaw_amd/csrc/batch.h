@@ -57,7 +57,7 @@ __global__ __launch_bounds__(TILE_NT) void k_star_tile_batch(const BatchGroup* _
         const int band = r / sh.nchunk;
         const i64 boff = (i64)band * gp->ng;
         const int i0s = ch * sh.q;
-        star_tile_sweep<V, R, TZ, ROWS, ST_APPLY, false, true, true>(
+        star_tile_sweep<V, R, TZ, ROWS, ST_APPLY, false, true>(
             (const V*)gp->in + boff, (V*)gp->out + boff, nullptr, gp->g, gp->ph, gp->sc, 0.0,
             tz * TZ, ty * TG::TY, i0s, min(i0s + sh.q, gp->g.n[0]), lds);
         __syncthreads();   // the next tile rewrites the LDS plane buffers

@@ -350,7 +350,7 @@ bool tile_geometry(const Geom& g, int R, int esize, int nin, int TZ, int q, Tile
     if (R < 1 || R > 4 || (TZ != 32 && TZ != 64) || q < 1 || nin < 1) return false;
     if (nt != TILE_NT && nt != TILE_NT_WIDE) return false;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
-    const int TY = (nt / TZ) * tile_rows(R, esize);
+    const int TY = (nt / TZ) * tile_rows(R, esize, nt);
     sh.nt = nt;
     sh.ntz = (n2 + TZ - 1) / TZ;
     sh.nty = (n1 + TY - 1) / TY;
@@ -371,7 +371,7 @@ int tile_default_q(const Geom& g, int R, int esize, int nin, int TZ, int nt)
     // enough tiles to fill the chip twice at 8 blocks per CU, runs of >= 16
     // planes (each run re-reads 2R planes of queue)
     const int n0 = g.n[0];
-    const int TY = (nt / TZ) * tile_rows(R, esize);
+    const int TY = (nt / TZ) * tile_rows(R, esize, nt);
     const long ntile = (long)((g.n[2] + TZ - 1) / TZ) * ((g.n[1] + TY - 1) / TY) * nin;
     long nch = (4096 * TILE_NT / nt + ntile - 1) / ntile;
     const long maxch = n0 / 16 > 1 ? n0 / 16 : 1;
@@ -450,7 +450,7 @@ static hipError_t launch_cfg(const LaunchCfg& c, const Stencil& s, int dtype, in
 // ---------------------------------------------------------------------------
 // Autotuner.  For a star stencil of range 1..4 the launch shape -- the
 // register-queue kernel, or the tile kernel with TZ = 32 / 64 columns and
-// q = 8 .. n0 planes per tile -- is chosen by timing every candidate on the
+// q = 1 .. n0 planes per tile -- is chosen by timing every candidate on the
 // caller's own launch the first time a (type, mode, range, grid, bands)
 // combination is seen.  Every shape computes identical bits and input and
 // output never alias, so running the candidates repeats the same result.
@@ -492,22 +492,32 @@ bool tune_enabled()
     return v > 0;
 }
 
-// time `reps` launches of one candidate (after one untimed launch)
+// mean time of one candidate's launch (after one untimed launch)
 float time_cfg(const LaunchCfg& c, const Stencil& s, int dtype, int mode, const void* in,
                void* out, const void* aux, const Geom& g, const Phases& ph, int nin, double w,
                bool write, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
-    const int reps = 3;
     if (launch_cfg(c, s, dtype, mode, in, out, aux, g, ph, nin, w, nullptr, write, st) != hipSuccess)
         return -1.0f;
-    hipEventRecord(e0, st);
-    for (int r = 0; r < reps; r++)
-        launch_cfg(c, s, dtype, mode, in, out, aux, g, ph, nin, w, nullptr, write, st);
-    hipEventRecord(e1, st);
-    if (hipEventSynchronize(e1) != hipSuccess || hipGetLastError() != hipSuccess) return -1.0f;
-    float ms = 0.0f;
-    hipEventElapsedTime(&ms, e0, e1);
-    return 1e3f * ms / reps;
+    // 3 launches, then (for short kernels) enough launches for ~300 us so
+    // that launches of a few microseconds are told apart
+    float us = 0.0f;
+    for (int reps = 3, pass = 0; pass < 2; pass++) {
+        hipEventRecord(e0, st);
+        for (int r = 0; r < reps; r++)
+            launch_cfg(c, s, dtype, mode, in, out, aux, g, ph, nin, w, nullptr, write, st);
+        hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) != hipSuccess || hipGetLastError() != hipSuccess)
+            return -1.0f;
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, e0, e1);
+        us = 1e3f * ms / reps;
+        if (us * reps >= 100.0f) break;
+        reps = us > 0.0f ? (int)(300.0f / us) : 64;
+        if (reps > 64) reps = 64;
+        if (reps < 3) break;
+    }
+    return us;
 }
 
 TuneVal tune(const TuneKey& key, const LaunchCfg& heuristic, const Stencil& s, int dtype,
@@ -521,7 +531,7 @@ TuneVal tune(const TuneKey& key, const LaunchCfg& heuristic, const Stencil& s, i
     for (int nt : {TILE_NT, TILE_NT_WIDE}) {
         if (nt == TILE_NT_WIDE && !wide) continue;
         for (int TZ : {64, 32})
-            for (int q : {8, 16, 32, 64, 1 << 20}) {
+            for (int q : {1, 2, 4, 8, 16, 32, 64, 1 << 20}) {
                 LaunchCfg c{K_TILE, TZ, TileShape{}};
                 if (!tile_geometry(g, s.star_r, dtype_size(dtype), nin, TZ, q, c.sh, nt)) continue;
                 bool dup = false;

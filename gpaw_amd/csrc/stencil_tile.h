@@ -31,6 +31,7 @@ namespace gpaw {
 // slower on float64)
 constexpr int TILE_NT = 256;
 constexpr int TILE_NT_WIDE = 1024;
+constexpr bool TILE_STREAM_STORE = false;
 
 struct TileCoefs {
     double c[6 * GPAW_MAXR + 1];
@@ -63,13 +64,69 @@ struct TileGeom {
     static constexpr int LDS = 2 * LH * LW;            // two plane buffers
 };
 
+// In-plane part of a ghost reference: the wrapped offset within a plane and
+// the wrap flags of axes 1 and 2 (wrap_index codes, f1 | f2 << 2).  A
+// reference to a zero-boundary ghost points at offset 0 (any valid address:
+// the value is discarded).
+struct PlaneRef {
+    int off;
+    int code;
+};
+
+__device__ __forceinline__ PlaneRef plane_ref(int j1, int j2, const Geom& g)
+{
+    const int f1 = wrap_index(j1, g.n[1], g.per[1]);
+    const int f2 = wrap_index(j2, g.n[2], g.per[2]);
+    PlaneRef r;
+    r.code = f1 | (f2 << 2);
+    r.off = (f1 == 3 || f2 == 3) ? 0 : j1 * g.n[2] + j2;
+    return r;
+}
+
+// Value of a ghost reference from its raw load: zero outside a zero-boundary
+// axis, else the phases of the wrapped sides applied in axis order 0, 1, 2
+// (the order of ghost_fetch).  Phase sides are selected, not indexed, so the
+// phases stay in scalar registers.
+template <class V>
+__device__ __forceinline__ V ghost_fix(V v, int f0, int code, const Phases& ph)
+{
+    const int f1 = code & 3, f2 = code >> 2;
+    if (f0 == 3 || f1 == 3 || f2 == 3) return vzero<V>();
+    if (VT<V>::cplx) {
+        if (f0) v = pside<true>(v, ph, 0, f0);
+        if (f1) v = pside<true>(v, ph, 1, f1);
+        if (f2) v = pside<true>(v, ph, 2, f2);
+    }
+    return v;
+}
+
+// Make a loaded value available at this point (an empty asm that reads it:
+// the compiler places the wait for the load before it; no instruction).
+__device__ __forceinline__ void vtouch(double v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void vtouch(float v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void vtouch(double2 v) { asm volatile("" ::"v"(v.x), "v"(v.y)); }
+__device__ __forceinline__ void vtouch(float2 v) { asm volatile("" ::"v"(v.x), "v"(v.y)); }
+
+template <class V>
+__device__ __forceinline__ void tile_store(V* p, V v)
+{
+    if (TILE_STREAM_STORE)
+        vstore_nt(p, v);
+    else
+        *p = v;
+}
+
 // One tile of one band: columns z0 .. z0+TZ-1, rows y0 .. y0+TY-1, planes
 // i0s .. i0e-1.  a / o / sa point at the band's input, output and aux
 // arrays; lds holds TileGeom::LDS elements.  Every thread of the block
 // calls it with the same tile.  Returns the thread's sum of |result|^2
 // (NORM).
-template <class V, int R, int TZ, int ROWS, int MODE, bool NORM, bool WRITE, bool SEL = false,
-          int NT = TILE_NT>
+//
+// Loads for the next plane are issued unconditionally from precomputed,
+// always-valid addresses and only corrected (zero / Bloch phase) when they
+// are consumed one plane later: no branch or phase lookup sits between a
+// load and the barrier, so the load latency overlaps a whole plane of work.
+template <class V, int R, int TZ, int ROWS, int MODE, bool NORM, bool WRITE, int NT = TILE_NT>
 __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __restrict__ o,
                                                   const V* __restrict__ sa, const Geom g,
                                                   const Phases ph, const TileCoefs sc, double w,
@@ -85,16 +142,19 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
     const i64 plane = (i64)n1 * n2;
     const int i2 = z0 + tx;
 
-    // owned rows
-    int i1r[ROWS];
+    // owned rows: output flag and in-plane reference (out-of-grid rows hold
+    // the ghost values their in-grid neighbours read)
     bool inr[ROWS];
+    PlaneRef qr[ROWS];
 #pragma unroll
     for (int j = 0; j < ROWS; j++) {
-        i1r[j] = y0 + tyi * ROWS + j;
-        inr[j] = (i1r[j] < n1) && (i2 < n2);
+        const int i1 = y0 + tyi * ROWS + j;
+        inr[j] = (i1 < n1) && (i2 < n2);
+        qr[j] = plane_ref(i1, i2, g);
     }
-    // halo cells of this thread: LDS offset and (i1, i2)
-    int hoff[NH], hy[NH], hz[NH];
+    // halo cells of this thread: LDS offset (-1: none) and in-plane reference
+    int hoff[NH];
+    PlaneRef hr[NH];
 #pragma unroll
     for (int k = 0; k < NH; k++) {
         const int h = tid + NT * k;
@@ -110,60 +170,90 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
             lc = zc < R ? zc : TZ + zc;
         }
         hoff[k] = (h < NHALO) ? lr * LW + lc : -1;
-        hy[k] = y0 + lr - R;
-        hz[k] = z0 + lc - R;
+        // threads without a halo cell load offset 0 of the plane (discarded)
+        hr[k] = (h < NHALO) ? plane_ref(y0 + lr - R, z0 + lc - R, g) : PlaneRef{0, 0};
     }
-
-    auto qload = [&](int p, int j) -> V {
-        if (inr[j]) {
-            int pp = p;
-            const int f = wrap_index(pp, n0, g.per[0]);
-            if (f == 3) return vzero<V>();
-            V v = a[(i64)pp * plane + (i64)i1r[j] * n2 + i2];
-            if (VT<V>::cplx && f) v = pside<SEL>(v, ph, 0, f);
-            return v;
-        }
-        return ghost_fetch<V, SEL>(a, p, i1r[j], i2, g, ph);
+    // plane p: wrap flag of axis 0 (uniform) and the plane's base offset
+    auto pflag = [&](int p, i64& base) -> int {
+        const int f = wrap_index(p, n0, g.per[0]);
+        base = f == 3 ? 0 : (i64)p * plane;
+        return f;
     };
 
     // planes i0-R .. i0+R of the owned rows; the queue plane and the halo of
-    // plane i0 + 1 are loaded during plane i0
+    // plane i0 + 1 are loaded raw during plane i0 (flags qf / hf)
     V q[ROWS][2 * R + 1];
-    V qa[ROWS];
-    V ha[NH];
+    V qa[ROWS], ha[NH], sn[ROWS];
+    int qf, hf;
 #pragma unroll
-    for (int j = 0; j < ROWS; j++) {
+    for (int k = 0; k < 2 * R; k++) {
+        i64 base;
+        const int f = pflag(i0s - R + k, base);
 #pragma unroll
-        for (int k = 0; k < 2 * R; k++) q[j][k] = qload(i0s - R + k, j);
-        qa[j] = qload(i0s + R, j);
+        for (int j = 0; j < ROWS; j++) q[j][k] = ghost_fix(a[base + qr[j].off], f, qr[j].code, ph);
     }
+    {
+        i64 base;
+        qf = pflag(i0s + R, base);
 #pragma unroll
-    for (int k = 0; k < NH; k++)
-        ha[k] = hoff[k] >= 0 ? ghost_fetch<V, SEL>(a, i0s, hy[k], hz[k], g, ph) : vzero<V>();
+        for (int j = 0; j < ROWS; j++) qa[j] = a[base + qr[j].off];
+        hf = pflag(i0s, base);
+#pragma unroll
+        for (int k = 0; k < NH; k++) ha[k] = a[base + hr[k].off];
+        if (MODE != ST_APPLY) {
+#pragma unroll
+            for (int j = 0; j < ROWS; j++) sn[j] = sa[(i64)i0s * plane + (inr[j] ? qr[j].off : 0)];
+        }
+    }
 
+    // Each plane step first consumes the loads issued one step earlier, then
+    // issues the store of the previous plane's result and the next plane's
+    // loads together, then computes.  Loads and stores share one counter
+    // that does not retire them in order, so a wait for a load also waits
+    // for every store issued before it: storing one step late keeps the
+    // store out of the next wait's critical path.
     double acc = 0.0;
+    V rp[ROWS];
     for (int i0 = i0s; i0 < i0e; i0++) {
         V* B = lds + ((i0 - i0s) & 1) * (LH * LW);
+        // wait for all of the step's loads here, on every path: a load
+        // consumed only inside a branch would make the next load into the
+        // same registers wait again after the new loads were issued
 #pragma unroll
         for (int j = 0; j < ROWS; j++) {
-            q[j][2 * R] = qa[j];
+            vtouch(qa[j]);
+            if (MODE != ST_APPLY) vtouch(sn[j]);
+        }
+#pragma unroll
+        for (int k = 0; k < NH; k++) vtouch(ha[k]);
+        V s[ROWS];
+#pragma unroll
+        for (int j = 0; j < ROWS; j++) {
+            q[j][2 * R] = ghost_fix(qa[j], qf, qr[j].code, ph);
             B[(R + tyi * ROWS + j) * LW + R + tx] = q[j][R];
+            if (MODE != ST_APPLY) s[j] = sn[j];
         }
 #pragma unroll
         for (int k = 0; k < NH; k++)
-            if (hoff[k] >= 0) B[hoff[k]] = ha[k];
-        V s[ROWS];
-        if (MODE != ST_APPLY) {
+            if (hoff[k] >= 0) B[hoff[k]] = ghost_fix(ha[k], hf, hr[k].code, ph);
+        if (WRITE && i0 > i0s) {
 #pragma unroll
             for (int j = 0; j < ROWS; j++)
-                s[j] = inr[j] ? sa[(i64)i0 * plane + (i64)i1r[j] * n2 + i2] : vzero<V>();
+                if (inr[j]) tile_store(o + (i64)(i0 - 1) * plane + qr[j].off, rp[j]);
         }
         if (i0 + 1 < i0e) {
+            i64 base;
+            qf = pflag(i0 + 1 + R, base);
 #pragma unroll
-            for (int j = 0; j < ROWS; j++) qa[j] = qload(i0 + 1 + R, j);
+            for (int j = 0; j < ROWS; j++) qa[j] = a[base + qr[j].off];
+            hf = pflag(i0 + 1, base);
 #pragma unroll
-            for (int k = 0; k < NH; k++)
-                if (hoff[k] >= 0) ha[k] = ghost_fetch<V, SEL>(a, i0 + 1, hy[k], hz[k], g, ph);
+            for (int k = 0; k < NH; k++) ha[k] = a[base + hr[k].off];
+            if (MODE != ST_APPLY) {
+#pragma unroll
+                for (int j = 0; j < ROWS; j++)
+                    sn[j] = sa[(i64)(i0 + 1) * plane + (inr[j] ? qr[j].off : 0)];
+            }
         }
         __syncthreads();
         V win[ROWS + 2 * R];
@@ -191,17 +281,19 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
             for (int k = 1; k <= R; k++) x = vadd(x, vmul(q[j][R + k], (RT)sc.c[4 * R + k]));
 #pragma unroll
             for (int k = 1; k <= R; k++) x = vadd(x, vmul(q[j][R - k], (RT)sc.c[5 * R + k]));
-            const V r = st_epilogue<MODE>(x, q[j][R], MODE == ST_APPLY ? vzero<V>() : s[j],
-                                          (RT)w, (RT)sc.c[0]);
-            if (inr[j]) {
-                if (NORM) acc += tnorm2(r);
-                if (WRITE) o[(i64)i0 * plane + (i64)i1r[j] * n2 + i2] = r;
-            }
+            rp[j] = st_epilogue<MODE>(x, q[j][R], MODE == ST_APPLY ? vzero<V>() : s[j], (RT)w,
+                                      (RT)sc.c[0]);
+            if (NORM && inr[j]) acc += tnorm2(rp[j]);
         }
 #pragma unroll
         for (int j = 0; j < ROWS; j++)
 #pragma unroll
             for (int k = 0; k < 2 * R; k++) q[j][k] = q[j][k + 1];
+    }
+    if (WRITE && i0e > i0s) {
+#pragma unroll
+        for (int j = 0; j < ROWS; j++)
+            if (inr[j]) tile_store(o + (i64)(i0e - 1) * plane + qr[j].off, rp[j]);
     }
     return acc;
 }
@@ -224,7 +316,7 @@ __global__ __launch_bounds__(NT) void k_star_tile(const V* __restrict__ in, V* _
     const int band = rest / sh.nchunk;
     const i64 boff = (i64)band * ng;
     const int i0s = ch * sh.q;
-    const double acc = star_tile_sweep<V, R, TZ, ROWS, MODE, NORM, WRITE, false, NT>(
+    const double acc = star_tile_sweep<V, R, TZ, ROWS, MODE, NORM, WRITE, NT>(
         in + boff, out + boff, aux ? aux + boff : nullptr, g, ph, sc, w, tz * TZ, ty * TG::TY,
         i0s, min(i0s + sh.q, g.n[0]), lds);
     if (NORM) {
@@ -239,7 +331,8 @@ __global__ __launch_bounds__(NT) void k_star_tile(const V* __restrict__ in, V* _
 // occupancy gained outweighs the extra L2 reads of the taller halo: 1 row
 // measured 4.1 / 3.6 TB/s on 64^3 float64 / complex128 bands against
 // 2.7-3.3 / 3.0-3.2 TB/s for 2-4 rows.
-constexpr int tile_rows(int, int) { return 1; }
+constexpr int TILE_ROWS_WIDE = 1;   // rows per thread in 1024-thread blocks
+constexpr int tile_rows(int, int, int nt = TILE_NT) { return nt == TILE_NT_WIDE ? TILE_ROWS_WIDE : 1; }
 
 // host-side shape selection and launch (stencil_tile_*.hip)
 int tile_tz(int n2);
