@@ -149,6 +149,48 @@ __device__ __forceinline__ V ghost_fetch(const V* __restrict__ a, int j0, int j1
     return v;
 }
 
+// In-plane part of a ghost reference, packed in one int: the wrapped offset
+// within a plane in bits 4..30 (planes of fewer than 2^27 points) and the
+// wrap flags of axes 1 and 2 (wrap_index codes) in bits 0..1 and 2..3.  A
+// reference to a zero-boundary ghost has offset 0 (any valid address: the
+// value is discarded).
+typedef int PlaneRef;
+constexpr long PLANE_REF_MAX = 1L << 27;   // points per plane
+
+__device__ __forceinline__ PlaneRef plane_ref(int j1, int j2, const Geom& g)
+{
+    const int f1 = wrap_index(j1, g.n[1], g.per[1]);
+    const int f2 = wrap_index(j2, g.n[2], g.per[2]);
+    const int off = (f1 == 3 || f2 == 3) ? 0 : j1 * g.n[2] + j2;
+    return (off << 4) | f1 | (f2 << 2);
+}
+__device__ __forceinline__ int pref_off(PlaneRef r) { return r >> 4; }
+constexpr PlaneRef PREF_ZERO = 3;   // offset 0, axis-1 flag "zero"
+
+// Value of a ghost reference from its raw load: zero outside a zero-boundary
+// axis, else the phases of the wrapped sides applied in axis order 0, 1, 2
+// (the order of ghost_fetch).  Phase sides are selected, not indexed, so the
+// phases stay in scalar registers.
+template <class V>
+__device__ __forceinline__ V ghost_fix(V v, int f0, PlaneRef r, const Phases& ph)
+{
+    const int f1 = r & 3, f2 = (r >> 2) & 3;
+    if (f0 == 3 || f1 == 3 || f2 == 3) return vzero<V>();
+    if (VT<V>::cplx) {
+        if (f0) v = pside<true>(v, ph, 0, f0);
+        if (f1) v = pside<true>(v, ph, 1, f1);
+        if (f2) v = pside<true>(v, ph, 2, f2);
+    }
+    return v;
+}
+
+// Make a loaded value available at this point (an empty asm that reads it:
+// the compiler places the wait for the load before it; no instruction).
+__device__ __forceinline__ void vtouch(double v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void vtouch(float v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void vtouch(double2 v) { asm volatile("" ::"v"(v.x), "v"(v.y)); }
+__device__ __forceinline__ void vtouch(float2 v) { asm volatile("" ::"v"(v.x), "v"(v.y)); }
+
 // Tile index of block b of nb: block b runs on XCD b % 8, so each XCD gets
 // a contiguous range of tiles and neighbouring tiles, which share halo
 // rows, meet in one L2.  A bijection on [0, nb).

@@ -358,6 +358,7 @@ bool tile_geometry(const Geom& g, int R, int esize, int nin, int TZ, int q, Tile
     // period on either side of the grid: ghost indices wrap only once
     if (sh.ntz * TZ + R > 2 * n2 || sh.nty * TY + R > 2 * n1 || n0 < R || n1 < R || n2 < R)
         return false;
+    if ((long)n1 * n2 >= PLANE_REF_MAX) return false;   // packed in-plane offsets
     sh.q = q < n0 ? q : n0;
     sh.nchunk = (n0 + sh.q - 1) / sh.q;
     const long nb = (long)sh.ntz * sh.nty * nin * sh.nchunk;

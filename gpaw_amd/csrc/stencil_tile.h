@@ -64,49 +64,6 @@ struct TileGeom {
     static constexpr int LDS = 2 * LH * LW;            // two plane buffers
 };
 
-// In-plane part of a ghost reference: the wrapped offset within a plane and
-// the wrap flags of axes 1 and 2 (wrap_index codes, f1 | f2 << 2).  A
-// reference to a zero-boundary ghost points at offset 0 (any valid address:
-// the value is discarded).
-struct PlaneRef {
-    int off;
-    int code;
-};
-
-__device__ __forceinline__ PlaneRef plane_ref(int j1, int j2, const Geom& g)
-{
-    const int f1 = wrap_index(j1, g.n[1], g.per[1]);
-    const int f2 = wrap_index(j2, g.n[2], g.per[2]);
-    PlaneRef r;
-    r.code = f1 | (f2 << 2);
-    r.off = (f1 == 3 || f2 == 3) ? 0 : j1 * g.n[2] + j2;
-    return r;
-}
-
-// Value of a ghost reference from its raw load: zero outside a zero-boundary
-// axis, else the phases of the wrapped sides applied in axis order 0, 1, 2
-// (the order of ghost_fetch).  Phase sides are selected, not indexed, so the
-// phases stay in scalar registers.
-template <class V>
-__device__ __forceinline__ V ghost_fix(V v, int f0, int code, const Phases& ph)
-{
-    const int f1 = code & 3, f2 = code >> 2;
-    if (f0 == 3 || f1 == 3 || f2 == 3) return vzero<V>();
-    if (VT<V>::cplx) {
-        if (f0) v = pside<true>(v, ph, 0, f0);
-        if (f1) v = pside<true>(v, ph, 1, f1);
-        if (f2) v = pside<true>(v, ph, 2, f2);
-    }
-    return v;
-}
-
-// Make a loaded value available at this point (an empty asm that reads it:
-// the compiler places the wait for the load before it; no instruction).
-__device__ __forceinline__ void vtouch(double v) { asm volatile("" ::"v"(v)); }
-__device__ __forceinline__ void vtouch(float v) { asm volatile("" ::"v"(v)); }
-__device__ __forceinline__ void vtouch(double2 v) { asm volatile("" ::"v"(v.x), "v"(v.y)); }
-__device__ __forceinline__ void vtouch(float2 v) { asm volatile("" ::"v"(v.x), "v"(v.y)); }
-
 template <class V>
 __device__ __forceinline__ void tile_store(V* p, V v)
 {
@@ -171,7 +128,7 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
         }
         hoff[k] = (h < NHALO) ? lr * LW + lc : -1;
         // threads without a halo cell load offset 0 of the plane (discarded)
-        hr[k] = (h < NHALO) ? plane_ref(y0 + lr - R, z0 + lc - R, g) : PlaneRef{0, 0};
+        hr[k] = (h < NHALO) ? plane_ref(y0 + lr - R, z0 + lc - R, g) : 0;
     }
     // plane p: wrap flag of axis 0 (uniform) and the plane's base offset
     auto pflag = [&](int p, i64& base) -> int {
@@ -190,19 +147,19 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
         i64 base;
         const int f = pflag(i0s - R + k, base);
 #pragma unroll
-        for (int j = 0; j < ROWS; j++) q[j][k] = ghost_fix(a[base + qr[j].off], f, qr[j].code, ph);
+        for (int j = 0; j < ROWS; j++) q[j][k] = ghost_fix(a[base + pref_off(qr[j])], f, qr[j], ph);
     }
     {
         i64 base;
         qf = pflag(i0s + R, base);
 #pragma unroll
-        for (int j = 0; j < ROWS; j++) qa[j] = a[base + qr[j].off];
+        for (int j = 0; j < ROWS; j++) qa[j] = a[base + pref_off(qr[j])];
         hf = pflag(i0s, base);
 #pragma unroll
-        for (int k = 0; k < NH; k++) ha[k] = a[base + hr[k].off];
+        for (int k = 0; k < NH; k++) ha[k] = a[base + pref_off(hr[k])];
         if (MODE != ST_APPLY) {
 #pragma unroll
-            for (int j = 0; j < ROWS; j++) sn[j] = sa[(i64)i0s * plane + (inr[j] ? qr[j].off : 0)];
+            for (int j = 0; j < ROWS; j++) sn[j] = sa[(i64)i0s * plane + (inr[j] ? pref_off(qr[j]) : 0)];
         }
     }
 
@@ -229,30 +186,30 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
         V s[ROWS];
 #pragma unroll
         for (int j = 0; j < ROWS; j++) {
-            q[j][2 * R] = ghost_fix(qa[j], qf, qr[j].code, ph);
+            q[j][2 * R] = ghost_fix(qa[j], qf, qr[j], ph);
             B[(R + tyi * ROWS + j) * LW + R + tx] = q[j][R];
             if (MODE != ST_APPLY) s[j] = sn[j];
         }
 #pragma unroll
         for (int k = 0; k < NH; k++)
-            if (hoff[k] >= 0) B[hoff[k]] = ghost_fix(ha[k], hf, hr[k].code, ph);
+            if (hoff[k] >= 0) B[hoff[k]] = ghost_fix(ha[k], hf, hr[k], ph);
         if (WRITE && i0 > i0s) {
 #pragma unroll
             for (int j = 0; j < ROWS; j++)
-                if (inr[j]) tile_store(o + (i64)(i0 - 1) * plane + qr[j].off, rp[j]);
+                if (inr[j]) tile_store(o + (i64)(i0 - 1) * plane + pref_off(qr[j]), rp[j]);
         }
         if (i0 + 1 < i0e) {
             i64 base;
             qf = pflag(i0 + 1 + R, base);
 #pragma unroll
-            for (int j = 0; j < ROWS; j++) qa[j] = a[base + qr[j].off];
+            for (int j = 0; j < ROWS; j++) qa[j] = a[base + pref_off(qr[j])];
             hf = pflag(i0 + 1, base);
 #pragma unroll
-            for (int k = 0; k < NH; k++) ha[k] = a[base + hr[k].off];
+            for (int k = 0; k < NH; k++) ha[k] = a[base + pref_off(hr[k])];
             if (MODE != ST_APPLY) {
 #pragma unroll
                 for (int j = 0; j < ROWS; j++)
-                    sn[j] = sa[(i64)(i0 + 1) * plane + (inr[j] ? qr[j].off : 0)];
+                    sn[j] = sa[(i64)(i0 + 1) * plane + (inr[j] ? pref_off(qr[j]) : 0)];
             }
         }
         __syncthreads();
@@ -293,7 +250,7 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
     if (WRITE && i0e > i0s) {
 #pragma unroll
         for (int j = 0; j < ROWS; j++)
-            if (inr[j]) tile_store(o + (i64)(i0e - 1) * plane + qr[j].off, rp[j]);
+            if (inr[j]) tile_store(o + (i64)(i0e - 1) * plane + pref_off(qr[j]), rp[j]);
     }
     return acc;
 }

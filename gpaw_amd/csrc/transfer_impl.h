@@ -145,7 +145,8 @@ hipError_t tr_launch(const Transfer& tr, int mode, const void* in, void* out, vo
         return launch_error();
     }
     // wide transfers on large planes: LDS-tiled plane sweep (8 x 32 tiles)
-    if (K >= 4 && t.nout[1] >= 4 && t.nout[2] >= 16 && !getenv("GPAW_AMD_POINTWISE_TRANSFER")) {
+    if (K >= 4 && t.nout[1] >= 4 && t.nout[2] >= 16 &&
+        (long)t.gin.n[1] * t.gin.n[2] < PLANE_REF_MAX && !getenv("GPAW_AMD_POINTWISE_TRANSFER")) {
         TileTr tt;
         tt.nt1 = (t.nout[1] + 7) / 8;
         tt.nt2 = (t.nout[2] + 31) / 32;

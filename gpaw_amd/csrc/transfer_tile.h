@@ -56,29 +56,59 @@ __global__ __launch_bounds__(256) void k_restrict_tile(const V* __restrict__ in,
     const int hi2 = 2 * t.nout[2] + 2 * K - 3 - t.lo[2];
     const int p0s = -t.lo[0];
     const int np = 2 * t.nout[0] + 2 * K - 3;   // input planes swept
-    // slab cells of this thread: c = tid + 256 k; the next plane's cells are
-    // loaded into registers while the current plane is combined
+    // slab cells of this thread: c = tid + 256 k, with their in-plane
+    // references (code 3: a cell no valid output uses, zero without a
+    // load).  The next plane's cells are loaded raw into registers while the
+    // current plane is combined and fixed (zero / phase) when stored to LDS.
     constexpr int NS = (SR * SC + 255) / 256;
-    auto slab = [&](int p0, int k) -> V {
+    auto cref = [&](int k) -> PlaneRef {   // recomputed: registers are scarce here
         const int c = tid + 256 * k;
-        const int r = c / SC, s = c % SC;
-        const int j1 = r0 + r, j2 = s0 + s;
-        return (c >= SR * SC || j1 < lo1 || j1 >= hi1 || j2 < lo2 || j2 >= hi2 || p0 < lo0 ||
-                p0 >= hi0)
-                   ? vzero<V>()
-                   : ghost_fetch(a, p0, j1, j2, t.gin, ph);
+        const int j1 = r0 + c / SC, j2 = s0 + c % SC;
+        const bool ok = c < SR * SC && j1 >= lo1 && j1 < hi1 && j2 >= lo2 && j2 < hi2;
+        return ok ? plane_ref(j1, j2, t.gin) : PREF_ZERO;
+    };
+    const i64 plane = (i64)t.gin.n[1] * t.gin.n[2];
+    auto pflag = [&](int p0, i64& base) -> int {
+        int f = 3;
+        if (p0 >= lo0 && p0 < hi0) f = wrap_index(p0, t.gin.n[0], t.gin.per[0]);
+        base = f == 3 ? 0 : (i64)p0 * plane;
+        return f;
     };
     V pre[NS];
+    int pf;
+    {
+        i64 base;
+        pf = pflag(p0s, base);
 #pragma unroll
-    for (int k = 0; k < NS; k++) pre[k] = slab(p0s, k);
+        for (int k = 0; k < NS; k++) pre[k] = a[base + pref_off(cref(k))];
+    }
+    const bool own = o1 < t.nout[1] && o2 < t.nout[2];
     V ring[W];
+    // x-pass at the start of plane step j, over the ring as step j - 1 left
+    // it: output plane o0 is complete when its last tap plane arrived.  Its
+    // store is issued with the next plane's loads.
+    auto xpass = [&](int j) {
+        if (j >= W && ((j - W) & 1) == 0) {
+            V s[K + 1];
+#pragma unroll
+            for (int m = 0; m <= K; m++) s[m] = ring[K - 1 + rst_tap<K>(m)];
+            const V r = rst_combine<V, K>(s);
+            if (own) o[((i64)((j - W) >> 1) * t.nout[1] + o1) * t.nout[2] + o2] = r;
+        }
+    };
     for (int j = 0; j < np; j++) {
+        // one wait per plane step, for loads issued a step earlier
+#pragma unroll
+        for (int k = 0; k < NS; k++) vtouch(pre[k]);
 #pragma unroll
         for (int k = 0; k < NS; k++)
-            if (tid + 256 * k < SR * SC) A[tid + 256 * k] = pre[k];
+            if (tid + 256 * k < SR * SC) A[tid + 256 * k] = ghost_fix(pre[k], pf, cref(k), ph);
+        xpass(j);
         if (j + 1 < np) {
+            i64 base;
+            pf = pflag(p0s + j + 1, base);
 #pragma unroll
-            for (int k = 0; k < NS; k++) pre[k] = slab(p0s + j + 1, k);
+            for (int k = 0; k < NS; k++) pre[k] = a[base + pref_off(cref(k))];
         }
         __syncthreads();
         for (int c = tid; c < SR * T2; c += 256) {
@@ -99,17 +129,8 @@ __global__ __launch_bounds__(256) void k_restrict_tile(const V* __restrict__ in,
 #pragma unroll
         for (int m = 0; m < W - 1; m++) ring[m] = ring[m + 1];
         ring[W - 1] = y;
-        // output plane o0 is complete when its last tap plane arrived
-        if (j >= W - 1 && ((j - (W - 1)) & 1) == 0) {
-            const int o0 = (j - (W - 1)) >> 1;
-            V s[K + 1];
-#pragma unroll
-            for (int m = 0; m <= K; m++) s[m] = ring[K - 1 + rst_tap<K>(m)];
-            const V r = rst_combine<V, K>(s);
-            if (o1 < t.nout[1] && o2 < t.nout[2])
-                o[((i64)o0 * t.nout[1] + o1) * t.nout[2] + o2] = r;
-        }
     }
+    xpass(np);
 }
 
 // ----------------------------------------------------------- interpolation
@@ -145,13 +166,86 @@ __global__ __launch_bounds__(256) void k_interp_tile(const V* __restrict__ in, V
     const int c0f = centre(0, 0);                         // first output's centre
     const int p0s = c0f - K / 2 + 1;                      // first plane swept
     const int p0e = centre(t.nout[0] - 1, 0) + K / 2;     // last plane swept
+    // coarse slab cells of this thread (c = tid + 256 k) with their in-plane
+    // references; loaded raw one plane ahead, fixed when stored to LDS
+    constexpr int NSI = (SR * SC + 255) / 256;
+    auto cref = [&](int k) -> PlaneRef {
+        const int c = tid + 256 * k;
+        const int j1 = r0 + c / SC, j2 = s0 + c % SC;
+        const bool ok = c < SR * SC && j1 >= lo1 && j1 < hi1 && j2 >= lo2 && j2 < hi2;
+        return ok ? plane_ref(j1, j2, t.gin) : PREF_ZERO;
+    };
+    const i64 plane = (i64)t.gin.n[1] * t.gin.n[2];
+    auto pflag = [&](int p0, i64& base) -> int {
+        const int f = wrap_index(p0, t.gin.n[0], t.gin.per[0]);
+        base = f == 3 ? 0 : (i64)p0 * plane;
+        return f;
+    };
+    const bool own = o1 < t.nout[1] && o2 < t.nout[2];
+    // index of this thread's fine output of parity par among the outputs
+    // with coarse centre c0 (-1: none)
+    auto outq = [&](int c0, int par) -> i64 {
+        const int f0 = 2 * (c0 - (K / 2 - 1) + t.lo[0]) + par - t.skip[0][0];
+        return (own && f0 >= 0 && f0 < t.nout[0]) ? ((i64)f0 * t.nout[1] + o1) * t.nout[2] + o2
+                                                   : -1;
+    };
+    V pre[NSI];
+    int pf;
+    {
+        i64 base;
+        pf = pflag(p0s, base);
+#pragma unroll
+        for (int k = 0; k < NSI; k++) pre[k] = a[base + pref_off(cref(k))];
+    }
     V ring[K];
+    V old[2] = {vzero<V>(), vzero<V>()};   // subtract: outputs' values, loaded a step ahead
+    // outputs completed by step p0 - 1 (fine planes whose coarse centre is
+    // p0 - 1 - K/2, even and odd), written at the start of step p0 so that
+    // their stores are issued together with the next plane's loads
+    auto opass = [&](int p0) {
+        if (p0 - p0s >= K) {
+            const int c0 = p0 - 1 - K / 2;
+#pragma unroll
+            for (int par = 0; par < 2; par++) {
+                const i64 q = outq(c0, par);
+                if (q < 0) continue;
+                V r;
+                if (!par) {
+                    r = ring[K / 2 - 1];
+                } else {
+                    V s[K];
+#pragma unroll
+                    for (int m = 0; m < K; m++) s[m] = ring[K / 2 - 1 + ip_tap(m)];
+                    r = ip_combine<V, K>(s);
+                }
+                o[q] = MODE == TR_SUBTRACT ? vsub(old[par], r) : r;
+            }
+        }
+    };
     for (int p0 = p0s; p0 <= p0e; p0++) {
-        for (int c = tid; c < SR * SC; c += 256) {
-            const int r = c / SC, s = c % SC;
-            const int j1 = r0 + r, j2 = s0 + s;
-            A[c] = (j1 < lo1 || j1 >= hi1 || j2 < lo2 || j2 >= hi2) ? vzero<V>()
-                                                                  : ghost_fetch(a, p0, j1, j2, t.gin, ph);
+        // one wait per plane step, for loads issued a step earlier
+#pragma unroll
+        for (int k = 0; k < NSI; k++) vtouch(pre[k]);
+        if (MODE == TR_SUBTRACT) {
+            vtouch(old[0]);
+            vtouch(old[1]);
+        }
+#pragma unroll
+        for (int k = 0; k < NSI; k++)
+            if (tid + 256 * k < SR * SC) A[tid + 256 * k] = ghost_fix(pre[k], pf, cref(k), ph);
+        opass(p0);
+        if (p0 < p0e) {
+            i64 base;
+            pf = pflag(p0 + 1, base);
+#pragma unroll
+            for (int k = 0; k < NSI; k++) pre[k] = a[base + pref_off(cref(k))];
+        }
+        if (MODE == TR_SUBTRACT && p0 + 1 - p0s >= K) {
+#pragma unroll
+            for (int par = 0; par < 2; par++) {
+                const i64 q = outq(p0 - K / 2, par);
+                old[par] = o[q < 0 ? 0 : q];
+            }
         }
         __syncthreads();
         for (int c = tid; c < SR * T2; c += 256) {
@@ -185,31 +279,8 @@ __global__ __launch_bounds__(256) void k_interp_tile(const V* __restrict__ in, V
 #pragma unroll
         for (int m = 0; m < K - 1; m++) ring[m] = ring[m + 1];
         ring[K - 1] = y;
-        // fine outputs whose coarse centre is p0 - K/2 (even and odd)
-        const int c0 = p0 - K / 2;
-        if (p0 - p0s >= K - 1 && o1 < t.nout[1] && o2 < t.nout[2]) {
-            const int i0 = c0 - (K / 2 - 1) + t.lo[0];
-#pragma unroll
-            for (int par = 0; par < 2; par++) {
-                const int f0 = 2 * i0 + par - t.skip[0][0];
-                if (f0 < 0 || f0 >= t.nout[0]) continue;
-                V r;
-                if (!par) {
-                    r = ring[K / 2 - 1];
-                } else {
-                    V s[K];
-#pragma unroll
-                    for (int m = 0; m < K; m++) s[m] = ring[K / 2 - 1 + ip_tap(m)];
-                    r = ip_combine<V, K>(s);
-                }
-                const i64 q = ((i64)f0 * t.nout[1] + o1) * t.nout[2] + o2;
-                if (MODE == TR_SUBTRACT)
-                    o[q] = vsub(o[q], r);
-                else
-                    o[q] = r;
-            }
-        }
     }
+    opass(p0e + 1);
 }
 
 }  // namespace gpaw
