@@ -177,9 +177,17 @@ __device__ __forceinline__ V ghost_fix(V v, int f0, PlaneRef r, const Phases& ph
     const int f1 = r & 3, f2 = (r >> 2) & 3;
     if (f0 == 3 || f1 == 3 || f2 == 3) return vzero<V>();
     if (VT<V>::cplx) {
-        if (f0) v = pside<true>(v, ph, 0, f0);
-        if (f1) v = pside<true>(v, ph, 1, f1);
-        if (f2) v = pside<true>(v, ph, 2, f2);
+        // real branches, not selects (the empty asm keeps the compiler from
+        // computing both sides): phases apply on few planes and lanes
+        if (f0) {
+            asm volatile("");
+            v = pside<true>(v, ph, 0, f0);
+        }
+        if (r & 15) {
+            asm volatile("");
+            if (f1) v = pside<true>(v, ph, 1, f1);
+            if (f2) v = pside<true>(v, ph, 2, f2);
+        }
     }
     return v;
 }

@@ -32,6 +32,8 @@ namespace gpaw {
 constexpr int TILE_NT = 256;
 constexpr int TILE_NT_WIDE = 1024;
 constexpr bool TILE_STREAM_STORE = false;
+// planes by which halo loads lead their use beyond the one-plane prefetch
+constexpr int tile_halo_lead(int R, int, int nt) { return nt == TILE_NT_WIDE ? R : 0; }
 
 struct TileCoefs {
     double c[6 * GPAW_MAXR + 1];
@@ -137,11 +139,16 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
         return f;
     };
 
-    // planes i0-R .. i0+R of the owned rows; the queue plane and the halo of
-    // plane i0 + 1 are loaded raw during plane i0 (flags qf / hf)
+    // planes i0-R .. i0+R of the owned rows; the queue plane i0 + 1 + R is
+    // loaded raw during plane i0 (flag qf).  The halo cells of a plane are
+    // loaded together with that plane's queue rows (HA = R planes before
+    // they are needed; slot p of hq holds plane i0 + p): the rows a
+    // neighbouring tile loads for its queue are then still in the L2 when
+    // they are re-read as halo.
+    constexpr int HA = tile_halo_lead(R, (int)sizeof(V), NT);
     V q[ROWS][2 * R + 1];
-    V qa[ROWS], ha[NH], sn[ROWS];
-    int qf, hf;
+    V qa[ROWS], hq[HA + 1][NH], sn[ROWS];
+    int qf, hf[HA + 1];
 #pragma unroll
     for (int k = 0; k < 2 * R; k++) {
         i64 base;
@@ -154,9 +161,12 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
         qf = pflag(i0s + R, base);
 #pragma unroll
         for (int j = 0; j < ROWS; j++) qa[j] = a[base + pref_off(qr[j])];
-        hf = pflag(i0s, base);
 #pragma unroll
-        for (int k = 0; k < NH; k++) ha[k] = a[base + pref_off(hr[k])];
+        for (int p = 0; p <= HA; p++) {
+            hf[p] = pflag(i0s + p, base);
+#pragma unroll
+            for (int k = 0; k < NH; k++) hq[p][k] = a[base + pref_off(hr[k])];
+        }
         if (MODE != ST_APPLY) {
 #pragma unroll
             for (int j = 0; j < ROWS; j++) sn[j] = sa[(i64)i0s * plane + (inr[j] ? pref_off(qr[j]) : 0)];
@@ -182,7 +192,9 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
             if (MODE != ST_APPLY) vtouch(sn[j]);
         }
 #pragma unroll
-        for (int k = 0; k < NH; k++) vtouch(ha[k]);
+        for (int p = 0; p <= HA; p++)
+#pragma unroll
+            for (int k = 0; k < NH; k++) vtouch(hq[p][k]);
         V s[ROWS];
 #pragma unroll
         for (int j = 0; j < ROWS; j++) {
@@ -192,7 +204,13 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
         }
 #pragma unroll
         for (int k = 0; k < NH; k++)
-            if (hoff[k] >= 0) B[hoff[k]] = ghost_fix(ha[k], hf, hr[k], ph);
+            if (hoff[k] >= 0) B[hoff[k]] = ghost_fix(hq[0][k], hf[0], hr[k], ph);
+#pragma unroll
+        for (int p = 0; p < HA; p++) {
+            hf[p] = hf[p + 1];
+#pragma unroll
+            for (int k = 0; k < NH; k++) hq[p][k] = hq[p + 1][k];
+        }
         if (WRITE && i0 > i0s) {
 #pragma unroll
             for (int j = 0; j < ROWS; j++)
@@ -203,9 +221,15 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
             qf = pflag(i0 + 1 + R, base);
 #pragma unroll
             for (int j = 0; j < ROWS; j++) qa[j] = a[base + pref_off(qr[j])];
-            hf = pflag(i0 + 1, base);
+            if (i0 + 1 + HA < i0e) {
+                if (HA) {   // same plane as the queue rows
+                    hf[HA] = qf;
+                } else {
+                    hf[HA] = pflag(i0 + 1, base);
+                }
 #pragma unroll
-            for (int k = 0; k < NH; k++) ha[k] = a[base + pref_off(hr[k])];
+                for (int k = 0; k < NH; k++) hq[HA][k] = a[base + pref_off(hr[k])];
+            }
             if (MODE != ST_APPLY) {
 #pragma unroll
                 for (int j = 0; j < ROWS; j++)
