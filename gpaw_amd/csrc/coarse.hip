@@ -80,17 +80,42 @@ __device__ __forceinline__ double nb_sum(const double* __restrict__ a, const int
     return x;
 }
 
-__device__ void jacobi_sweeps(const CoarseLevel& L, const Pts& p, double* const* ptr, int k,
-                              int& cur)
+// The four arrays of one level.  In LDS mode they are derived from the
+// kernel's shared array, so the compiler addresses them as LDS (ds_read /
+// ds_write); from a pointer table in memory it could only use flat
+// accesses, which take the vector-memory path even for LDS.
+struct LevelArrays {
+    double* a[4];
+};
+
+template <bool LDS>
+__device__ __forceinline__ LevelArrays level_arrays(const CoarseLevel& L, double* smem)
 {
-    const double* __restrict__ rho = ptr[RHO];
+    LevelArrays r;
+    if (LDS) {
+        const int n = L.g.n[0] * L.g.n[1] * L.g.n[2];
+#pragma unroll
+        for (int k = 0; k < 4; k++) r.a[k] = smem + L.off + k * n;
+    } else {
+        r.a[PA] = L.phiA;
+        r.a[PB] = L.phiB;
+        r.a[RHO] = L.rho;
+        r.a[RES] = L.res;
+    }
+    return r;
+}
+
+__device__ __forceinline__ void jacobi_sweeps(const CoarseLevel& L, const Pts& p,
+                                              const LevelArrays& A, int k, int& cur)
+{
+    const double* __restrict__ rho = A.a[RHO];
     double c[7];
 #pragma unroll
     for (int m = 0; m < 7; m++) c[m] = L.c[m];
     const double w = L.w;
     for (int it = 0; it < k; it++) {
-        const double* __restrict__ src = ptr[cur];
-        double* __restrict__ dst = ptr[cur ^ 1];
+        const double* __restrict__ src = cur == PA ? A.a[PA] : A.a[PB];
+        double* __restrict__ dst = cur == PA ? A.a[PB] : A.a[PA];
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const int idx = p.idx[q];
@@ -108,11 +133,11 @@ __device__ void jacobi_sweeps(const CoarseLevel& L, const Pts& p, double* const*
 // TB: storage type of the first level's right-hand side and solution in HBM
 // (double, or float when the fused kernel is the FP64 coarse solve of an
 // FP32 V-cycle; then the arrays must live in LDS and are converted there).
-template <class TB>
+// LDS: the level arrays live in the kernel's shared memory (plan->lds).
+template <class TB, bool LDS>
 __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __restrict__ P)
 {
     extern __shared__ double smem[];
-    __shared__ double* ptr[GPAW_MAX_LEVELS][4];
     Phases ph;
     for (int a = 0; a < 3; a++)
         for (int d = 0; d < 2; d++) {
@@ -120,41 +145,28 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
             ph.im[a][d] = 0.0;
         }
     const int nl = P->nlev;
-    const bool lds = P->lds;
-    if (threadIdx.x < nl) {   // level arrays: global, or four LDS blocks per level
-        const CoarseLevel& L = P->lev[threadIdx.x];
-        if (lds) {
-            const int n = L.g.n[0] * L.g.n[1] * L.g.n[2];
-            double* base = smem + L.off;
-            for (int k = 0; k < 4; k++) ptr[threadIdx.x][k] = base + k * n;
-        } else {
-            ptr[threadIdx.x][PA] = L.phiA;
-            ptr[threadIdx.x][PB] = L.phiB;
-            ptr[threadIdx.x][RHO] = L.rho;
-            ptr[threadIdx.x][RES] = L.res;
-        }
-    }
-    __syncthreads();
-    if (lds) {   // right-hand side of the first coarse level, phi = 0
+    if (LDS) {   // right-hand side of the first coarse level, phi = 0
         const CoarseLevel& G = P->lev[0];
+        const LevelArrays A = level_arrays<LDS>(G, smem);
         const int n = G.g.n[0] * G.g.n[1] * G.g.n[2];
         const TB* __restrict__ grho = (const TB*)G.rho;
         for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
-            ptr[0][RHO][idx] = (double)grho[idx];
-            ptr[0][PA][idx] = 0.0;
+            A.a[RHO][idx] = (double)grho[idx];
+            A.a[PA][idx] = 0.0;
         }
         __syncthreads();
     }
     // down: pre-smooth, residual, restrict (phi of the next level zeroed)
     for (int l = 0; l < nl - 1; l++) {
         const CoarseLevel& L = P->lev[l];
+        const LevelArrays A = level_arrays<LDS>(L, smem);
         Pts pts;
         level_points(L, pts);
         int cur = PA;
-        jacobi_sweeps(L, pts, ptr[l], L.pre, cur);
-        const double* __restrict__ phi = ptr[l][cur];
-        const double* __restrict__ rho = ptr[l][RHO];
-        double* __restrict__ res = ptr[l][RES];
+        jacobi_sweeps(L, pts, A, L.pre, cur);
+        const double* __restrict__ phi = cur == PA ? A.a[PA] : A.a[PB];
+        const double* __restrict__ rho = A.a[RHO];
+        double* __restrict__ res = A.a[RES];
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const int idx = pts.idx[q];
@@ -166,32 +178,34 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
         }
         __syncthreads();
         const CoarseLevel& N = P->lev[l + 1];
+        const LevelArrays NA = level_arrays<LDS>(N, smem);
         const int m = N.g.n[0] * N.g.n[1] * N.g.n[2];
-        double* __restrict__ nrho = ptr[l + 1][RHO];
-        double* __restrict__ nphi = ptr[l + 1][PA];
         for (int idx = threadIdx.x; idx < m; idx += blockDim.x) {
             int o0, o1, o2;
             split(idx, N.g, o0, o1, o2);
-            nrho[idx] = restrict_point<double, 2>(res, o0, o1, o2, L.rst, ph);
-            nphi[idx] = 0.0;
+            NA.a[RHO][idx] = restrict_point<double, 2>(res, o0, o1, o2, L.rst, ph);
+            NA.a[PA][idx] = 0.0;
         }
         __syncthreads();
     }
     {   // bottom: post-smooth only
         const CoarseLevel& L = P->lev[nl - 1];
+        const LevelArrays A = level_arrays<LDS>(L, smem);
         Pts pts;
         level_points(L, pts);
         int cur = PA;
-        jacobi_sweeps(L, pts, ptr[nl - 1], L.post, cur);
+        jacobi_sweeps(L, pts, A, L.post, cur);
     }
     // up: correct from the coarser level, post-smooth
     for (int l = nl - 2; l >= 0; l--) {
         const CoarseLevel& L = P->lev[l];
         const CoarseLevel& N = P->lev[l + 1];
+        const LevelArrays A = level_arrays<LDS>(L, smem);
+        const LevelArrays NA = level_arrays<LDS>(N, smem);
         int cur = (L.pre & 1) ? PB : PA;
         const int kc = (l + 1 < nl - 1 ? N.pre : 0) + N.post;
-        const double* __restrict__ coarse = ptr[l + 1][(kc & 1) ? PB : PA];
-        double* __restrict__ phi = ptr[l][cur];
+        const double* __restrict__ coarse = (kc & 1) ? NA.a[PB] : NA.a[PA];
+        double* __restrict__ phi = cur == PA ? A.a[PA] : A.a[PB];
         const int n = L.g.n[0] * L.g.n[1] * L.g.n[2];
         for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
             int i0, i1, i2;
@@ -201,12 +215,13 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
         __syncthreads();
         Pts pts;
         level_points(L, pts);
-        jacobi_sweeps(L, pts, ptr[l], L.post, cur);
+        jacobi_sweeps(L, pts, A, L.post, cur);
     }
-    if (lds) {   // hand the first level's solution back where the caller expects it
+    if (LDS) {   // hand the first level's solution back where the caller expects it
         const CoarseLevel& G = P->lev[0];
+        const LevelArrays A = level_arrays<LDS>(G, smem);
         const int k = (nl > 1 ? G.pre : 0) + G.post;
-        const double* src = ptr[0][(k & 1) ? PB : PA];
+        const double* src = (k & 1) ? A.a[PB] : A.a[PA];
         TB* dst = (TB*)((k & 1) ? G.phiB : G.phiA);
         const int n = G.g.n[0] * G.g.n[1] * G.g.n[2];
         for (int idx = threadIdx.x; idx < n; idx += blockDim.x) dst[idx] = (TB)src[idx];
@@ -215,23 +230,28 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
 
 hipError_t coarse_set_lds_limit(int bytes)
 {
-    hipError_t e = hipFuncSetAttribute((const void*)k_coarse_vcycle<double>,
+    hipError_t e = hipFuncSetAttribute((const void*)k_coarse_vcycle<double, true>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)k_coarse_vcycle<float>,
+        e = hipFuncSetAttribute((const void*)k_coarse_vcycle<float, true>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     return e;
 }
 
+// lds_bytes > 0: the plan keeps its level arrays in LDS (plan->lds)
 hipError_t launch_coarse_vcycle(const CoarsePlan* d_plan, int lds_bytes, hipStream_t st)
 {
-    launch_kernel(k_coarse_vcycle<double>, dim3(1), dim3(1024), lds_bytes, st, d_plan);
+    if (lds_bytes > 0)
+        launch_kernel(k_coarse_vcycle<double, true>, dim3(1), dim3(1024), lds_bytes, st, d_plan);
+    else
+        launch_kernel(k_coarse_vcycle<double, false>, dim3(1), dim3(1024), 0, st, d_plan);
     return launch_error();
 }
 
 hipError_t launch_coarse_vcycle_f32(const CoarsePlan* d_plan, int lds_bytes, hipStream_t st)
 {
-    launch_kernel(k_coarse_vcycle<float>, dim3(1), dim3(1024), lds_bytes, st, d_plan);
+    if (lds_bytes <= 0) return hipErrorInvalidValue;   // float levels need LDS mode
+    launch_kernel(k_coarse_vcycle<float, true>, dim3(1), dim3(1024), lds_bytes, st, d_plan);
     return launch_error();
 }
 

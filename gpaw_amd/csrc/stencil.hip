@@ -369,13 +369,13 @@ bool tile_geometry(const Geom& g, int R, int esize, int nin, int TZ, int q, Tile
 
 int tile_default_q(const Geom& g, int R, int esize, int nin, int TZ, int nt)
 {
-    // enough tiles to fill the chip twice at 8 blocks per CU, runs of >= 16
+    // enough tiles to fill the chip twice at 8 blocks per CU, runs of >= 8
     // planes (each run re-reads 2R planes of queue)
     const int n0 = g.n[0];
     const int TY = (nt / TZ) * tile_rows(R, esize, nt);
     const long ntile = (long)((g.n[2] + TZ - 1) / TZ) * ((g.n[1] + TY - 1) / TY) * nin;
     long nch = (4096 * TILE_NT / nt + ntile - 1) / ntile;
-    const long maxch = n0 / 16 > 1 ? n0 / 16 : 1;
+    const long maxch = n0 / 8 > 1 ? n0 / 8 : 1;
     if (nch > maxch) nch = maxch;
     if (nch < 1) nch = 1;
     return (int)((n0 + nch - 1) / nch);
