@@ -1,0 +1,146 @@
+// Weighted-Jacobi sweeps fused by temporal blocking, for the small levels
+// of the multigrid hierarchy (gfx950).
+//
+// On a 47^3 or 23^3 level one sweep is a few microseconds of work and about
+// as long again of launch and drain, so the coarse levels' eight sweeps per
+// V-cycle are dominated by launch boundaries.  Here one launch runs S
+// sweeps: a workgroup owns an 8^3 box of the output, loads the box with an
+// S-cell halo (16^3 cells for S = 4) into LDS, and sweeps S times between
+// two LDS buffers; sweep s updates the cells that sweep s + 1 will read (the
+// box plus S - s halo cells), so after S sweeps the box holds exactly the
+// values S separate sweeps give.  Halo cells are recomputed by every box
+// that needs them, which costs arithmetic on these small grids, not
+// bandwidth.
+//
+// Each cell update is the single-sweep kernel's expression (7-point star,
+// c/bmgs/relax.c:47-71 via st_epilogue<ST_JACOBI>: x summed over the six
+// neighbours in the order z+, z-, y+, y-, x+, x-, then
+// (1-w) f + w (s - x) / c_0), so results are bit-identical.  Cells outside a
+// zero-boundary axis stay zero; periodic axes wrap (any number of periods:
+// the halo may exceed a small level's extent).  Real data only (the
+// Poisson hierarchy); phases are 1.
+#include "kernels.h"
+
+namespace gpaw {
+
+namespace {
+
+constexpr int JF_B = 8;   // output box edge
+
+__device__ __forceinline__ int jf_wrap(int j, int n, int per, bool& out)
+{
+    if (j >= 0 && j < n) return j;
+    if (!per) {
+        out = true;
+        return 0;
+    }
+    j %= n;
+    return j < 0 ? j + n : j;
+}
+
+// 1024 threads: one (y, z) column of the box per 256 threads, the box's E
+// planes split into four runs of PX planes (four waves per SIMD hide the
+// LDS and division latencies of the sweeps).
+constexpr int JF_NT = 1024;
+
+template <class V, int S>
+__global__ __launch_bounds__(JF_NT) void k_jacobi_fused(const V* __restrict__ in, V* __restrict__ out,
+                                                        const V* __restrict__ rho, Geom g,
+                                                        StarCoefs7 c, double w, int nb1, int nb2)
+{
+    constexpr int E = JF_B + 2 * S;   // box edge with halo
+    static_assert(E * E == 256, "one thread per (y, z) column of the box and plane run");
+    constexpr int PX = E * E * E / JF_NT;   // planes per thread
+    __shared__ V P[2][E * E * E];
+    const int b = blockIdx.x;
+    const int bz = b % nb2, by = (b / nb2) % nb1, bx = b / (nb2 * nb1);
+    const int o0 = bx * JF_B - S, o1 = by * JF_B - S, o2 = bz * JF_B - S;
+    const int tz = threadIdx.x % E, ty = (threadIdx.x / E) % E;
+    const int x0 = (threadIdx.x / (E * E)) * PX;   // first plane of this thread
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    bool out12 = false;
+    const int j1 = jf_wrap(o1 + ty, n1, g.per[1], out12);
+    const int j2 = jf_wrap(o2 + tz, n2, g.per[2], out12);
+    const int col = j1 * n2 + j2;
+    const long plane = (long)n1 * n2;
+    typedef typename VT<V>::R RT;
+    const RT wr = (RT)w;
+    // load the box (zero outside a zero-boundary axis) and rho of this
+    // thread's cells
+    V r[PX];
+    unsigned outside = out12 ? ~0u : 0u;   // bit p: cell (x0 + p, ty, tz) is outside
+#pragma unroll
+    for (int p = 0; p < PX; p++) {
+        bool o = out12;
+        const int j0 = jf_wrap(o0 + x0 + p, n0, g.per[0], o);
+        if (o) outside |= 1u << p;
+        const long idx = (long)j0 * plane + col;
+        const V v = in[o ? 0 : idx];
+        r[p] = rho[o ? 0 : idx];
+        P[0][((x0 + p) * E + ty) * E + tz] = o ? V(0) : v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 1; s <= S; s++) {
+        const V* __restrict__ src = P[(s - 1) & 1];
+        V* __restrict__ dst = P[s & 1];
+        if (ty >= s && ty < E - s && tz >= s && tz < E - s) {
+#pragma unroll
+            for (int p = 0; p < PX; p++) {
+                const int x = x0 + p;
+                if (x < s || x >= E - s) continue;
+                const int cell = (x * E + ty) * E + tz;
+                V xs = V(0);
+                xs = xs + src[cell + 1] * (RT)c.c[1];
+                xs = xs + src[cell - 1] * (RT)c.c[2];
+                xs = xs + src[cell + E] * (RT)c.c[3];
+                xs = xs + src[cell - E] * (RT)c.c[4];
+                xs = xs + src[cell + E * E] * (RT)c.c[5];
+                xs = xs + src[cell - E * E] * (RT)c.c[6];
+                const V f = st_epilogue<ST_JACOBI>(xs, src[cell], r[p], wr, (RT)c.c[0]);
+                dst[cell] = ((outside >> p) & 1) ? V(0) : f;
+            }
+        }
+        __syncthreads();
+    }
+    // the box: cells S .. S + B - 1 along each axis, inside the grid
+    const int i1 = o1 + ty, i2 = o2 + tz;
+    if (ty >= S && ty < S + JF_B && tz >= S && tz < S + JF_B && i1 < n1 && i2 < n2) {
+#pragma unroll
+        for (int p = 0; p < PX; p++) {
+            const int x = x0 + p;
+            const int i0 = o0 + x;
+            if (x >= S && x < S + JF_B && i0 < n0)
+                out[(long)i0 * plane + (long)i1 * n2 + i2] = P[S & 1][(x * E + ty) * E + tz];
+        }
+    }
+}
+
+}  // namespace
+
+bool jacobi_fused_ok(const Stencil& s, const Geom& g)
+{
+    return s.star_r == 1 && g.n[0] >= 1 && g.n[1] >= 1 && g.n[2] >= 1;
+}
+
+hipError_t launch_jacobi_fused(const Stencil& s, int dtype, const void* in, void* out,
+                               const void* rho, const Geom& g, double w, hipStream_t st)
+{
+    if (!jacobi_fused_ok(s, g)) return hipErrorInvalidValue;
+    StarCoefs7 c;
+    for (int k = 0; k < 7; k++) c.c[k] = s.coefs[k];
+    const int nb0 = (g.n[0] + JF_B - 1) / JF_B, nb1 = (g.n[1] + JF_B - 1) / JF_B,
+              nb2 = (g.n[2] + JF_B - 1) / JF_B;
+    const dim3 grid((unsigned)(nb0 * nb1 * nb2));
+    if (dtype == DT_F64)
+        launch_kernel(k_jacobi_fused<double, JF_SWEEPS>, grid, dim3(JF_NT), 0, st,
+                      (const double*)in, (double*)out, (const double*)rho, g, c, w, nb1, nb2);
+    else if (dtype == DT_F32)
+        launch_kernel(k_jacobi_fused<float, JF_SWEEPS>, grid, dim3(JF_NT), 0, st,
+                      (const float*)in, (float*)out, (const float*)rho, g, c, w, nb1, nb2);
+    else
+        return hipErrorInvalidValue;
+    return launch_error();
+}
+
+}  // namespace gpaw

@@ -79,6 +79,18 @@ long autotune_report(char* buf, long len);
 // Work estimate used to size partial-sum buffers.
 int stencil_partials(const Stencil& s, const Geom& g, int nin);
 
+// S = JF_SWEEPS weighted-Jacobi sweeps of a 7-point star stencil in one
+// launch (temporal blocking, jacobi_fused.hip): real data, in != out;
+// bit-identical to S single sweeps ping-ponging between two arrays, with
+// the result in `out`.
+struct StarCoefs7 {
+    double c[7];
+};
+#define JF_SWEEPS 4
+bool jacobi_fused_ok(const Stencil& s, const Geom& g);
+hipError_t launch_jacobi_fused(const Stencil& s, int dtype, const void* in, void* out,
+                               const void* rho, const Geom& g, double w, hipStream_t st);
+
 // Launch one stencil pass.  `partial` (optional) receives one FP64 partial
 // sum of |result|^2 per thread block; `write` = false skips the store.
 hipError_t launch_stencil(const Stencil& s, int dtype, int mode, const void* in, void* out,

@@ -37,7 +37,9 @@ int alloc(double** p, long long n)
 }
 
 // relax k sweeps on level l, ping-ponging between phi[l] and alt; returns via
-// `cur` which buffer holds the result
+// `cur` which buffer holds the result.  On the levels below 0, runs of
+// JF_SWEEPS Jacobi sweeps are one fused launch (the result then sits in the
+// other buffer than after as many single sweeps; callers follow `cur`).
 hipError_t relax(gpaw_poisson* ps, int l, int k, double*& cur, double*& alt, hipStream_t st)
 {
     Phases ph;
@@ -48,6 +50,16 @@ hipError_t relax(gpaw_poisson* ps, int l, int k, double*& cur, double*& alt, hip
         return gpaw_op_gauss_seidel(op, cur, ps->rho[l], k, alt, st) == GPAW_OK
                    ? hipSuccess
                    : hipErrorLaunchFailure;
+    }
+    if (l > 0 && ps->fuse_jacobi && jacobi_fused_ok(op->st, op->g)) {
+        for (; k >= JF_SWEEPS; k -= JF_SWEEPS) {
+            hipError_t e = launch_jacobi_fused(op->st, DT_F64, cur, alt, ps->rho[l], op->g,
+                                               ps->w[l], st);
+            if (e != hipSuccess) return e;
+            double* t = cur;
+            cur = alt;
+            alt = t;
+        }
     }
     for (int it = 0; it < k; it++) {
         hipError_t e = launch_stencil(op->st, DT_F64, ST_JACOBI, cur, alt, ps->rho[l], op->g, ph,
@@ -64,7 +76,12 @@ hipError_t emit_level(gpaw_poisson* ps, int l, double*& cur0, double*& alt0, hip
 {
     Phases ph;
     gpaw_phases(nullptr, ph);
-    if (l == ps->lc) return launch_coarse_vcycle(ps->d_coarse, ps->coarse_lds, st);
+    if (l == ps->lc) {
+        // the fused coarse kernel leaves its result where single sweeps would
+        const int k = (l < ps->L ? ps->pre[l] : 0) + ps->post[l];
+        cur0 = (ps->method == 2 && k % 2) ? ps->phiB[l] : ps->phiA[l];
+        return launch_coarse_vcycle(ps->d_coarse, ps->coarse_lds, st);
+    }
     double* cur = l == 0 ? cur0 : ps->phiA[l];
     double* alt = l == 0 ? alt0 : ps->phiB[l];
     hipError_t e = hipSuccess;
@@ -79,16 +96,12 @@ hipError_t emit_level(gpaw_poisson* ps, int l, double*& cur0, double*& alt0, hip
         e = launch_transfer(ps->rst[l]->t, DT_F64, TR_STORE, ps->res[l], ps->rho[l + 1],
                             ps->phiA[l + 1], ph, 1, st);
         if (e != hipSuccess) return e;
-        double* dummy0 = nullptr;
-        double* dummy1 = nullptr;
-        e = emit_level(ps, l + 1, dummy0, dummy1, st);
+        // the next level starts from phiA[l+1]; ccur returns its result buffer
+        double* ccur = nullptr;
+        double* calt = nullptr;
+        e = emit_level(ps, l + 1, ccur, calt, st);
         if (e != hipSuccess) return e;
-        // the coarse result sits in phiA[l+1] when the coarse level does an
-        // even number of sweeps, else in phiB[l+1]
-        const int kc = (l + 1 < ps->L ? ps->pre[l + 1] : 0) + ps->post[l + 1];
-        const double* coarse =
-            (ps->method == 2 && kc % 2) ? ps->phiB[l + 1] : ps->phiA[l + 1];
-        e = launch_transfer(ps->itp[l]->t, DT_F64, TR_SUBTRACT, coarse, cur, nullptr, ph, 1, st);
+        e = launch_transfer(ps->itp[l]->t, DT_F64, TR_SUBTRACT, ccur, cur, nullptr, ph, 1, st);
         if (e != hipSuccess) return e;
     }
     e = relax(ps, l, ps->post[l], cur, alt, st);
@@ -100,9 +113,9 @@ hipError_t emit_level(gpaw_poisson* ps, int l, double*& cur0, double*& alt0, hip
         if (e != hipSuccess) return e;
         e = launch_reduce_partials(op->partial, np, ps->dv, ps->err_dst ? ps->err_dst : ps->d_err,
                                    st);
-        cur0 = cur;
-        alt0 = alt;
     }
+    cur0 = cur;
+    alt0 = alt;
     return e;
 }
 
@@ -244,6 +257,8 @@ extern "C" int gpaw_poisson_new(int levels, gpaw_operator* const* ops,
     ps->swaps = relax_method == 2 ? ((levels > 0 ? ps->pre[0] : 0) + ps->post[0]) % 2 : 0;
     const char* env = getenv("GPAW_AMD_GRAPH");
     ps->use_graph = !(env && env[0] == '0');
+    env = getenv("GPAW_AMD_FUSED_JACOBI");
+    ps->fuse_jacobi = !(env && env[0] == '0');
     int rc = GPAW_OK;
     for (int l = 0; l <= levels && !rc; l++) {
         const Geom& g = ps->ops[l]->g;

@@ -36,6 +36,7 @@ struct gpaw_poisson {
     int lc = -1;               // first level run by the fused coarse kernel (-1: none)
     gpaw::CoarsePlan* d_coarse = nullptr;
     int coarse_lds = 0;        // dynamic LDS bytes of the fused kernel
+    bool fuse_jacobi = true;   // fused Jacobi sweeps on levels > 0 (GPAW_AMD_FUSED_JACOBI=0: off)
     gpaw::CoarsePlan h_coarse; // host copy of the coarse plan
     MixedState* mixed = nullptr;
 };

@@ -201,3 +201,30 @@ def test_reference_symmetry_case(relax):
     ref = og.zeros()
     assert n == ops.solve(ref, rho.copy(), charge=0)
     assert rel_maxnorm(b, ref) < 1e-13
+
+
+@pytest.mark.parametrize('N,pbc', [(40, True), (48, False), (64, True)])
+def test_fused_jacobi_sweeps_identical(N, pbc, monkeypatch):
+    """Levels below 0 run their Jacobi sweeps four at a time in one fused
+    launch; the potential and niter must be bit-identical to single sweeps
+    (GPAW_AMD_FUSED_JACOBI=0, read when the plan is built)."""
+    print('case fused Jacobi', N, pbc, flush=True)
+    torch = _torch()
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.poisson import PoissonSolver
+    h = 0.2 / BOHR
+    og = O.Grid((N,) * 3, (N * h,) * 3, pbc)
+    rho = _density(og, N + 1, 4, 0.5 / BOHR, 1.5 / BOHR, 2.0 / BOHR)
+    rho -= og.integrate(rho) / og.dv / og.size_of_global_array().prod()
+    out = []
+    for fused in ('1', '0'):
+        monkeypatch.setenv('GPAW_AMD_FUSED_JACOBI', fused)
+        gd = GridDescriptor((N,) * 3, (N * h,) * 3, pbc)
+        gps = PoissonSolver(nn=3, relax='J')
+        gps.set_grid_descriptor(gd)
+        gps.initialize()
+        phi = dev(np.zeros(tuple(og.n_c)))
+        n = gps.solve_neutral(phi, dev(rho), eps=2e-10)
+        out.append((n, phi))
+    assert out[0][0] == out[1][0]
+    assert torch.equal(out[0][1], out[1][1])
