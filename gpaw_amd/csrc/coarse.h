@@ -10,6 +10,7 @@ namespace gpaw {
 struct CoarseLevel {
     Geom g;
     int off;              // LDS mode: offset (doubles) of this level's 4 arrays
+    int nb_off;           // offset of this level's neighbour table in the plan's table
     double c[7];          // 7-point star: centre, z+, z-, y+, y-, x+, x-
     double w;             // Jacobi weight
     int pre, post;        // smoothing counts
@@ -25,10 +26,17 @@ struct CoarsePlan {
     int nlev;
     int lds;              // 1: all level arrays live in LDS (lds_doubles total)
     int lds_doubles;
+    int nb_words;         // neighbour tables: 4-byte words (LDS mode: after the arrays)
+    // per level and point, the flat indices of its six neighbours in the
+    // 7-point operator's summation order (z+, z-, y+, y-, x+, x-), -1 outside
+    // a zero boundary: built on the host once per plan
+    const short* nbtab;
     CoarseLevel lev[GPAW_MAX_LEVELS];
 };
 
 TrArgs make_trargs(const Transfer& tr);
+// neighbour table of one level (6 entries per point, see CoarsePlan::nbtab)
+void coarse_neighbour_table(const Geom& g, short* tab);
 hipError_t launch_coarse_vcycle(const CoarsePlan* d_plan, int lds_bytes, hipStream_t st);
 hipError_t coarse_set_lds_limit(int bytes);
 // FP64 coarse solve inside an FP32 V-cycle: first-level rho / phi are float

@@ -35,38 +35,24 @@ enum { PA = 0, PB = 1, RHO = 2, RES = 3 };
 
 // A level has at most 2048 points (two per thread).  Each thread's points
 // and their six neighbours (z+, z-, y+, y-, x+, x-: the summation order of
-// the 7-point operator, centre first) are found once per level visit; a
-// neighbour outside a zero boundary is -1.  The sweeps then need no index
-// arithmetic.
+// the 7-point operator, centre first) come from the plan's table; a
+// neighbour outside a zero boundary is -1.
 struct Pts {
     int idx[2];       // flat index, -1 when the thread has no such point
     int nb[2][6];
 };
 
-__device__ __forceinline__ int nb_index(int j0, int j1, int j2, const Geom& g)
-{
-    const int f0 = wrap_index(j0, g.n[0], g.per[0]);
-    const int f1 = wrap_index(j1, g.n[1], g.per[1]);
-    const int f2 = wrap_index(j2, g.n[2], g.per[2]);
-    if (f0 == 3 || f1 == 3 || f2 == 3) return -1;
-    return (j0 * g.n[1] + j1) * g.n[2] + j2;
-}
-
-__device__ __forceinline__ void level_points(const CoarseLevel& L, Pts& p)
+__device__ __forceinline__ void level_points(const CoarseLevel& L, const short* __restrict__ tab,
+                                             Pts& p)
 {
     const int n = L.g.n[0] * L.g.n[1] * L.g.n[2];
 #pragma unroll
     for (int k = 0; k < 2; k++) {
         const int idx = threadIdx.x + 1024 * k;
         p.idx[k] = idx < n ? idx : -1;
-        int i0 = 0, i1 = 0, i2 = 0;
-        if (idx < n) split(idx, L.g, i0, i1, i2);
-        p.nb[k][0] = nb_index(i0, i1, i2 + 1, L.g);
-        p.nb[k][1] = nb_index(i0, i1, i2 - 1, L.g);
-        p.nb[k][2] = nb_index(i0, i1 + 1, i2, L.g);
-        p.nb[k][3] = nb_index(i0, i1 - 1, i2, L.g);
-        p.nb[k][4] = nb_index(i0 + 1, i1, i2, L.g);
-        p.nb[k][5] = nb_index(i0 - 1, i1, i2, L.g);
+        const short* t = tab + L.nb_off + 6 * (idx < n ? idx : 0);
+#pragma unroll
+        for (int m = 0; m < 6; m++) p.nb[k][m] = t[m];
     }
 }
 
@@ -75,8 +61,13 @@ __device__ __forceinline__ void level_points(const CoarseLevel& L, Pts& p)
 __device__ __forceinline__ double nb_sum(const double* __restrict__ a, const int* nb,
                                          const double* c, double x)
 {
+    // every load issued unconditionally (index 0 for a zero neighbour, whose
+    // value is then replaced by 0): no branch between the loads
+    double v[6];
 #pragma unroll
-    for (int m = 0; m < 6; m++) x = x + (nb[m] >= 0 ? a[nb[m]] : 0.0) * c[m + 1];
+    for (int m = 0; m < 6; m++) v[m] = a[nb[m] >= 0 ? nb[m] : 0];
+#pragma unroll
+    for (int m = 0; m < 6; m++) x = x + (nb[m] >= 0 ? v[m] : 0.0) * c[m + 1];
     return x;
 }
 
@@ -135,9 +126,31 @@ __device__ __forceinline__ void jacobi_sweeps(const CoarseLevel& L, const Pts& p
 // FP32 V-cycle; then the arrays must live in LDS and are converted there).
 // LDS: the level arrays live in the kernel's shared memory (plan->lds).
 template <class TB, bool LDS>
-__global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __restrict__ P)
+__global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __restrict__ Pg)
 {
     extern __shared__ double smem[];
+    // the plan is read from an LDS copy: its fields are read at every phase,
+    // and from global memory each such read would miss the scalar cache
+    // (the kernel runs once per V-cycle, between much larger kernels)
+    __shared__ CoarsePlan Ps;
+    {
+        static_assert(sizeof(CoarsePlan) % 4 == 0, "plan copied in 4-byte words");
+        const int* src = (const int*)Pg;
+        int* dst = (int*)&Ps;
+        for (int i = threadIdx.x; i < (int)(sizeof(CoarsePlan) / 4); i += blockDim.x)
+            dst[i] = src[i];
+        __syncthreads();
+    }
+    const CoarsePlan* P = &Ps;
+    // neighbour tables: copied next to the level arrays in LDS mode
+    const short* tab = P->nbtab;
+    if (LDS) {
+        int* dst = (int*)(smem + P->lds_doubles);
+        const int* src = (const int*)P->nbtab;
+        for (int i = threadIdx.x; i < P->nb_words; i += blockDim.x) dst[i] = src[i];
+        tab = (const short*)dst;
+        // made visible by the barrier after the first level's load below
+    }
     Phases ph;
     for (int a = 0; a < 3; a++)
         for (int d = 0; d < 2; d++) {
@@ -161,7 +174,7 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
         const CoarseLevel& L = P->lev[l];
         const LevelArrays A = level_arrays<LDS>(L, smem);
         Pts pts;
-        level_points(L, pts);
+        level_points(L, tab, pts);
         int cur = PA;
         jacobi_sweeps(L, pts, A, L.pre, cur);
         const double* __restrict__ phi = cur == PA ? A.a[PA] : A.a[PB];
@@ -192,7 +205,7 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
         const CoarseLevel& L = P->lev[nl - 1];
         const LevelArrays A = level_arrays<LDS>(L, smem);
         Pts pts;
-        level_points(L, pts);
+        level_points(L, tab, pts);
         int cur = PA;
         jacobi_sweeps(L, pts, A, L.post, cur);
     }
@@ -206,15 +219,16 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
         const int kc = (l + 1 < nl - 1 ? N.pre : 0) + N.post;
         const double* __restrict__ coarse = (kc & 1) ? NA.a[PB] : NA.a[PA];
         double* __restrict__ phi = cur == PA ? A.a[PA] : A.a[PB];
-        const int n = L.g.n[0] * L.g.n[1] * L.g.n[2];
-        for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
-            int i0, i1, i2;
-            split(idx, L.g, i0, i1, i2);
-            phi[idx] = phi[idx] - interp_point<double, 2>(coarse, i0, i1, i2, L.itp, ph);
+        // one coarse cell (2 x 2 x 2 fine points) per thread, as k_interp2
+        const int nc0 = interp2_cells(L.itp, 0), nc1 = interp2_cells(L.itp, 1),
+                  nc2 = interp2_cells(L.itp, 2);
+        for (int idx = threadIdx.x; idx < nc0 * nc1 * nc2; idx += blockDim.x) {
+            const int k2 = idx % nc2, k1 = (idx / nc2) % nc1, k0 = idx / (nc2 * nc1);
+            interp2_cell<double, TR_SUBTRACT>(coarse, phi, L.itp, k0, k1, k2, ph);
         }
         __syncthreads();
         Pts pts;
-        level_points(L, pts);
+        level_points(L, tab, pts);
         jacobi_sweeps(L, pts, A, L.post, cur);
     }
     if (LDS) {   // hand the first level's solution back where the caller expects it
@@ -226,6 +240,35 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
         const int n = G.g.n[0] * G.g.n[1] * G.g.n[2];
         for (int idx = threadIdx.x; idx < n; idx += blockDim.x) dst[idx] = (TB)src[idx];
     }
+}
+
+static int nb_host(int j0, int j1, int j2, const Geom& g)
+{
+    const int j[3] = {j0, j1, j2};
+    int w[3];
+    for (int a = 0; a < 3; a++) {
+        w[a] = j[a];
+        if (w[a] < 0 || w[a] >= g.n[a]) {
+            if (!g.per[a]) return -1;
+            w[a] = w[a] < 0 ? w[a] + g.n[a] : w[a] - g.n[a];
+        }
+    }
+    return (w[0] * g.n[1] + w[1]) * g.n[2] + w[2];
+}
+
+void coarse_neighbour_table(const Geom& g, short* tab)
+{
+    for (int i0 = 0; i0 < g.n[0]; i0++)
+        for (int i1 = 0; i1 < g.n[1]; i1++)
+            for (int i2 = 0; i2 < g.n[2]; i2++) {
+                short* t = tab + 6 * ((i0 * g.n[1] + i1) * g.n[2] + i2);
+                t[0] = (short)nb_host(i0, i1, i2 + 1, g);
+                t[1] = (short)nb_host(i0, i1, i2 - 1, g);
+                t[2] = (short)nb_host(i0, i1 + 1, i2, g);
+                t[3] = (short)nb_host(i0, i1 - 1, i2, g);
+                t[4] = (short)nb_host(i0 + 1, i1, i2, g);
+                t[5] = (short)nb_host(i0 - 1, i1, i2, g);
+            }
 }
 
 hipError_t coarse_set_lds_limit(int bytes)

@@ -337,16 +337,33 @@ static int setup_coarse(gpaw_poisson* ps)
             C.itp = make_trargs(ps->itp[l]->t);
         }
     }
-    // keep the coarse arrays in LDS when they fit (4 arrays per level)
+    // neighbour tables (6 shorts per point), built here once
+    std::vector<short> tab;
+    for (int l = lc; l <= ps->L; l++) {
+        P.lev[l - lc].nb_off = (int)tab.size();
+        tab.resize(tab.size() + 6 * ps->npts[l]);
+        coarse_neighbour_table(P.lev[l - lc].g, tab.data() + P.lev[l - lc].nb_off);
+    }
+    if (tab.size() % 2) tab.push_back(0);   // whole 4-byte words
+    P.nb_words = (int)(tab.size() / 2);
+    {
+        hipError_t e = hipMalloc((void**)&ps->d_coarse_nb, sizeof(short) * tab.size());
+        if (e == hipSuccess)
+            e = hipMemcpy(ps->d_coarse_nb, tab.data(), sizeof(short) * tab.size(),
+                          hipMemcpyHostToDevice);
+        if (e != hipSuccess) return gpaw_check(e, "poisson: coarse neighbour tables");
+        P.nbtab = ps->d_coarse_nb;
+    }
+    // keep the coarse arrays (4 per level) and the tables in LDS when they fit
     long long nd = 0;
     for (int l = lc; l <= ps->L; l++) {
         P.lev[l - lc].off = (int)nd;
         nd += 4 * ps->npts[l];
     }
-    if (nd * 8 <= 96 * 1024) {
+    if (nd * 8 + 4LL * P.nb_words <= 128 * 1024) {
         P.lds = 1;
         P.lds_doubles = (int)nd;
-        ps->coarse_lds = (int)(nd * 8);
+        ps->coarse_lds = (int)(nd * 8 + 4LL * P.nb_words);
         if (ps->coarse_lds > 64 * 1024) {
             hipError_t e = coarse_set_lds_limit(ps->coarse_lds);
             if (e != hipSuccess) {
@@ -496,6 +513,7 @@ extern "C" void gpaw_poisson_free(gpaw_poisson* ps)
     if (ps->d_errs) hipFree(ps->d_errs);
     if (ps->h_errs) hipHostFree(ps->h_errs);
     if (ps->d_coarse) hipFree(ps->d_coarse);
+    if (ps->d_coarse_nb) hipFree(ps->d_coarse_nb);
     if (ps->mixed) gpaw_mixed_free(ps->mixed);
     if (ps->h_err) hipHostFree(ps->h_err);
     delete ps;

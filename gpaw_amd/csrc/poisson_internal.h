@@ -35,6 +35,7 @@ struct gpaw_poisson {
     int swaps = 0;             // 1 if level 0 does an odd number of sweeps per cycle
     int lc = -1;               // first level run by the fused coarse kernel (-1: none)
     gpaw::CoarsePlan* d_coarse = nullptr;
+    short* d_coarse_nb = nullptr;     // neighbour tables of the fused coarse levels
     int coarse_lds = 0;        // dynamic LDS bytes of the fused kernel
     bool fuse_jacobi = true;   // fused Jacobi sweeps on levels > 0 (GPAW_AMD_FUSED_JACOBI=0: off)
     gpaw::CoarsePlan h_coarse; // host copy of the coarse plan

@@ -16,7 +16,7 @@
 // needs, then y, then x) reproduces the reference's three-sweep rounding
 // exactly.
 #pragma once
-#include "device_common.h"
+#include "kernels.h"
 
 namespace gpaw {
 
@@ -181,6 +181,79 @@ __device__ __forceinline__ V interp_point(const V* __restrict__ a, int o0, int o
                               interp_yz<V, K>(a, c0 + j, c1, c2, oy, oz, t, ph)),
                          (RT)W<K>::w(j)));
     return x;
+}
+
+// K = 2 interpolation of one coarse cell (k0, k1, k2): its 2 x 2 x 2 corner
+// values produce the 2 x 2 x 2 fine points p = 2i + {0, 1} (output
+// o = p - skip_lo).  Even fine coordinates copy, odd ones average
+// (0.5 * (a_i + a_{i+1}), c/bmgs/interpolate.c:47-48), pass order z, y, x
+// as in interp_point, so the values are bit-identical with 1/8 of the loads.
+// MODE: TR_STORE (o = value) or TR_SUBTRACT (o -= value).
+template <class V, int MODE>
+__device__ __forceinline__ void interp2_cell(const V* __restrict__ a, V* __restrict__ o,
+                                             const TrArgs& t, int k0, int k1, int k2,
+                                             const Phases& ph)
+{
+    typedef typename VT<V>::R RT;
+    // first coarse cell index per axis
+    const int i0 = k0 + (t.skip[0][0] >> 1), i1 = k1 + (t.skip[1][0] >> 1),
+              i2 = k2 + (t.skip[2][0] >> 1);
+    const int c0 = i0 - t.lo[0], c1 = i1 - t.lo[1], c2 = i2 - t.lo[2];
+    V v[2][2][2];
+#pragma unroll
+    for (int d0 = 0; d0 < 2; d0++)
+#pragma unroll
+        for (int d1 = 0; d1 < 2; d1++)
+#pragma unroll
+            for (int d2 = 0; d2 < 2; d2++)
+                v[d0][d1][d2] = ghost_fetch(a, c0 + d0, c1 + d1, c2 + d2, t.gin, ph);
+    const RT h = (RT)0.5;
+    // z pass: z[d0][d1][pz]
+    V z[2][2][2];
+#pragma unroll
+    for (int d0 = 0; d0 < 2; d0++)
+#pragma unroll
+        for (int d1 = 0; d1 < 2; d1++) {
+            z[d0][d1][0] = v[d0][d1][0];
+            z[d0][d1][1] = vmul(vadd(v[d0][d1][0], v[d0][d1][1]), h);
+        }
+    // y pass: y[d0][py][pz]
+    V y[2][2][2];
+#pragma unroll
+    for (int d0 = 0; d0 < 2; d0++)
+#pragma unroll
+        for (int pz = 0; pz < 2; pz++) {
+            y[d0][0][pz] = z[d0][0][pz];
+            y[d0][1][pz] = vmul(vadd(z[d0][0][pz], z[d0][1][pz]), h);
+        }
+#pragma unroll
+    for (int px = 0; px < 2; px++) {
+        const int f0 = 2 * i0 + px - t.skip[0][0];
+        if (f0 < 0 || f0 >= t.nout[0]) continue;
+#pragma unroll
+        for (int py = 0; py < 2; py++) {
+            const int f1 = 2 * i1 + py - t.skip[1][0];
+            if (f1 < 0 || f1 >= t.nout[1]) continue;
+#pragma unroll
+            for (int pz = 0; pz < 2; pz++) {
+                const int f2 = 2 * i2 + pz - t.skip[2][0];
+                if (f2 < 0 || f2 >= t.nout[2]) continue;
+                const V r = px ? vmul(vadd(y[0][py][pz], y[1][py][pz]), h) : y[0][py][pz];
+                const i64 q = ((i64)f0 * t.nout[1] + f1) * t.nout[2] + f2;
+                if (MODE == TR_SUBTRACT)
+                    o[q] = vsub(o[q], r);
+                else
+                    o[q] = r;
+            }
+        }
+    }
+}
+
+// coarse cells covering every output of a K = 2 interpolation, per axis:
+// i from skip_lo / 2 to (nout - 1 + skip_lo) / 2
+__host__ __device__ inline int interp2_cells(const TrArgs& t, int a)
+{
+    return ((t.nout[a] - 1 + t.skip[a][0]) >> 1) - (t.skip[a][0] >> 1) + 1;
 }
 
 }  // namespace gpaw
