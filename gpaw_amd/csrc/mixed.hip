@@ -174,6 +174,18 @@ hipError_t relax32(gpaw_poisson* ps, MixedState* m, int l, int k, float*& cur, f
     Phases ph;
     gpaw_phases(nullptr, ph);
     gpaw_operator* op = ps->ops[l];
+    // below level 0, runs of JF_SWEEPS sweeps are one fused launch (callers
+    // follow `cur`, not the sweep parity)
+    if (l > 0 && ps->fuse_jacobi && jacobi_fused_ok(op->st, op->g)) {
+        for (; k >= JF_SWEEPS; k -= JF_SWEEPS) {
+            hipError_t e = launch_jacobi_fused(op->st, DT_F32, cur, alt, m->rho[l], op->g,
+                                               ps->w[l], st);
+            if (e != hipSuccess) return e;
+            float* t = cur;
+            cur = alt;
+            alt = t;
+        }
+    }
     for (int it = 0; it < k; it++) {
         hipError_t e = launch_stencil(op->st, DT_F32, ST_JACOBI, cur, alt, m->rho[l], op->g, ph,
                                       1, ps->w[l], nullptr, nullptr, true, st);
@@ -185,12 +197,18 @@ hipError_t relax32(gpaw_poisson* ps, MixedState* m, int l, int k, float*& cur, f
     return hipSuccess;
 }
 
-// one FP32 V-cycle on level l (level 0: rho = d32, result in phiA[0])
-hipError_t emit32(gpaw_poisson* ps, MixedState* m, int l, bool zero0, hipStream_t st)
+// one FP32 V-cycle on level l (level 0: rho = d32, result in phiA[0]);
+// `res` returns the buffer that holds level l's result
+hipError_t emit32(gpaw_poisson* ps, MixedState* m, int l, bool zero0, float*& res,
+                  hipStream_t st)
 {
     Phases ph;
     gpaw_phases(nullptr, ph);
-    if (l == m->lc) return launch_coarse_vcycle_f32(m->d_coarse32, ps->coarse_lds, st);
+    if (l == m->lc) {   // the fused coarse kernel leaves it where single sweeps would
+        const int kc = (l < m->L ? m->pre[l] : 0) + m->post[l];
+        res = (kc % 2) ? m->phiB[l] : m->phiA[l];
+        return launch_coarse_vcycle_f32(m->d_coarse32, ps->coarse_lds, st);
+    }
     gpaw_operator* op = ps->ops[l];
     float* cur = m->phiA[l];
     float* alt = m->phiB[l];
@@ -206,14 +224,23 @@ hipError_t emit32(gpaw_poisson* ps, MixedState* m, int l, bool zero0, hipStream_
         e = launch_transfer(ps->rst[l]->t, DT_F32, TR_STORE, m->res[l], m->rho[l + 1],
                             m->phiA[l + 1], ph, 1, st);
         if (e != hipSuccess) return e;
-        e = emit32(ps, m, l + 1, false, st);
+        float* coarse = nullptr;
+        e = emit32(ps, m, l + 1, false, coarse, st);
         if (e != hipSuccess) return e;
-        const int kc = (l + 1 < m->L ? m->pre[l + 1] : 0) + m->post[l + 1];
-        const float* coarse = (kc % 2) ? m->phiB[l + 1] : m->phiA[l + 1];
         e = launch_transfer(ps->itp[l]->t, DT_F32, TR_SUBTRACT, coarse, cur, nullptr, ph, 1, st);
         if (e != hipSuccess) return e;
     }
-    return relax32(ps, m, l, m->post[l], cur, alt, st);
+    e = relax32(ps, m, l, m->post[l], cur, alt, st);
+    res = cur;
+    return e;
+}
+
+// the whole FP32 V-cycle from level 0 (its result lands in phiA[0]: level 0
+// does an even number of single sweeps)
+hipError_t emit32_top(gpaw_poisson* ps, MixedState* m, bool zero0, hipStream_t st)
+{
+    float* res = nullptr;
+    return emit32(ps, m, 0, zero0, res, st);
 }
 
 float* z_of(MixedState* m) { return m->phiA[0]; }   // even level-0 sweep count
@@ -308,7 +335,7 @@ hipError_t emit_a(gpaw_poisson* ps, MixedState* m, hipStream_t st)
 hipError_t emit_b(gpaw_poisson* ps, MixedState* m, hipStream_t st)
 {
     const i64 n = ps->npts[0];
-    hipError_t e = emit32(ps, m, 0, true, st);
+    hipError_t e = emit32_top(ps, m, true, st);
     if (e != hipSuccess) return e;
     launch_kernel(k_dot_dz_zq, gridn(n), dim3(256), 0, st, ps->res[0], z_of(m), m->q, n,
                   m->partial);
@@ -322,10 +349,10 @@ int capture_all(gpaw_poisson* ps)
     MixedState* m = ps->mixed;
     if (m->g_b) return GPAW_OK;   // the last one built: all four exist
     int rc = capture_graph(&m->g_vcycle,
-                           [&](hipStream_t st) { return emit32(ps, m, 0, true, st); });
+                           [&](hipStream_t st) { return emit32_top(ps, m, true, st); });
     if (!rc)
         rc = capture_graph(&m->g_vwarm,
-                           [&](hipStream_t st) { return emit32(ps, m, 0, false, st); });
+                           [&](hipStream_t st) { return emit32_top(ps, m, false, st); });
     if (!rc) rc = capture_graph(&m->g_a, [&](hipStream_t st) { return emit_a(ps, m, st); });
     if (!rc) rc = capture_graph(&m->g_b, [&](hipStream_t st) { return emit_b(ps, m, st); });
     if (!rc) m->g_ba.assign(MIXED_CHUNK_MAX, nullptr);
@@ -404,7 +431,7 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
         // one eager pass of the graph bodies on the plan's own buffers (all
         // reloaded below), so that the graphs record tuned launch shapes
         ps->mixed->tuned = 1;
-        hipError_t e = emit32(ps, ps->mixed, 0, true, st);
+        hipError_t e = emit32_top(ps, ps->mixed, true, st);
         if (e == hipSuccess) e = emit_a(ps, ps->mixed, st);
         rc = gpaw_check(e, "mixed: tune");
     }
