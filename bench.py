@@ -112,6 +112,22 @@ def jacobi_kernel_name():
     return 'k_star<double,3,JACOBI> level-0 sweep'
 
 
+def measured_traffic(label):
+    """HBM bytes per launch of a kernel as measured by rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE runs (scaled by a copy kernel of known size) and
+    recorded in profiles/hbm_traffic_counters.md; None when absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                        'hbm_traffic_counters.md')
+    try:
+        for line in open(path):
+            cells = [c.strip() for c in line.split('|')]
+            if len(cells) > 6 and cells[1].endswith('(%s)' % label):
+                return round((float(cells[4]) + float(cells[5])) * 1e9)
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def jacobi_roofline(torch, np, ps, rho_d):
     """Average duration of one level-0 Jacobi sweep (the dominant kernel of
     the V-cycle) over a timed run, HIP events on torch's current stream,
@@ -125,9 +141,11 @@ def jacobi_roofline(torch, np, ps, rho_d):
     return {'bound': 'hbm', 'achieved': round(algo / t / 1e9, 1),
             'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
             'frac': round(algo / t / 1e9 / PEAK_HBM_GBS, 4),
-            'traffic': None, 'kernel': jacobi_kernel_name(),
+            'traffic': measured_traffic('C2 level-0 Jacobi'), 'kernel': jacobi_kernel_name(),
             'bytes_per_launch': algo, 'avg_launch_us': round(t * 1e6, 2),
-            'note': '95^3 FP64 working set (20.6 MB) is cache resident'}
+            'note': '95^3 FP64 working set (20.6 MB) is cache resident; traffic = '
+                    'FETCH_SIZE + WRITE_SIZE per launch from profiles/hbm_traffic_counters.md '
+                    '(L2 fills from the fabric, MALL hits included)'}
 
 
 def copy_bandwidth(torch):
