@@ -452,6 +452,15 @@ def main():
                    'final_rel_residual': rel,
                    'parallelism': 'single GPU'},
     }
+    # the same solve with host (numpy) arrays: copies in and out over PCIe
+    # included; reported beside `value`, which keeps inputs resident in HBM
+    phi_h = np.zeros(tuple(gd.n_c))
+    ps.solve(phi_h, rho.copy())
+    t0 = time.perf_counter()
+    for _ in range(3):
+        phi_h[:] = 0.0
+        ps.solve(phi_h, rho.copy())
+    out['config']['host_arrays_ms_per_solve'] = round((time.perf_counter() - t0) / 3 * 1e3, 3)
     out['roofline'] = jacobi_roofline(torch, np, ps, rho_n)
     out['config']['autotuned_shapes'] = [
         l for l in ops.autotune_report().splitlines() if 'n=95x95x95' in l]

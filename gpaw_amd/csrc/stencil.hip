@@ -348,7 +348,7 @@ bool tile_geometry(const Geom& g, int R, int esize, int nin, int TZ, int q, Tile
                    int nt)
 {
     if (R < 1 || R > 4 || (TZ != 32 && TZ != 64) || q < 1 || nin < 1) return false;
-    if (nt != TILE_NT && nt != TILE_NT_WIDE) return false;
+    if (nt != TILE_NT && nt != TILE_NT_WIDE && nt != TILE_NT_MID) return false;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     const int TY = (nt / TZ) * tile_rows(R, esize, nt);
     sh.nt = nt;
@@ -529,8 +529,8 @@ TuneVal tune(const TuneKey& key, const LaunchCfg& heuristic, const Stencil& s, i
     LaunchCfg qc{K_QUEUE, 64, TileShape{}};
     cand.push_back(qc);
     const bool wide = dtype_complex(dtype) && mode == ST_APPLY && s.star_r <= 3;
-    for (int nt : {TILE_NT, TILE_NT_WIDE}) {
-        if (nt == TILE_NT_WIDE && !wide) continue;
+    for (int nt : {TILE_NT, TILE_NT_MID, TILE_NT_WIDE}) {
+        if (nt != TILE_NT && !wide) continue;
         for (int TZ : {64, 32})
             for (int q : {1, 2, 4, 8, 16, 32, 64, 1 << 20}) {
                 LaunchCfg c{K_TILE, TZ, TileShape{}};

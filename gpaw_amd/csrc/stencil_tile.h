@@ -31,9 +31,10 @@ namespace gpaw {
 // slower on float64)
 constexpr int TILE_NT = 256;
 constexpr int TILE_NT_WIDE = 1024;
+constexpr int TILE_NT_MID = 512;   // a tuner candidate for complex applies
 constexpr bool TILE_STREAM_STORE = false;
 // planes by which halo loads lead their use beyond the one-plane prefetch
-constexpr int tile_halo_lead(int R, int, int nt) { return nt == TILE_NT_WIDE ? R : 0; }
+constexpr int tile_halo_lead(int R, int, int nt) { return nt > TILE_NT ? R : 0; }
 
 struct TileCoefs {
     double c[6 * GPAW_MAXR + 1];

@@ -15,6 +15,11 @@ hipError_t tile_go(const Stencil& s, const void* in, void* out, const void* aux,
     for (int c = 0; c < 6 * R + 1; c++) sc.c[c] = s.coefs[c];
     const i64 ng = (i64)g.n[0] * g.n[1] * g.n[2];
     if constexpr (VT<V>::cplx && MODE == ST_APPLY && R <= 3) {
+        if (sh.nt == TILE_NT_MID)
+            return launch_kernel((k_star_tile<V, R, TZ, 1, MODE, NORM, WRITE, TILE_NT_MID>),
+                                 dim3(sh.nblocks), dim3(TZ, TILE_NT_MID / TZ), 0, st,
+                                 (const V*)in, (V*)out, (const V*)aux, g, ph, sc, w, sh, ng,
+                                 partial);
         if (sh.nt == TILE_NT_WIDE)
             return launch_kernel((k_star_tile<V, R, TZ, tile_rows(R, (int)sizeof(V), TILE_NT_WIDE),
                                               MODE, NORM, WRITE, TILE_NT_WIDE>),
