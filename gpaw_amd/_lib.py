@@ -61,6 +61,8 @@ PROTOTYPES = {
     'gpaw_sub_scalar': (_i, [_vp, _vp, _d, _l, _vp]),
     'gpaw_cast': (_i, [_vp, _i, _vp, _i, _l, _vp]),
     'gpaw_copy': (_i, [_vp, _vp, _l, _vp]),
+    'gpaw_add_potential': (_i, [_vp, _vp, _vp, _l, _l, _i, _vp]),
+    'gpaw_scale': (_i, [_vp, ctypes.c_double, _l, _vp]),
 }
 
 

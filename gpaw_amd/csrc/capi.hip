@@ -427,6 +427,24 @@ extern "C" int gpaw_copy(void* y, const void* x, long nbytes, void* stream)
     return gpaw_check(launch_copy_bytes(y, x, nbytes, (hipStream_t)stream), "copy");
 }
 
+extern "C" int gpaw_add_potential(void* out, const void* in, const double* v, long nin,
+                                  long ng, int dtype, void* stream)
+{
+    if (!out || !in || !v || nin < 0 || ng < 0)
+        return gpaw_fail(GPAW_EINVAL, "add_potential: bad argument");
+    if (dtype != DT_F64 && dtype != DT_C128)
+        return gpaw_fail(GPAW_EINVAL, "add_potential: float64 or complex128 bands");
+    if (out == in) return gpaw_fail(GPAW_EINVAL, "add_potential: out must not alias in");
+    return gpaw_check(launch_add_potential(out, in, v, nin, ng, dtype, (hipStream_t)stream),
+                      "add_potential");
+}
+
+extern "C" int gpaw_scale(double* y, double a, long n, void* stream)
+{
+    if (!y || n < 0) return gpaw_fail(GPAW_EINVAL, "scale: bad argument");
+    return gpaw_check(launch_scale(y, a, n, (hipStream_t)stream), "scale");
+}
+
 extern "C" int gpaw_autotune(int enable) { return autotune_set(enable); }
 
 extern "C" long gpaw_autotune_report(char* buf, long len) { return autotune_report(buf, len); }

@@ -130,5 +130,8 @@ hipError_t launch_copy(void* y, const void* x, int dtype, long long n, hipStream
 hipError_t launch_cast(void* y, int dty, const void* x, int dtx, long long n, hipStream_t st);
 // nbytes a multiple of 16, both pointers 16-byte aligned
 hipError_t launch_copy_bytes(void* y, const void* x, long long nbytes, hipStream_t st);
+hipError_t launch_add_potential(void* out, const void* in, const double* v, long long nin,
+                                long long ng, int dtype, hipStream_t st);
+hipError_t launch_scale(double* y, double a, long long n, hipStream_t st);
 
 }  // namespace gpaw

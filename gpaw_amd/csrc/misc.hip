@@ -61,6 +61,26 @@ __global__ __launch_bounds__(256) void k_copy(T* __restrict__ y, const T* __rest
     if (i < n) y[i] = x[i];
 }
 
+// out[b][g] += in[b][g] * v[g]: the local potential applied to bands
+// (numpy's Htpsit_G += psit_G * vt_G, gpaw/hamiltonian.py:393-398); v is
+// real and shared by all bands, complex bands scale both parts
+template <class V>
+__global__ __launch_bounds__(256) void k_add_pot(V* __restrict__ out, const V* __restrict__ in,
+                                                 const double* __restrict__ v, i64 ng, i64 n)
+{
+    typedef typename VT<V>::R RT;
+    const i64 stride = (i64)gridDim.x * 256;
+    for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+        out[i] = vadd(out[i], vmul(in[i], (RT)v[i % ng]));
+}
+
+// y *= a (numpy's in-place scaling, e.g. nt_sg *= x, gpaw/density.py:234)
+__global__ __launch_bounds__(256) void k_scale(double* __restrict__ y, double a, i64 n)
+{
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = y[i] * a;
+}
+
 // 16-byte copy, four loads in flight per thread, grid-stride over blocks of
 // 4 x 256 vectors (the attainable HBM read + write rate)
 __global__ __launch_bounds__(256) void k_copy16(uint4* __restrict__ y, const uint4* __restrict__ x,
@@ -157,6 +177,29 @@ hipError_t launch_cast(void* y, int dty, const void* x, int dtx, long long n, hi
                            (const double*)x, m);
     else
         return launch_copy(y, x, dty, n, st);
+    return launch_error();
+}
+
+hipError_t launch_add_potential(void* out, const void* in, const double* v, long long nin,
+                                long long ng, int dtype, hipStream_t st)
+{
+    const i64 n = (i64)nin * ng;
+    if (n <= 0) return hipSuccess;
+    i64 b = (n + 255) / 256;
+    if (b > 16384) b = 16384;
+    const dim3 g((unsigned)b);
+    switch (dtype) {
+    case DT_F64: launch_kernel(k_add_pot<double>, g, dim3(256), 0, st, (double*)out, (const double*)in, v, (i64)ng, n); break;
+    case DT_C128: launch_kernel(k_add_pot<double2>, g, dim3(256), 0, st, (double2*)out, (const double2*)in, v, (i64)ng, n); break;
+    default: return hipErrorInvalidValue;
+    }
+    return launch_error();
+}
+
+hipError_t launch_scale(double* y, double a, long long n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    launch_kernel(k_scale, grid1(n), dim3(256), 0, st, y, a, (i64)n);
     return launch_error();
 }
 

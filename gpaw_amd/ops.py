@@ -222,6 +222,34 @@ def copy(dst, src):
     d.finish()
 
 
+def add_potential(out, psi, v):
+    """out += psi * v on the device: psi / out of shape (..., n0, n1, n2)
+    (float64 or complex128 bands), v the float64 grid function shared by
+    the bands (gpaw/hamiltonian.py:393-398)."""
+    o = Arg(out, write=True, name='out')
+    a = Arg(psi, name='psi')
+    p = Arg(v, name='v')
+    if p.code != _lib.GPAW_F64:
+        raise TypeError('add_potential: the potential is float64')
+    if o.code != a.code or tuple(o.shape) != tuple(a.shape):
+        raise ValueError('add_potential: out and psi differ in dtype or shape')
+    ng = int(np.prod(p.shape))
+    if tuple(a.shape[-len(p.shape):]) != tuple(p.shape):
+        raise ValueError('add_potential: potential shape %s, bands %s' % (p.shape, a.shape))
+    nin = int(np.prod(a.shape)) // max(ng, 1)
+    _lib.check(_lib.lib().gpaw_add_potential(o.ptr, a.ptr, p.ptr, nin, ng, a.code,
+                                             _arrays.stream()))
+    o.finish()
+
+
+def scale(y, a):
+    """y *= a on float64 device tensors."""
+    yy = Arg(y, write=True)
+    _lib.check(_lib.lib().gpaw_scale(yy.ptr, float(a), int(np.prod(yy.shape)),
+                                     _arrays.stream()))
+    yy.finish()
+
+
 def autotune(enable=True):
     """Switch the launch-shape autotuner of the stencil kernels on or off;
     returns the previous setting."""

@@ -239,6 +239,18 @@ int gpaw_sub_scalar(double* y, const double* x, double b, long n,
  * Also the kernel that calibrates the attainable HBM read + write rate. */
 int gpaw_copy(void* y, const void* x, long nbytes, void* stream);
 
+/* out[b][g] += in[b][g] * v[g] for nin bands of ng points: the local
+ * potential v (float64, shared by the bands) applied after the kinetic
+ * stencil (replaces gpaw/hamiltonian.py:393-398, Htpsit_G += psit_G * vt_G,
+ * called from gpaw/wavefunctions/fd.py:50-52).  dtype GPAW_F64 or
+ * GPAW_C128 (both parts scaled); out must not alias in. */
+int gpaw_add_potential(void* out, const void* in, const double* v, long nin,
+                       long ng, int dtype, void* stream);
+
+/* y *= a over n doubles (numpy's in-place scaling, e.g. nt_sg *= x,
+ * gpaw/density.py:234). */
+int gpaw_scale(double* y, double a, long n, void* stream);
+
 /* Precision conversion between GPAW_F64 <-> GPAW_F32 or GPAW_C128 <->
  * GPAW_C64 (n elements). */
 int gpaw_cast(void* y, int dty, const void* x, int dtx, long n, void* stream);

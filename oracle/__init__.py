@@ -7,5 +7,6 @@ from .gpaw_oracle import (  # noqa: F401
     Kernels, Grid, laplace_stencil, gradient_stencil, mehrstellen_a,
     mehrstellen_b, Operator, Transformer, PoissonSolver, flatten_offsets,
     transformer_plan, gaussian_charges, madelung, bohr_per_angstrom,
-    Preconditioner, laplace,
+    Preconditioner, laplace, apply_pseudo_hamiltonian, interpolate_density,
+    restrict_potential, pair_potentials,
 )

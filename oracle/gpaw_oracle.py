@@ -15,6 +15,10 @@ points.  This module restates the Python layers that drive them:
 * charged-system helpers  gpaw/utilities/gauss.py:47-83,
                           gpaw/utilities/tools.py:92-124,
                           gpaw/utilities/ewald.py:12-150
+* preconditioner          gpaw/preconditioner.py:13-70
+* callers of the path     gpaw/wavefunctions/fd.py:50-52,
+                          gpaw/hamiltonian.py:254-259,371-398,
+                          gpaw/density.py:217-234, gpaw/xc/hybrid.py:148-152
 
 It is imported only by tests, the smoke check and the CPU-baseline leg of
 the benchmark.
@@ -673,3 +677,49 @@ class Preconditioner:
         d0 += (-s) * q0
         d0 *= -1.0
         return d0
+
+
+# ---------------------------------------------------------------------------
+# Callers of the hot path (restated call sites)
+# ---------------------------------------------------------------------------
+def apply_pseudo_hamiltonian(kin, vt_G, psit_xG, Htpsit_xG, phase_cd=None):
+    """FDWaveFunctions.apply_pseudo_hamiltonian (gpaw/wavefunctions/fd.py:50-52):
+    kin.apply over the bands, then Hamiltonian.apply_local_potential
+    (gpaw/hamiltonian.py:393-398), Htpsit_G += psit_G * vt_G band by band."""
+    kin.apply(psit_xG, Htpsit_xG, phase_cd)
+    if psit_xG.ndim == 3:
+        Htpsit_xG += psit_xG * vt_G
+    else:
+        for psit_G, Htpsit_G in zip(psit_xG, Htpsit_xG):
+            Htpsit_G += psit_G * vt_G
+
+
+def interpolate_density(interpolator, finegd, nt_sG, nt_sg, charge=0.0,
+                        comp_charge=0.0):
+    """Density.interpolate (gpaw/density.py:217-234) for given charges."""
+    for s in range(nt_sG.shape[0]):
+        interpolator.apply(nt_sG[s], nt_sg[s])
+    if not finegd.pbc_c.all():
+        pseudo_charge = -(charge + comp_charge)
+        if abs(pseudo_charge) > 1.0e-14:
+            x = pseudo_charge / finegd.integrate(nt_sg).sum()
+            nt_sg *= x
+
+
+def restrict_potential(restrictor, vt_sg, vHt_g, vt_sG):
+    """The restriction loop of Hamiltonian.update
+    (gpaw/hamiltonian.py:254-259): vt_g += vHt_g; restrict(vt_g, vt_G)."""
+    for vt_g, vt_G in zip(vt_sg, vt_sG):
+        vt_g += vHt_g
+        restrictor.apply(vt_g, vt_G)
+
+
+def pair_potentials(poisson, rhot_pg, charges, eps=1e-12):
+    """The exact-exchange pair loop's solves (gpaw/xc/hybrid.py:148-152)."""
+    vt_pg = np.zeros_like(rhot_pg)
+    niter = []
+    for p in range(rhot_pg.shape[0]):
+        vt_pg[p] = 0.0
+        niter.append(poisson.solve(vt_pg[p], -rhot_pg[p], charge=charges[p],
+                                   eps=eps, zero_initial_phi=True))
+    return vt_pg, niter
