@@ -243,6 +243,94 @@ def preconditioner_c3(torch, np):
             'bytes_per_call': algo}
 
 
+def callers_c3(torch, np):
+    """The hot path's callers (gpaw_amd.callers) at production shapes:
+    * pseudo-Hamiltonian: nn=3 kinetic stencil over 256 complex128 bands of
+      64^3 (Bloch phases) plus the local potential; algorithmic bytes per
+      band point: 32 (stencil) + 48 (read psi, read and write Htpsi), the
+      potential read once per call;
+    * density interpolation: 2 spins, 64^3 -> 128^3 zero boundary, nn=3,
+      with the charge renormalisation (interpolation reads 8 B per coarse
+      and writes 8 B per fine point; the scaling reads and writes the fine
+      grid once more and the integral reads it once);
+    * potential restriction: 2 spins, vt_g += vHt_g (24 B per fine point)
+      and nn=3 restriction to 64^3 (8 B per fine and per coarse point);
+    * pair potentials: 8 pair-density solves on a 64^3 zero-boundary grid
+      (63^3 arrays), eps = 1e-12, half of them charged."""
+    from gpaw_amd import callers
+    from gpaw_amd.fd_operators import Laplace
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.poisson import PoissonSolver
+    from gpaw_amd.transformers import Transformer
+    from gpaw_amd.utilities import gaussian_density, random_gaussian_charges
+    out = {}
+    N, nb = 64, 256
+    gd = GridDescriptor((N,) * 3, (N * 0.2 / BOHR,) * 3, True)
+    g = torch.Generator(device='cuda').manual_seed(21)
+    psi = torch.randn((nb, N, N, N), dtype=torch.complex128, device='cuda', generator=g)
+    hpsi = torch.empty_like(psi)
+    v = torch.randn((N, N, N), dtype=torch.float64, device='cuda', generator=g)
+    kin = Laplace(gd, -0.5, 3, complex)
+    ph = gd.phase_cd(np.random.default_rng(5).uniform(-0.5, 0.5, 3))
+    t = event_time(torch, lambda: callers.apply_pseudo_hamiltonian(kin, v, psi, hpsi, ph), 10)
+    algo = 80.0 * nb * N ** 3 + 8.0 * N ** 3
+    out['pseudo_hamiltonian'] = {'config': 'nn=3 kinetic + local potential, complex128, '
+                                           '256 bands x 64^3, Bloch phases',
+                                 'avg_call_ms': round(t * 1e3, 3),
+                                 'GBps': round(algo / t / 1e9, 1)}
+    del psi, hpsi
+    # density interpolation and potential restriction, 2 spins
+    cg = GridDescriptor((N,) * 3, (N * 0.2 / BOHR,) * 3, False)
+    fg = cg.refine()
+    ns = 2
+    nt_sG = torch.rand((ns,) + tuple(cg.n_c), dtype=torch.float64, device='cuda', generator=g)
+    nt_sg = torch.empty((ns,) + tuple(fg.n_c), dtype=torch.float64, device='cuda')
+    itp = Transformer(cg, fg, 3)
+    t = event_time(torch, lambda: callers.interpolate_density(itp, fg, nt_sG, nt_sg,
+                                                              charge=-8.0), 10)
+    nc, nf = float(np.prod(cg.n_c)), float(np.prod(fg.n_c))
+    algo = ns * (8 * nc + 8 * nf) + ns * nf * 24
+    out['interpolate_density'] = {'config': '2 spins, 64^3 -> 128^3 zero boundary, nn=3, '
+                                            'charge renormalisation',
+                                  'avg_call_ms': round(t * 1e3, 3),
+                                  'GBps': round(algo / t / 1e9, 1)}
+    vt_sg = torch.randn((ns,) + tuple(fg.n_c), dtype=torch.float64, device='cuda', generator=g)
+    vH = torch.randn(tuple(fg.n_c), dtype=torch.float64, device='cuda', generator=g)
+    vt_sG = torch.empty((ns,) + tuple(cg.n_c), dtype=torch.float64, device='cuda')
+    rst = Transformer(fg, cg, 3)
+    t = event_time(torch, lambda: callers.restrict_potential(rst, vt_sg, vH, vt_sG), 10)
+    algo = ns * (24 * nf + 8 * nf + 8 * nc)
+    out['restrict_potential'] = {'config': '2 spins, 128^3 -> 64^3 zero boundary, nn=3',
+                                 'avg_call_ms': round(t * 1e3, 3),
+                                 'GBps': round(algo / t / 1e9, 1)}
+    del nt_sG, nt_sg, vt_sg, vH, vt_sG
+    # pair potentials
+    pg = GridDescriptor((N,) * 3, (N * 0.2 / BOHR,) * 3, False)
+    rng = np.random.default_rng(64)
+    rho = []
+    for p in range(8):
+        c, w, q = random_gaussian_charges(pg, rng, 2, 0.5 / BOHR, 1.0 / BOHR, 2.0 / BOHR)
+        if p % 2:
+            q = np.abs(q)   # a charged pair density (n1 == n2 in the exchange loop)
+        rho.append(gaussian_density(pg, c, w, q))
+    rho_d = torch.from_numpy(np.stack(rho)).to('cuda')
+    charges = [-float(pg.integrate(r)) if p % 2 else 0.0 for p, r in enumerate(rho)]
+    ps = PoissonSolver(nn=3, relax='J')
+    ps.set_grid_descriptor(pg)
+    ps.initialize()
+    callers.pair_potentials(ps, rho_d, charges, eps=1e-12)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, niter = callers.pair_potentials(ps, rho_d, charges, eps=1e-12)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out['pair_potentials'] = {'config': '8 pair densities, 64^3 zero boundary (63^3), '
+                                        'eps=1e-12, 4 charged',
+                              'solves_per_s': round(8 / dt, 2),
+                              'iterations': niter}
+    return out
+
+
 def mixed_vs_fp64(torch, np, N=160, h=0.18, count=32, reps=10):
     """Mixed-precision vs plain FP64 multigrid on the 160^3 zero-boundary
     configuration (159^3 arrays, h = 0.18 A, 32 seeded neutral Gaussians of
@@ -375,6 +463,7 @@ def cpu_baseline(np, gd_shape_args, rho, eps, budget):
 SECTIONS = {
     'fd_laplace': lambda torch, np, args: fd_laplace_c3(torch, np),
     'preconditioner': lambda torch, np, args: preconditioner_c3(torch, np),
+    'callers': lambda torch, np, args: callers_c3(torch, np),
     'mixed_precision': lambda torch, np, args: mixed_vs_fp64(torch, np),
     'serving': lambda torch, np, args: serving(torch, np, n=args.serve_n,
                                                rate=args.serve_rate),
@@ -468,6 +557,7 @@ def main():
     if not args.no_extra:
         out['fd_laplace'] = fd_laplace_c3(torch, np)
         out['preconditioner'] = preconditioner_c3(torch, np)
+        out['callers'] = callers_c3(torch, np)
         out['mixed_precision'] = mixed_vs_fp64(torch, np)
         out['serving'] = serving(torch, np, n=args.serve_n, rate=args.serve_rate)
     if not args.no_cpu:
