@@ -62,6 +62,7 @@ PROTOTYPES = {
     'gpaw_cast': (_i, [_vp, _i, _vp, _i, _l, _vp]),
     'gpaw_copy': (_i, [_vp, _vp, _l, _vp]),
     'gpaw_add_potential': (_i, [_vp, _vp, _vp, _l, _l, _i, _vp]),
+    'gpaw_operator_apply_potential': (_i, [_vp, _vp, _vp, _l, _i, _dp, _vp, _vp]),
     'gpaw_scale': (_i, [_vp, ctypes.c_double, _l, _vp]),
 }
 

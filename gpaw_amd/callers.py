@@ -37,8 +37,10 @@ def apply_pseudo_hamiltonian(kin, vt_G, psit_xG, Htpsit_xG, phase_cd=None):
     out = Arg(Htpsit_xG, write=True, name='Htpsit_xG')
     psi = Arg(psit_xG, name='psit_xG')
     v = Arg(vt_G, name='vt_G')
-    kin.apply(psi.tensor, out.tensor, phase_cd)
-    ops.add_potential(out.tensor, psi.tensor, v.tensor)
+    if not kin.is_allocated():
+        kin.allocate()
+    # one pass: the stencil sum, then + psi * vt, rounded as the two calls
+    kin.operator.apply_potential(psi.tensor, out.tensor, v.tensor, phase_cd)
     out.finish()
 
 

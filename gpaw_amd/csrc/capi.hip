@@ -199,6 +199,35 @@ extern "C" int gpaw_operator_apply(gpaw_operator* op, const void* in, void* out,
                       "operator apply");
 }
 
+extern "C" int gpaw_operator_apply_potential(gpaw_operator* op, const void* in, void* out,
+                                             long nin, int dtype, const double* phases,
+                                             const double* v, void* stream)
+{
+    if (!op) return gpaw_fail(GPAW_EINVAL, "operator: NULL handle");
+    int rc = gpaw_check_dtype(op->real, dtype);
+    if (rc) return rc;
+    if (dtype != DT_F64 && dtype != DT_C128)
+        return gpaw_fail(GPAW_EINVAL, "apply_potential: float64 or complex128 bands");
+    if (nin < 0 || nin > 65535) return gpaw_fail(GPAW_EINVAL, "operator: nin = %ld", nin);
+    if (nin == 0) return GPAW_OK;
+    if (!in || !out || !v) return gpaw_fail(GPAW_EINVAL, "operator: NULL array");
+    if (in == out) return gpaw_fail(GPAW_EINVAL, "operator: in and out must differ");
+    Phases ph;
+    gpaw_phases(phases, ph);
+    hipStream_t st = (hipStream_t)stream;
+    if (op->st.star_r > 0)   // one pass: out = L in + in * v
+        return gpaw_check(launch_stencil(op->st, dtype, ST_HPSI, in, out, v, op->g, ph, (int)nin,
+                                         0.0, nullptr, nullptr, true, st),
+                          "operator apply_potential");
+    // other stencils: the stencil, then the potential (same arithmetic)
+    hipError_t e = launch_stencil(op->st, dtype, ST_APPLY, in, out, nullptr, op->g, ph, (int)nin,
+                                  0.0, nullptr, nullptr, true, st);
+    if (e == hipSuccess)
+        e = launch_add_potential(out, in, v, nin,
+                                 (long long)op->g.n[0] * op->g.n[1] * op->g.n[2], dtype, st);
+    return gpaw_check(e, "operator apply_potential");
+}
+
 // Jacobi sweeps on f in place: ping-pong through `scratch`, ending in f.
 int gpaw_op_jacobi(gpaw_operator* op, double* f, const double* s, int nrelax, double w,
                    double* scratch, hipStream_t st)

@@ -64,6 +64,17 @@ __device__ __forceinline__ void vstore_nt(float2* p, float2 v)
     __builtin_nontemporal_store(v.y, &p->y);
 }
 
+// real part, and a value with the given real part (imaginary part 0)
+__device__ __forceinline__ double vreal(double v) { return v; }
+__device__ __forceinline__ float vreal(float v) { return v; }
+__device__ __forceinline__ double vreal(double2 v) { return v.x; }
+__device__ __forceinline__ float vreal(float2 v) { return v.x; }
+template <class V> __device__ __forceinline__ V vfromreal(typename VT<V>::R r);
+template <> __device__ __forceinline__ double vfromreal<double>(double r) { return r; }
+template <> __device__ __forceinline__ float vfromreal<float>(float r) { return r; }
+template <> __device__ __forceinline__ double2 vfromreal<double2>(double r) { return make_double2(r, 0.0); }
+template <> __device__ __forceinline__ float2 vfromreal<float2>(float r) { return make_float2(r, 0.0f); }
+
 __device__ __forceinline__ double vmul(double a, double c) { return a * c; }
 __device__ __forceinline__ float vmul(float a, float c) { return a * c; }
 __device__ __forceinline__ double2 vmul(double2 a, double c) { return make_double2(a.x * c, a.y * c); }

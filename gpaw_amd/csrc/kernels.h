@@ -17,6 +17,8 @@ enum StencilMode {
     ST_JACOBI = 2,   // out = (1-w) in + w (aux - sum_{c>=1} ...) / coef_0
     ST_PCUPD = 3,    // out = in - w (apply(in) - aux)     (FP64 types only)
     ST_PCFIN = 4,    // out = in - w (apply(in) + aux)     (FP64 types only)
+    ST_HPSI = 5,     // out = apply(in) + in * v, v a real grid function shared
+                     // by all bands, passed as aux      (FP64 types only)
 };
 
 // Per-point result of a stencil pass from the stencil sum x (over c >= 1 for
@@ -28,7 +30,9 @@ enum StencilMode {
 //   ST_PCFIN   q = L d; q -= res; d += -w * q; d *= -1 evaluated on e = -d
 //              (gpaw/preconditioner.py:67-70: negation commutes exactly with
 //              the linear stencil and with rounding, so the value equals
-//              e - w * (L e + res)).
+//              e - w * (L e + res));
+//   ST_HPSI    Htpsi = kin psi; Htpsi += psi * vt (gpaw/wavefunctions/fd.py:51-52,
+//              gpaw/hamiltonian.py:393-398); s carries vt in its real part.
 template <int MODE, class V, class RT>
 __device__ __forceinline__ V st_epilogue(V x, V f, V s, RT w, RT c0)
 {
@@ -36,14 +40,26 @@ __device__ __forceinline__ V st_epilogue(V x, V f, V s, RT w, RT c0)
     if (MODE == ST_RESID) return vsub(x, s);
     if (MODE == ST_JACOBI) return vadd(vmul(f, (RT)1 - w), vdiv_r(vmul(vsub(s, x), w), c0));
     if (MODE == ST_PCUPD) return vsub(f, vmul(vsub(x, s), w));
+    if (MODE == ST_HPSI) return vadd(x, vmul(f, vreal(s)));
     return vsub(f, vmul(vadd(x, s), w));
+}
+
+// The aux value of point `idx` of a band whose first point is `boff`: a
+// band's own aux array, or for ST_HPSI the real potential shared by all
+// bands (its value in the real part).
+template <int MODE, class V>
+__device__ __forceinline__ V aux_value(const V* __restrict__ aux, i64 boff, i64 idx)
+{
+    if (MODE == ST_HPSI) return vfromreal<V>(((const typename VT<V>::R*)aux)[idx]);
+    return aux[boff + idx];
 }
 
 // Which (type, mode, norm) combinations are compiled.
 template <class V, int MODE>
 constexpr bool st_mode_ok()
 {
-    if (MODE == ST_PCUPD || MODE == ST_PCFIN) return sizeof(typename VT<V>::R) == 8;
+    if (MODE == ST_PCUPD || MODE == ST_PCFIN || MODE == ST_HPSI)
+        return sizeof(typename VT<V>::R) == 8;
     if (MODE == ST_JACOBI) return !VT<V>::cplx;
     return true;
 }

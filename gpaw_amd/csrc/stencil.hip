@@ -131,7 +131,9 @@ __global__ __launch_bounds__(256) void k_star(const V* __restrict__ in, V* __res
             for (int k = 1; k <= R; k++)      // axis 0: -k
                 x = vadd(x, vmul(qv[R - k], (RT)sc.c[5 * R + k]));
             const i64 idx = boff + (i64)i0 * plane + row + i2;
-            const V r = st_epilogue<MODE>(x, qv[R], MODE == ST_APPLY ? vzero<V>() : aux[idx],
+            const V r = st_epilogue<MODE>(x, qv[R],
+                                          MODE == ST_APPLY ? vzero<V>()
+                                                           : aux_value<MODE>(aux, boff, idx - boff),
                                           (RT)w, (RT)sc.c[0]);
             if (NORM) acc += norm2(r);
             if (WRITE) out[idx] = r;
@@ -170,7 +172,7 @@ __global__ __launch_bounds__(256) void k_generic(const V* __restrict__ in, V* __
                                          i2 + off[3 * c + 2], g, ph),
                              (RT)coefs[c]));
         const V r = st_epilogue<MODE>(x, MODE == ST_APPLY || MODE == ST_RESID ? vzero<V>() : a[t],
-                                      MODE == ST_APPLY ? vzero<V>() : aux[boff + t], (RT)w,
+                                      MODE == ST_APPLY ? vzero<V>() : aux_value<MODE>(aux, boff, t), (RT)w,
                                       (RT)coefs[0]);
         if (NORM) acc = norm2(r);
         if (WRITE) out[boff + t] = r;
@@ -315,6 +317,7 @@ hipError_t dispatch_mode(bool star, const Stencil& s, int mode, const void* in, 
     case ST_JACOBI: return kind_launch<V, ST_JACOBI>(star, s, in, out, aux, g, ph, nin, w, partial, write, st);
     case ST_PCUPD: return kind_launch<V, ST_PCUPD>(star, s, in, out, aux, g, ph, nin, w, partial, write, st);
     case ST_PCFIN: return kind_launch<V, ST_PCFIN>(star, s, in, out, aux, g, ph, nin, w, partial, write, st);
+    case ST_HPSI: return kind_launch<V, ST_HPSI>(star, s, in, out, aux, g, ph, nin, w, partial, write, st);
     }
     return hipErrorInvalidValue;
 }
@@ -528,7 +531,7 @@ TuneVal tune(const TuneKey& key, const LaunchCfg& heuristic, const Stencil& s, i
     std::vector<LaunchCfg> cand;
     LaunchCfg qc{K_QUEUE, 64, TileShape{}};
     cand.push_back(qc);
-    const bool wide = dtype_complex(dtype) && mode == ST_APPLY && s.star_r <= 3;
+    const bool wide = dtype_complex(dtype) && (mode == ST_APPLY || mode == ST_HPSI) && s.star_r <= 3;
     for (int nt : {TILE_NT, TILE_NT_MID, TILE_NT_WIDE}) {
         if (nt != TILE_NT && !wide) continue;
         for (int TZ : {64, 32})
@@ -603,8 +606,9 @@ hipError_t launch_stencil(const Stencil& s, int dtype, int mode, const void* in,
 {
     LaunchCfg c{K_GENERIC, 64, TileShape{}};
     // complex applies of range <= 3 use 1024-thread tiles
-    const int nt = (dtype_complex(dtype) && mode == ST_APPLY && s.star_r <= 3) ? TILE_NT_WIDE
-                                                                              : TILE_NT;
+    const int nt = (dtype_complex(dtype) && (mode == ST_APPLY || mode == ST_HPSI) && s.star_r <= 3)
+                       ? TILE_NT_WIDE
+                       : TILE_NT;
     c.kind = stencil_kind(s, g, dtype_size(dtype), nin, c.TZ, c.sh, nt);
     if (npartial) *npartial = stencil_partials(s, g, nin);
     if (!partial && s.star_r >= 1 && s.star_r <= 4 && forced_kind() < 0 && tune_enabled()) {

@@ -170,7 +170,8 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
         }
         if (MODE != ST_APPLY) {
 #pragma unroll
-            for (int j = 0; j < ROWS; j++) sn[j] = sa[(i64)i0s * plane + (inr[j] ? pref_off(qr[j]) : 0)];
+            for (int j = 0; j < ROWS; j++)
+                sn[j] = aux_value<MODE>(sa, 0, (i64)i0s * plane + (inr[j] ? pref_off(qr[j]) : 0));
         }
     }
 
@@ -234,7 +235,7 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
             if (MODE != ST_APPLY) {
 #pragma unroll
                 for (int j = 0; j < ROWS; j++)
-                    sn[j] = sa[(i64)(i0 + 1) * plane + (inr[j] ? pref_off(qr[j]) : 0)];
+                    sn[j] = aux_value<MODE>(sa, 0, (i64)(i0 + 1) * plane + (inr[j] ? pref_off(qr[j]) : 0));
             }
         }
         __syncthreads();
@@ -299,7 +300,8 @@ __global__ __launch_bounds__(NT) void k_star_tile(const V* __restrict__ in, V* _
     const i64 boff = (i64)band * ng;
     const int i0s = ch * sh.q;
     const double acc = star_tile_sweep<V, R, TZ, ROWS, MODE, NORM, WRITE, NT>(
-        in + boff, out + boff, aux ? aux + boff : nullptr, g, ph, sc, w, tz * TZ, ty * TG::TY,
+        in + boff, out + boff, (aux && MODE != ST_HPSI) ? aux + boff : aux, g, ph, sc, w, tz * TZ,
+        ty * TG::TY,
         i0s, min(i0s + sh.q, g.n[0]), lds);
     if (NORM) {
         __shared__ double red[NT];

@@ -73,6 +73,25 @@ class OperatorHandle:
             _arrays.stream()))
         b.finish()
 
+    def apply_potential(self, in_xg, out_xg, v_g, phases=None):
+        """out = L in + in * v in one pass (v: float64 grid function shared
+        by the bands; gpaw/wavefunctions/fd.py:51-52)."""
+        a = Arg(in_xg, name='input')
+        b = Arg(out_xg, write=True, name='output')
+        v = Arg(v_g, name='potential')
+        if a.shape != b.shape or a.shape[-3:] != self.shape or tuple(v.shape) != self.shape:
+            raise ValueError('operator apply_potential: shapes %s -> %s, potential %s for grid %s'
+                             % (a.shape, b.shape, v.shape, self.shape))
+        if a.code != b.code:
+            raise TypeError('operator apply_potential: input and output dtypes differ')
+        if v.code != _lib.GPAW_F64:
+            raise TypeError('operator apply_potential: the potential is float64')
+        _, pp = _phases(phases)
+        _lib.check(_lib.lib().gpaw_operator_apply_potential(
+            self.h, a.ptr, b.ptr, _nin(a.shape, self.shape), a.code, pp, v.ptr,
+            _arrays.stream()))
+        b.finish()
+
     def relax(self, relax_method, f_g, s_g, nrelax, w=1.0):
         if w is None:   # PyArg_ParseTuple("iOOi|d") rejects None
             raise TypeError('a float is required')

@@ -60,6 +60,16 @@ int gpaw_operator_apply(gpaw_operator* op, const void* in, void* out,
                         long nin, int dtype, const double* phases,
                         void* stream);
 
+/* out = L in + in * v for nin bands: the kinetic stencil and the local
+ * potential v (float64, shared by the bands) in one pass.  Replaces the pair
+ * kin.apply(psit_xG, Htpsit_xG, phase_cd) + apply_local_potential
+ * (gpaw/wavefunctions/fd.py:51-52, gpaw/hamiltonian.py:393-398), with the
+ * same rounding (the stencil sum, then + psi * v).  dtype GPAW_F64 or
+ * GPAW_C128; in != out. */
+int gpaw_operator_apply_potential(gpaw_operator* op, const void* in, void* out,
+                                  long nin, int dtype, const double* phases,
+                                  const double* v, void* stream);
+
 /* Operator.relax(method, f, s, nrelax, w): Operator_relax,
  * c/operators.c:49-91 -> bmgs_relax, c/bmgs/relax.c:7-73.  float64 only;
  * f is updated in place.  method 2 = weighted Jacobi, 1 = lexicographic

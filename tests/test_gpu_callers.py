@@ -59,6 +59,27 @@ def test_apply_pseudo_hamiltonian(dtype, pbc):
     assert np.array_equal(out1, want[0])
 
 
+def test_apply_pseudo_hamiltonian_generic_stencil():
+    """A non-orthogonal cell (generic stencil): the stencil pass and the
+    potential pass run separately, with the same result as the oracle."""
+    print('case pseudo-Hamiltonian generic', flush=True)
+    from gpaw_amd.callers import apply_pseudo_hamiltonian
+    from gpaw_amd.fd_operators import Laplace
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    cell = np.array([[6.0, 0.0, 0.0], [3.0, 5.196152422706632, 0.0], [0.0, 0.0, 7.0]])
+    gd = GridDescriptor((16, 16, 20), cell, True)
+    og = O.Grid((16, 16, 20), cell, True)
+    rng = np.random.default_rng(11)
+    psi = rng.standard_normal((3, 16, 16, 20)) + 1j * rng.standard_normal((3, 16, 16, 20))
+    v = rng.standard_normal((16, 16, 20))
+    ph = og.phase_cd([0.1, 0.25, -0.2])
+    want = np.zeros_like(psi)
+    O.apply_pseudo_hamiltonian(O.laplace(og, -0.5, 2, dtype=complex), v, psi, want, ph)
+    got = dev(np.zeros_like(psi))
+    apply_pseudo_hamiltonian(Laplace(gd, -0.5, 2, complex), dev(v), dev(psi), got, ph)
+    assert np.array_equal(got.cpu().numpy(), want)
+
+
 @pytest.mark.parametrize('pbc', [True, False])
 def test_interpolate_density(pbc):
     print('case interpolate density', pbc, flush=True)
