@@ -531,7 +531,7 @@ TuneVal tune(const TuneKey& key, const LaunchCfg& heuristic, const Stencil& s, i
     std::vector<LaunchCfg> cand;
     LaunchCfg qc{K_QUEUE, 64, TileShape{}};
     cand.push_back(qc);
-    const bool wide = dtype_complex(dtype) && (mode == ST_APPLY || mode == ST_HPSI) && s.star_r <= 3;
+    const bool wide = dtype_complex(dtype) && s.star_r <= 3;
     for (int nt : {TILE_NT, TILE_NT_MID, TILE_NT_WIDE}) {
         if (nt != TILE_NT && !wide) continue;
         for (int TZ : {64, 32})

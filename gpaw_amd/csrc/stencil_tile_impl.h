@@ -14,7 +14,7 @@ hipError_t tile_go(const Stencil& s, const void* in, void* out, const void* aux,
     TileCoefs sc;
     for (int c = 0; c < 6 * R + 1; c++) sc.c[c] = s.coefs[c];
     const i64 ng = (i64)g.n[0] * g.n[1] * g.n[2];
-    if constexpr (VT<V>::cplx && (MODE == ST_APPLY || MODE == ST_HPSI) && R <= 3) {
+    if constexpr (VT<V>::cplx && R <= 3) {   // wider blocks: complex data, range <= 3
         if (sh.nt == TILE_NT_MID)
             return launch_kernel((k_star_tile<V, R, TZ, 1, MODE, NORM, WRITE, TILE_NT_MID>),
                                  dim3(sh.nblocks), dim3(TZ, TILE_NT_MID / TZ), 0, st,
