@@ -90,10 +90,32 @@ __device__ __forceinline__ float vsub(float a, float b) { return a - b; }
 __device__ __forceinline__ double2 vsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 vsub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
+// Division by a real coefficient.  FP64: a true division (the reference
+// divides).  FP32 (only the mixed solver's preconditioning V-cycle and the
+// FP32 variants, which are compared with FP64 to a tolerance): multiplication
+// by the reciprocal, which the compiler hoists out of the loops.
 __device__ __forceinline__ double vdiv_r(double a, double c) { return a / c; }
-__device__ __forceinline__ float vdiv_r(float a, float c) { return a / c; }
+__device__ __forceinline__ float vdiv_r(float a, float c) { return a * (1.0f / c); }
 __device__ __forceinline__ double2 vdiv_r(double2 a, double c) { return make_double2(a.x / c, a.y / c); }
-__device__ __forceinline__ float2 vdiv_r(float2 a, float c) { return make_float2(a.x / c, a.y / c); }
+__device__ __forceinline__ float2 vdiv_r(float2 a, float c)
+{
+    const float r = 1.0f / c;
+    return make_float2(a.x * r, a.y * r);
+}
+
+// x + v * c: FP64 rounds the product and the sum separately (the reference's
+// x += a * coef, never contracted); FP32 uses one fused multiply-add (half
+// the arithmetic of the FP32 stencils, one rounding instead of two).
+__device__ __forceinline__ double vmac(double x, double v, double c) { return x + v * c; }
+__device__ __forceinline__ double2 vmac(double2 x, double2 v, double c)
+{
+    return make_double2(x.x + v.x * c, x.y + v.y * c);
+}
+__device__ __forceinline__ float vmac(float x, float v, float c) { return __builtin_fmaf(v, c, x); }
+__device__ __forceinline__ float2 vmac(float2 x, float2 v, float c)
+{
+    return make_float2(__builtin_fmaf(v.x, c, x.x), __builtin_fmaf(v.y, c, x.y));
+}
 
 // multiply by a phase factor (complex only; identity for real data)
 __device__ __forceinline__ double pmul(double v, double, double) { return v; }

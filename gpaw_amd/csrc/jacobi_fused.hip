@@ -91,12 +91,12 @@ __global__ __launch_bounds__(JF_NT) void k_jacobi_fused(const V* __restrict__ in
                 if (x < s || x >= E - s) continue;
                 const int cell = (x * E + ty) * E + tz;
                 V xs = V(0);
-                xs = xs + src[cell + 1] * (RT)c.c[1];
-                xs = xs + src[cell - 1] * (RT)c.c[2];
-                xs = xs + src[cell + E] * (RT)c.c[3];
-                xs = xs + src[cell - E] * (RT)c.c[4];
-                xs = xs + src[cell + E * E] * (RT)c.c[5];
-                xs = xs + src[cell - E * E] * (RT)c.c[6];
+                xs = vmac(xs, src[cell + 1], (RT)c.c[1]);
+                xs = vmac(xs, src[cell - 1], (RT)c.c[2]);
+                xs = vmac(xs, src[cell + E], (RT)c.c[3]);
+                xs = vmac(xs, src[cell - E], (RT)c.c[4]);
+                xs = vmac(xs, src[cell + E * E], (RT)c.c[5]);
+                xs = vmac(xs, src[cell - E * E], (RT)c.c[6]);
                 const V f = st_epilogue<ST_JACOBI>(xs, src[cell], r[p], wr, (RT)c.c[0]);
                 dst[cell] = ((outside >> p) & 1) ? V(0) : f;
             }

@@ -116,20 +116,20 @@ __global__ __launch_bounds__(256) void k_star(const V* __restrict__ in, V* __res
             }
             const V* __restrict__ pl = a + (i64)i0 * plane;
             V x = vzero<V>();
-            if (MODE != ST_JACOBI) x = vadd(x, vmul(qv[R], (RT)sc.c[0]));
+            if (MODE != ST_JACOBI) x = vmac(x, qv[R], (RT)sc.c[0]);
 #pragma unroll
             for (int k = 0; k < 2 * R; k++)   // axis 2: +1..+R then -1..-R
-                x = vadd(x, vmul(flagged_load(pl + row + zc[k], zf[k], ph, 2), (RT)sc.c[1 + k]));
+                x = vmac(x, flagged_load(pl + row + zc[k], zf[k], ph, 2), (RT)sc.c[1 + k]);
 #pragma unroll
             for (int k = 0; k < 2 * R; k++)   // axis 1
-                x = vadd(x, vmul(flagged_load(pl + yc[k] + i2, yf[k], ph, 1),
-                                 (RT)sc.c[1 + 2 * R + k]));
+                x = vmac(x, flagged_load(pl + yc[k] + i2, yf[k], ph, 1),
+                         (RT)sc.c[1 + 2 * R + k]);
 #pragma unroll
             for (int k = 1; k <= R; k++)      // axis 0: +k from the queue
-                x = vadd(x, vmul(qv[R + k], (RT)sc.c[4 * R + k]));
+                x = vmac(x, qv[R + k], (RT)sc.c[4 * R + k]);
 #pragma unroll
             for (int k = 1; k <= R; k++)      // axis 0: -k
-                x = vadd(x, vmul(qv[R - k], (RT)sc.c[5 * R + k]));
+                x = vmac(x, qv[R - k], (RT)sc.c[5 * R + k]);
             const i64 idx = boff + (i64)i0 * plane + row + i2;
             const V r = st_epilogue<MODE>(x, qv[R],
                                           MODE == ST_APPLY ? vzero<V>()
@@ -168,9 +168,9 @@ __global__ __launch_bounds__(256) void k_generic(const V* __restrict__ in, V* __
         const V* __restrict__ a = in + boff;
         V x = vzero<V>();
         for (int c = (MODE == ST_JACOBI ? 1 : 0); c < nc; c++)
-            x = vadd(x, vmul(ghost_fetch(a, i0 + off[3 * c], i1 + off[3 * c + 1],
-                                         i2 + off[3 * c + 2], g, ph),
-                             (RT)coefs[c]));
+            x = vmac(x, ghost_fetch(a, i0 + off[3 * c], i1 + off[3 * c + 1], i2 + off[3 * c + 2],
+                                    g, ph),
+                     (RT)coefs[c]);
         const V r = st_epilogue<MODE>(x, MODE == ST_APPLY || MODE == ST_RESID ? vzero<V>() : a[t],
                                       MODE == ST_APPLY ? vzero<V>() : aux_value<MODE>(aux, boff, t), (RT)w,
                                       (RT)coefs[0]);

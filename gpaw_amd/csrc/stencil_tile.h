@@ -251,19 +251,19 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
         for (int j = 0; j < ROWS; j++) {
             const V* row = B + (R + tyi * ROWS + j) * LW + R + tx;
             V x = vzero<V>();
-            if (MODE != ST_JACOBI) x = vadd(x, vmul(q[j][R], (RT)sc.c[0]));
+            if (MODE != ST_JACOBI) x = vmac(x, q[j][R], (RT)sc.c[0]);
 #pragma unroll
-            for (int k = 1; k <= R; k++) x = vadd(x, vmul(row[k], (RT)sc.c[k]));
+            for (int k = 1; k <= R; k++) x = vmac(x, row[k], (RT)sc.c[k]);
 #pragma unroll
-            for (int k = 1; k <= R; k++) x = vadd(x, vmul(row[-k], (RT)sc.c[R + k]));
+            for (int k = 1; k <= R; k++) x = vmac(x, row[-k], (RT)sc.c[R + k]);
 #pragma unroll
-            for (int k = 1; k <= R; k++) x = vadd(x, vmul(win[R + j + k], (RT)sc.c[2 * R + k]));
+            for (int k = 1; k <= R; k++) x = vmac(x, win[R + j + k], (RT)sc.c[2 * R + k]);
 #pragma unroll
-            for (int k = 1; k <= R; k++) x = vadd(x, vmul(win[R + j - k], (RT)sc.c[3 * R + k]));
+            for (int k = 1; k <= R; k++) x = vmac(x, win[R + j - k], (RT)sc.c[3 * R + k]);
 #pragma unroll
-            for (int k = 1; k <= R; k++) x = vadd(x, vmul(q[j][R + k], (RT)sc.c[4 * R + k]));
+            for (int k = 1; k <= R; k++) x = vmac(x, q[j][R + k], (RT)sc.c[4 * R + k]);
 #pragma unroll
-            for (int k = 1; k <= R; k++) x = vadd(x, vmul(q[j][R - k], (RT)sc.c[5 * R + k]));
+            for (int k = 1; k <= R; k++) x = vmac(x, q[j][R - k], (RT)sc.c[5 * R + k]);
             rp[j] = st_epilogue<MODE>(x, q[j][R], MODE == ST_APPLY ? vzero<V>() : s[j], (RT)w,
                                       (RT)sc.c[0]);
             if (NORM && inr[j]) acc += tnorm2(rp[j]);

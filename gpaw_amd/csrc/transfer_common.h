@@ -52,9 +52,9 @@ template <class V, int K>
 __device__ __forceinline__ V rst_combine(const V* s)
 {
     typedef typename VT<V>::R RT;
-    V x = vadd(s[0], vmul(vadd(s[1], s[2]), (RT)W<K>::w(1)));
+    V x = vmac(s[0], vadd(s[1], s[2]), (RT)W<K>::w(1));
 #pragma unroll
-    for (int j = 2; j <= K / 2; j++) x = vadd(x, vmul(vadd(s[2 * j - 1], s[2 * j]), (RT)W<K>::w(j)));
+    for (int j = 2; j <= K / 2; j++) x = vmac(x, vadd(s[2 * j - 1], s[2 * j]), (RT)W<K>::w(j));
     return vmul(x, (RT)0.5);
 }
 
@@ -74,7 +74,7 @@ __device__ __forceinline__ V ip_combine(const V* s)
     typedef typename VT<V>::R RT;
     V x = vmul(vadd(s[0], s[1]), (RT)W<K>::w(1));
 #pragma unroll
-    for (int j = 2; j <= K / 2; j++) x = vadd(x, vmul(vadd(s[2 * j - 2], s[2 * j - 1]), (RT)W<K>::w(j)));
+    for (int j = 2; j <= K / 2; j++) x = vmac(x, vadd(s[2 * j - 2], s[2 * j - 1]), (RT)W<K>::w(j));
     return x;
 }
 
@@ -108,7 +108,7 @@ __device__ __forceinline__ V rst_yz(const V* __restrict__ a, int j0, int c1, int
         const int o = 2 * j - 1;
         const V p = rst_z<V, K>(a, j0, c1 + o, c2, t, ph);
         const V m = rst_z<V, K>(a, j0, c1 - o, c2, t, ph);
-        x = vadd(x, vmul(vadd(p, m), (RT)W<K>::w(j)));
+        x = vmac(x, vadd(p, m), (RT)W<K>::w(j));
     }
     return vmul(x, (RT)0.5);
 }
@@ -127,7 +127,7 @@ __device__ __forceinline__ V restrict_point(const V* __restrict__ a, int o0, int
         const int o = 2 * j - 1;
         const V p = rst_yz<V, K>(a, c0 + o, c1, c2, t, ph);
         const V m = rst_yz<V, K>(a, c0 - o, c1, c2, t, ph);
-        x = vadd(x, vmul(vadd(p, m), (RT)W<K>::w(j)));
+        x = vmac(x, vadd(p, m), (RT)W<K>::w(j));
     }
     return vmul(x, (RT)0.5);
 }
