@@ -49,21 +49,6 @@ template <> __device__ __forceinline__ float vzero<float>() { return 0.0f; }
 template <> __device__ __forceinline__ double2 vzero<double2>() { return make_double2(0.0, 0.0); }
 template <> __device__ __forceinline__ float2 vzero<float2>() { return make_float2(0.0f, 0.0f); }
 
-// Streaming (non-temporal) stores: results written once and not re-read by
-// the same kernel bypass the L2 so it keeps the input rows neighbours re-read.
-__device__ __forceinline__ void vstore_nt(double* p, double v) { __builtin_nontemporal_store(v, p); }
-__device__ __forceinline__ void vstore_nt(float* p, float v) { __builtin_nontemporal_store(v, p); }
-__device__ __forceinline__ void vstore_nt(double2* p, double2 v)
-{
-    __builtin_nontemporal_store(v.x, &p->x);
-    __builtin_nontemporal_store(v.y, &p->y);
-}
-__device__ __forceinline__ void vstore_nt(float2* p, float2 v)
-{
-    __builtin_nontemporal_store(v.x, &p->x);
-    __builtin_nontemporal_store(v.y, &p->y);
-}
-
 // real part, and a value with the given real part (imaginary part 0)
 __device__ __forceinline__ double vreal(double v) { return v; }
 __device__ __forceinline__ float vreal(float v) { return v; }

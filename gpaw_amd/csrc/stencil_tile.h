@@ -1,8 +1,8 @@
 // LDS-tiled 2.5-D star-stencil kernel for large grids (gfx950).
 //
-// A 256-thread block owns a TY x TZ tile of the (i1, i2) plane and sweeps a
-// run of i0 planes.  Each thread owns ROWS consecutive i1 rows of one i2
-// column:
+// A block of 256 threads (512 or 1024 for complex data of range <= 3) owns
+// a TY x TZ tile of the (i1, i2) plane and sweeps a run of i0 planes.  Each
+// thread owns ROWS consecutive i1 rows of one i2 column:
 //   * axis-0 neighbours come from a register queue of 2R+1 planes per row
 //     (every input value is read from HBM once per sweep);
 //   * the current plane, with an R-wide halo in i1 and i2, is staged in LDS
@@ -11,8 +11,7 @@
 //     column, so only the 2R window ends are LDS reads;
 //   * the next plane's queue values and halo cells are loaded before the
 //     barrier, overlapping their latency with the current plane's work
-//     (a distance of two planes was measured slower: the extra registers
-//     cost a wave per SIMD).
+//     (distances of two and three planes were measured no faster).
 // Waves are TZ lanes wide along the contiguous axis, so every HBM access is
 // a full coalesced row segment.  Tiles are ordered so that one XCD (one L2)
 // processes a contiguous range of tiles: the halo rows a block re-reads
@@ -28,11 +27,11 @@ namespace gpaw {
 
 // threads per tile block: 256, or 1024 for complex applies of range <= 3
 // (taller tiles, fewer halo rows: measured 7% faster on complex128 bands,
-// slower on float64)
+// slower on float64); 512 and 1024 are tuner candidates for every complex
+// mode of range <= 3
 constexpr int TILE_NT = 256;
 constexpr int TILE_NT_WIDE = 1024;
-constexpr int TILE_NT_MID = 512;   // a tuner candidate for complex applies
-constexpr bool TILE_STREAM_STORE = false;
+constexpr int TILE_NT_MID = 512;
 // planes by which halo loads lead their use beyond the one-plane prefetch
 constexpr int tile_halo_lead(int R, int, int nt) { return nt > TILE_NT ? R : 0; }
 
@@ -66,15 +65,6 @@ struct TileGeom {
     static constexpr int NH = (NHALO + NT - 1) / NT;   // halo cells per thread
     static constexpr int LDS = 2 * LH * LW;            // two plane buffers
 };
-
-template <class V>
-__device__ __forceinline__ void tile_store(V* p, V v)
-{
-    if (TILE_STREAM_STORE)
-        vstore_nt(p, v);
-    else
-        *p = v;
-}
 
 // One tile of one band: columns z0 .. z0+TZ-1, rows y0 .. y0+TY-1, planes
 // i0s .. i0e-1.  a / o / sa point at the band's input, output and aux
@@ -216,7 +206,7 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
         if (WRITE && i0 > i0s) {
 #pragma unroll
             for (int j = 0; j < ROWS; j++)
-                if (inr[j]) tile_store(o + (i64)(i0 - 1) * plane + pref_off(qr[j]), rp[j]);
+                if (inr[j]) o[(i64)(i0 - 1) * plane + pref_off(qr[j])] = rp[j];
         }
         if (i0 + 1 < i0e) {
             i64 base;
@@ -276,7 +266,7 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
     if (WRITE && i0e > i0s) {
 #pragma unroll
         for (int j = 0; j < ROWS; j++)
-            if (inr[j]) tile_store(o + (i64)(i0e - 1) * plane + pref_off(qr[j]), rp[j]);
+            if (inr[j]) o[(i64)(i0e - 1) * plane + pref_off(qr[j])] = rp[j];
     }
     return acc;
 }
@@ -315,8 +305,7 @@ __global__ __launch_bounds__(NT) void k_star_tile(const V* __restrict__ in, V* _
 // occupancy gained outweighs the extra L2 reads of the taller halo: 1 row
 // measured 4.1 / 3.6 TB/s on 64^3 float64 / complex128 bands against
 // 2.7-3.3 / 3.0-3.2 TB/s for 2-4 rows.
-constexpr int TILE_ROWS_WIDE = 1;   // rows per thread in 1024-thread blocks
-constexpr int tile_rows(int, int, int nt = TILE_NT) { return nt == TILE_NT_WIDE ? TILE_ROWS_WIDE : 1; }
+constexpr int tile_rows(int, int, int = TILE_NT) { return 1; }
 
 // host-side shape selection and launch (stencil_tile_*.hip)
 int tile_tz(int n2);
