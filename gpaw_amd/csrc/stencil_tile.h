@@ -289,10 +289,10 @@ __global__ __launch_bounds__(NT) void k_star_tile(const V* __restrict__ in, V* _
     const int band = rest / sh.nchunk;
     const i64 boff = (i64)band * ng;
     const int i0s = ch * sh.q;
+    const V* xa = (aux && MODE != ST_HPSI) ? aux + boff : aux;
     const double acc = star_tile_sweep<V, R, TZ, ROWS, MODE, NORM, WRITE, NT>(
-        in + boff, out + boff, (aux && MODE != ST_HPSI) ? aux + boff : aux, g, ph, sc, w, tz * TZ,
-        ty * TG::TY,
-        i0s, min(i0s + sh.q, g.n[0]), lds);
+        in + boff, out + boff, xa, g, ph, sc, w, tz * TZ, ty * TG::TY, i0s,
+        min(i0s + sh.q, g.n[0]), lds);
     if (NORM) {
         __shared__ double red[NT];
         const double sum = block_sum<NT>(acc, red);

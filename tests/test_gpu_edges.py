@@ -96,3 +96,32 @@ def test_largest_band_count_and_beyond():
     big = t.zeros((65536,) + N, dtype=t.float64, device='cuda')
     with pytest.raises(GpawAmdError):
         lap.apply(big, t.empty_like(big))
+
+
+@pytest.mark.parametrize('N,nn,pbc,nb', [((45, 48, 40), 3, True, 64), ((33, 34, 35), 2, False, 40),
+                                        ((64, 64, 64), 3, True, 32)])
+def test_wide_complex_tiles(N, nn, pbc, nb):
+    """Complex bands on grids large enough for the 1024-thread tiles (two
+    planes per step; an odd plane count leaves a one-plane last step), plain
+    and with the local potential, against the oracle."""
+    print('case wide complex tiles', N, nn, pbc, nb, flush=True)
+    t = _torch()
+    from gpaw_amd.fd_operators import Laplace
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    cell = tuple(0.35 * n for n in N)
+    og, gd = O.Grid(N, cell, pbc), GridDescriptor(N, cell, pbc)
+    rng = np.random.default_rng(nb)
+    shape = (nb,) + tuple(og.n_c)
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    v = rng.standard_normal(tuple(og.n_c))
+    ph = og.phase_cd([0.3, -0.1, 0.2])
+    ref = np.zeros_like(a)
+    O.laplace(og, -0.5, nn, dtype=complex).apply(a, ref, ph)
+    lap = Laplace(gd, -0.5, nn, complex)
+    ad = dev(a)
+    got = t.empty_like(ad)
+    lap.apply(ad, got, ph)
+    assert np.array_equal(got.cpu().numpy(), ref)
+    ref += a * v
+    lap.operator.apply_potential(ad, got, dev(v), ph)
+    assert np.array_equal(got.cpu().numpy(), ref)
