@@ -162,7 +162,8 @@ def copy_bandwidth(torch):
 
 
 def fd_laplace_c3(torch, np):
-    """C3 Laplace: complex128 nn=3 on 256 bands of 64^3, periodic, phases."""
+    """C3 Laplace: complex128 nn=3 on 256 bands of 64^3, periodic, phases;
+    its K=6 transfers and FP32 variant; and the C1 real apply (8 x 48^3)."""
     from gpaw_amd.fd_operators import Laplace
     from gpaw_amd.grid_descriptor import GridDescriptor
     N, nb = 64, 256
@@ -209,6 +210,18 @@ def fd_laplace_c3(torch, np):
                    'pct_peak': round(100 * algo / 2 / t32 / 1e9 / PEAK_HBM_GBS, 1),
                    'rel_maxnorm_err_vs_fp64': err}
     del a, b, c, a32, b32
+    # C1: real float64 nn=3 Laplace on 8 bands of 48^3, periodic (14.2 MB
+    # per apply: cache resident, a latency / cache number)
+    N1, nb1 = 48, 8
+    gd1 = GridDescriptor((N1,) * 3, (N1 * 0.2 / BOHR,) * 3, True)
+    lap1 = Laplace(gd1, 1.0, 3)
+    a1 = torch.rand((nb1, N1, N1, N1), dtype=torch.float64, device='cuda', generator=g)
+    b1 = torch.empty_like(a1)
+    t1 = event_time(torch, lambda: lap1.apply(a1, b1), 50)
+    algo1 = 16.0 * nb1 * N1 ** 3
+    res['c1_real_8x48'] = {'avg_apply_us': round(t1 * 1e6, 2),
+                           'GBps': round(algo1 / t1 / 1e9, 1),
+                           'bytes_per_apply': algo1}
     return res
 
 

@@ -125,3 +125,21 @@ def test_wide_complex_tiles(N, nn, pbc, nb):
     ref += a * v
     lap.operator.apply_potential(ad, got, dev(v), ph)
     assert np.array_equal(got.cpu().numpy(), ref)
+
+
+def test_c1_configuration():
+    """C1: Laplace(nn=3), float64, 8 bands of 48^3, periodic, h = 0.2 A,
+    inputs U[-1, 1]: bit-identical to the oracle."""
+    print('case C1', flush=True)
+    t = _torch()
+    from gpaw_amd.fd_operators import Laplace
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    h = 0.2 / 0.52917720859
+    N = (48,) * 3
+    og, gd = O.Grid(N, (48 * h,) * 3, True), GridDescriptor(N, (48 * h,) * 3, True)
+    a = np.random.default_rng(1).uniform(-1, 1, (8,) + N)
+    ref = np.zeros_like(a)
+    O.laplace(og, 1.0, 3).apply(a, ref)
+    got = t.empty((8,) + N, dtype=t.float64, device='cuda')
+    Laplace(gd, 1.0, 3).apply(dev(a), got)
+    assert np.array_equal(got.cpu().numpy(), ref)
