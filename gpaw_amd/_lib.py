@@ -57,6 +57,7 @@ PROTOTYPES = {
     'gpaw_autotune': (_i, [_i]),
     'gpaw_autotune_report': (_l, [ctypes.c_char_p, _l]),
     'gpaw_sum': (_i, [_vp, _l, _d, _vp, _vp]),
+    'gpaw_dots': (_i, [_vp, _vp, _l, _l, _d, _vp, _vp]),
     'gpaw_axpby': (_i, [_vp, _vp, _d, _d, _l, _vp]),
     'gpaw_sub_scalar': (_i, [_vp, _vp, _d, _l, _vp]),
     'gpaw_cast': (_i, [_vp, _i, _vp, _i, _l, _vp]),

@@ -12,7 +12,9 @@ package):
 * restrict_potential -- vt_g += vHt_g, then restriction to the coarse grid,
   for every spin (gpaw/hamiltonian.py:254-259);
 * pair_potentials -- one Poisson solve per pair density, as the exact
-  exchange does (gpaw/xc/hybrid.py:140-152).
+  exchange does (gpaw/xc/hybrid.py:140-152);
+* rpa_coupling -- one Poisson solve per pair density, then the coupling
+  integrals of the RPA omega matrix (gpaw/lrtddft/omega_matrix.py:383-430).
 
 The spins (or bands) of one call go to the device as the band axis of ONE
 stencil / transfer launch.
@@ -102,3 +104,43 @@ def pair_potentials(poisson, rhot_pg, charges, eps=1e-12):
     if rho.host is not None:
         return vt.cpu().numpy(), niter
     return vt, niter
+
+
+def rpa_coupling(poisson, rhot_pg, gd=None, restrictor=None, rhot_pG=None):
+    """The Hartree part of the RPA omega matrix
+    (gpaw/lrtddft/omega_matrix.py:383-430), without the pair prefactors and
+    the atomic corrections: for every pair density ij, phit = 0 and
+    poisson.solve(phit, rhot_ij, charge=None); with a restrictor
+    (finegrid == 1) phit is restricted and integrated against the coarse
+    densities rhot_pG; K[ij, kq] = gd.integrate(rhot_kq * phit) for kq >= ij
+    (one launch of gpaw_dots per row) and K[kq, ij] = K[ij, kq].
+
+    gd: the grid the integrals run over (default: the solver's).  Returns K
+    as a numpy array and the iteration counts."""
+    t = _arrays.torch()
+    rho = Arg(rhot_pg, name='rhot_pg')
+    if gd is None:
+        gd = poisson.gd
+    if restrictor is None:
+        rint = rho.tensor
+    else:
+        if rhot_pG is None:
+            raise ValueError('rpa_coupling: a restrictor needs the coarse densities rhot_pG')
+        rint = Arg(rhot_pG, name='rhot_pG').tensor
+    P = rho.tensor.shape[0]
+    if rint.shape[0] != P:
+        raise ValueError('rpa_coupling: %d coarse densities for %d pairs' % (rint.shape[0], P))
+    K = t.zeros((P, P), dtype=t.float64, device='cuda')
+    phit_p = t.empty_like(rho.tensor[0])
+    phit = phit_p if restrictor is None else t.empty_like(rint[0])
+    niter = []
+    for ij in range(P):
+        phit_p.zero_()
+        niter.append(poisson.solve(phit_p, rho.tensor[ij], charge=None))
+        if restrictor is not None:
+            restrictor.apply(phit_p, phit)
+        K[ij, ij:] = ops.dots(phit, rint[ij:], gd.dv)
+    Kh = K.cpu().numpy()
+    iu = np.triu_indices(P, 1)
+    Kh[iu[1], iu[0]] = Kh[iu]
+    return Kh, niter

@@ -429,6 +429,22 @@ extern "C" int gpaw_sum(const double* a, long n, double scale, double* result, v
     return gpaw_check(e, "sum");
 }
 
+extern "C" int gpaw_dots(const double* a, const double* b, long nb, long n, double scale,
+                         double* result, void* stream)
+{
+    if (!a || !result || (!b && nb > 0) || n < 0 || nb < 0 || nb > 65535)
+        return gpaw_fail(GPAW_EINVAL, "dots: bad argument (0 <= nb <= 65535)");
+    if (nb == 0) return GPAW_OK;
+    double* partial = nullptr;
+    const size_t bytes = sizeof(double) * (size_t)nb * (size_t)dots_chunks(n);
+    hipError_t e = hipMallocAsync((void**)&partial, bytes, (hipStream_t)stream);
+    if (e != hipSuccess) return gpaw_check(e, "dots: partials");
+    e = launch_dots(a, b, (int)nb, n, scale, partial, result, (hipStream_t)stream);
+    hipError_t e2 = hipFreeAsync(partial, (hipStream_t)stream);
+    if (e == hipSuccess) e = e2;
+    return gpaw_check(e, "dots");
+}
+
 extern "C" int gpaw_axpby(double* y, const double* x, double a, double b, long n, void* stream)
 {
     if (!y || (!x && b != 0.0) || n < 0) return gpaw_fail(GPAW_EINVAL, "axpby: bad argument");

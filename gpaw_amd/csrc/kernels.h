@@ -137,6 +137,11 @@ hipError_t launch_reduce_partials(const double* partial, int n, double scale, do
                                   hipStream_t st);
 hipError_t launch_sum(const double* a, long long n, double* partial, int* npartial,
                       hipStream_t st);
+// result[j] = scale * sum_g a[g] * b[j n + g] for j < nb (nb <= 65535);
+// partial holds nb * dots_chunks(n) doubles.
+int dots_chunks(long long n);
+hipError_t launch_dots(const double* a, const double* b, int nb, long long n, double scale,
+                       double* partial, double* result, hipStream_t st);
 hipError_t launch_axpby(double* y, const double* x, double a, double b, long long n,
                         hipStream_t st);   // y = a*y + b*x
 hipError_t launch_sub_scalar(double* y, const double* x, double b, long long n,

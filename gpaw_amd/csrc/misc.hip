@@ -32,6 +32,34 @@ __global__ __launch_bounds__(256) void k_sum(const double* __restrict__ a, i64 n
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// Row j of a block of dot products: partial sums of a[g] * b[j n + g] over
+// chunk blockIdx.x of the grid (the product rounded on its own, then added,
+// as numpy's rhot * phit followed by a sum).
+__global__ __launch_bounds__(256) void k_dots(const double* __restrict__ a,
+                                              const double* __restrict__ b, i64 n, int nchunk,
+                                              double* __restrict__ partial)
+{
+    __shared__ double red[256];
+    const double* __restrict__ bj = b + (i64)blockIdx.y * n;
+    double s = 0.0;
+    const i64 stride = (i64)nchunk * 256;
+    for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) s += a[i] * bj[i];
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) partial[(i64)blockIdx.y * nchunk + blockIdx.x] = s;
+}
+
+// out[j] = scale * (sum of row j's partials, in index order)
+__global__ __launch_bounds__(256) void k_reduce_rows(const double* __restrict__ p, int nchunk,
+                                                     double scale, double* __restrict__ out)
+{
+    __shared__ double red[256];
+    const double* __restrict__ pj = p + (i64)blockIdx.x * nchunk;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nchunk; i += 256) s += pj[i];
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = s * scale;
+}
+
 __global__ __launch_bounds__(256) void k_axpby(double* __restrict__ y, const double* __restrict__ x,
                                                double a, double b, i64 n)
 {
@@ -121,6 +149,23 @@ hipError_t launch_sum(const double* a, long long n, double* partial, int* nparti
     if (nb < 1) nb = 1;
     *npartial = nb;
     launch_kernel(k_sum, dim3(nb), dim3(256), 0, st, a, (i64)n, partial);
+    return launch_error();
+}
+
+int dots_chunks(long long n)
+{
+    long long c = (n + 1023) / 1024;
+    return (int)(c < 1 ? 1 : c > 256 ? 256 : c);
+}
+
+hipError_t launch_dots(const double* a, const double* b, int nb, long long n, double scale,
+                       double* partial, double* result, hipStream_t st)
+{
+    if (nb <= 0) return hipSuccess;
+    const int nchunk = dots_chunks(n);
+    launch_kernel(k_dots, dim3(nchunk, nb), dim3(256), 0, st, a, b, (i64)n, nchunk, partial);
+    launch_kernel(k_reduce_rows, dim3(nb), dim3(256), 0, st, (const double*)partial, nchunk,
+                  scale, result);
     return launch_error();
 }
 

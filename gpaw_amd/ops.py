@@ -201,6 +201,26 @@ def integrate(gd, a_xg, b_xg=None):
     return vals.reshape(lead)
 
 
+def dots(a_g, b_xg, scale=1.0):
+    """scale * sum_g a_g * b_xg[j] for every leading index j of b_xg, as an
+    FP64 device tensor of b_xg's leading shape (one row of the pair-density
+    integrals gd.integrate(rhot * phit), gpaw/lrtddft/omega_matrix.py:427)."""
+    t = _arrays.torch()
+    a, b = Arg(a_g), Arg(b_xg)
+    if a.tensor.dtype != t.float64 or b.tensor.dtype != t.float64:
+        raise TypeError('dots needs float64 arrays')
+    n = a.tensor.numel()
+    lead = tuple(b.shape[:b.tensor.dim() - a.tensor.dim()])
+    if tuple(b.shape[len(lead):]) != tuple(a.shape):
+        raise ValueError('dots: trailing shape %s of b differs from %s'
+                         % (tuple(b.shape), tuple(a.shape)))
+    nb = int(np.prod(lead)) if lead else 1
+    out = t.empty(nb, dtype=t.float64, device='cuda')
+    _lib.check(_lib.lib().gpaw_dots(a.ptr, b.ptr, nb, n, float(scale), out.data_ptr(),
+                                    _arrays.stream()))
+    return out.reshape(lead)
+
+
 def sub_scalar(y, x, b):
     """y = x - b on float64 device tensors."""
     yy = Arg(y, write=True)

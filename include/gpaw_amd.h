@@ -234,6 +234,13 @@ long gpaw_autotune_report(char* buf, long len);
 int gpaw_sum(const double* a, long n, double scale, double* result,
              void* stream);
 
+/* result[j] = scale * sum_g a[g] * b[j*n + g] for j < nb (0 <= nb <= 65535;
+ * device pointers, FP64, deterministic order): one row of the pair-density
+ * coupling integrals gd.integrate(rhot * phit) of the RPA omega matrix
+ * (gpaw/lrtddft/omega_matrix.py:427, gpaw/grid_descriptor.py:197-208). */
+int gpaw_dots(const double* a, const double* b, long nb, long n, double scale,
+              double* result, void* stream);
+
 /* y = a*y + b*x over n doubles (numpy's in-place updates around the
  * solver, e.g. phi += q*phi_gauss, gpaw/poisson.py:180,186). */
 int gpaw_axpby(double* y, const double* x, double a, double b, long n,

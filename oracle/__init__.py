@@ -8,5 +8,5 @@ from .gpaw_oracle import (  # noqa: F401
     mehrstellen_b, Operator, Transformer, PoissonSolver, flatten_offsets,
     transformer_plan, gaussian_charges, madelung, bohr_per_angstrom,
     Preconditioner, laplace, apply_pseudo_hamiltonian, interpolate_density,
-    restrict_potential, pair_potentials,
+    restrict_potential, pair_potentials, rpa_coupling,
 )

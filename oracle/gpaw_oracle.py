@@ -18,7 +18,8 @@ points.  This module restates the Python layers that drive them:
 * preconditioner          gpaw/preconditioner.py:13-70
 * callers of the path     gpaw/wavefunctions/fd.py:50-52,
                           gpaw/hamiltonian.py:254-259,371-398,
-                          gpaw/density.py:217-234, gpaw/xc/hybrid.py:148-152
+                          gpaw/density.py:217-234, gpaw/xc/hybrid.py:148-152,
+                          gpaw/lrtddft/omega_matrix.py:383-430
 
 It is imported only by tests, the smoke check and the CPU-baseline leg of
 the benchmark.
@@ -723,3 +724,31 @@ def pair_potentials(poisson, rhot_pg, charges, eps=1e-12):
         niter.append(poisson.solve(vt_pg[p], -rhot_pg[p], charge=charges[p],
                                    eps=eps, zero_initial_phi=True))
     return vt_pg, niter
+
+
+def rpa_coupling(poisson, rhot_pg, gd=None, restrictor=None, rhot_pG=None):
+    """The Hartree part of the RPA omega matrix (gpaw/lrtddft/omega_matrix.py:
+    383-430), without the pair prefactors and the atomic corrections: for
+    every pair density ij, phit = 0, poisson.solve(phit, rhot_ij,
+    charge=None); with a restrictor (finegrid == 1) phit is restricted to the
+    coarse grid and integrated against the coarse densities rhot_pG; then
+    K[ij, kq] = gd.integrate(rhot_kq * phit) for kq >= ij and K[kq, ij] =
+    K[ij, kq].  Returns K and the iteration counts."""
+    P = rhot_pg.shape[0]
+    if gd is None:
+        gd = poisson.gd
+    rhot_int = rhot_pg if restrictor is None else rhot_pG
+    K = np.zeros((P, P))
+    niter = []
+    for ij in range(P):
+        phit_p = np.zeros(rhot_pg.shape[1:])
+        niter.append(poisson.solve(phit_p, rhot_pg[ij].copy(), charge=None))
+        if restrictor is None:
+            phit = phit_p
+        else:
+            phit = np.zeros(rhot_pG.shape[1:])
+            restrictor.apply(phit_p, phit)
+        for kq in range(ij, P):
+            K[ij, kq] = gd.integrate(rhot_int[kq] * phit)
+            K[kq, ij] = K[ij, kq]
+    return K, niter

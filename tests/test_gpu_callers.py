@@ -146,3 +146,69 @@ def test_pair_potentials():
     g = got.cpu().numpy()
     for p in range(3):
         assert np.abs(g[p] - want[p]).max() <= 1e-13 * np.abs(want[p]).max()
+
+
+def test_rpa_coupling_periodic_modes():
+    """Cos-mode pair densities on a periodic grid (a diagonal coupling
+    matrix in closed form): same matrix as the oracle to 1e-13 of its
+    largest entry, same iteration counts."""
+    print('case rpa coupling periodic', flush=True)
+    _torch()
+    from gpaw_amd.callers import rpa_coupling
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.poisson import PoissonSolver
+    N, h = 16, 0.3
+    og = O.Grid((N,) * 3, (N * h,) * 3, True)
+    gd = GridDescriptor((N,) * 3, (N * h,) * 3, True)
+    i0 = np.indices((N,) * 3)[0]
+    rho = np.stack([np.cos(2 * np.pi * m * i0 / N) for m in (1, 2, 3)])
+    ops_ = O.PoissonSolver(nn=3, relax='J', eps=1e-20)
+    ops_.set_grid_descriptor(og)
+    ops_.initialize()
+    want, n_want = O.rpa_coupling(ops_, rho)
+    ps = PoissonSolver(nn=3, relax='J', eps=1e-20)
+    ps.set_grid_descriptor(gd)
+    ps.initialize()
+    got, n_got = rpa_coupling(ps, dev(rho))
+    assert n_got == n_want
+    assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()
+    assert np.array_equal(got, got.T)
+
+
+def test_rpa_coupling_restricted_zero_boundary():
+    """finegrid == 1 on a zero-boundary cell: pair densities solved on the
+    fine grid (one exactly neutral, which takes the neutral branch of
+    charge=None, and two with a net charge, which take the Gaussian
+    branch), potentials restricted with K=6, integrated against coarse
+    densities."""
+    print('case rpa coupling restricted', flush=True)
+    _torch()
+    from gpaw_amd.callers import rpa_coupling
+    from gpaw_amd.poisson import PoissonSolver
+    from gpaw_amd.transformers import Transformer
+    gd, og = _gds(32, False)
+    cg, oc = gd.coarsen(), og.coarsen()
+    rng = np.random.default_rng(12)
+    lo, hi = 2.0 / BOHR, (32 * 0.2 - 2.0) / BOHR
+    cs = [rng.uniform(lo, hi, 3) for _ in range(4)]
+    sig = [0.7 / BOHR] * 2
+    rho = np.stack([O.gaussian_charges(og, cs[:2], sig, [1.0, -1.0]),
+                    O.gaussian_charges(og, cs[2:], sig, [1.0, -1.0]),
+                    O.gaussian_charges(og, cs[1:3], sig, [1.0, 0.5])])
+    rho_c = np.stack([O.gaussian_charges(oc, cs[:2], sig, [1.0, -1.0]),
+                      O.gaussian_charges(oc, cs[2:], sig, [1.0, -1.0]),
+                      O.gaussian_charges(oc, cs[1:3], sig, [1.0, 0.5])])
+    rho[1] -= rho[1].mean()
+    rho_c[1] -= rho_c[1].mean()
+    ops_ = O.PoissonSolver(nn=3, relax='J')
+    ops_.set_grid_descriptor(og)
+    ops_.initialize()
+    want, n_want = O.rpa_coupling(ops_, rho, gd=oc, restrictor=O.Transformer(og, oc, 3),
+                                  rhot_pG=rho_c)
+    ps = PoissonSolver(nn=3, relax='J')
+    ps.set_grid_descriptor(gd)
+    ps.initialize()
+    got, n_got = rpa_coupling(ps, dev(rho), gd=cg, restrictor=Transformer(gd, cg, 3),
+                              rhot_pG=dev(rho_c))
+    assert n_got == n_want
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()

@@ -117,3 +117,46 @@ def test_pair_potentials_are_minus_the_solutions():
         phi = g.zeros()
         ps.solve(phi, rho[p].copy(), charge=0.0, eps=1e-14, zero_initial_phi=True)
         assert np.abs(vt[p] + phi).max() <= 1e-12 * np.abs(phi).max()
+
+
+def _cos_modes(g, ms):
+    i0 = np.indices(tuple(g.n_c))[0]
+    return np.stack([np.cos(2 * np.pi * m * i0 / g.n_c[0]) for m in ms])
+
+
+def _mu(t, h):
+    """-(eigenvalue) of the nn=3 Laplacian for cos(t i0): the table
+    [-49/18, 3/2, -3/20, 1/90] of gpaw/fd_operators.py:286-292."""
+    c = [-49 / 18, 3 / 2, -3 / 20, 1 / 90]
+    return -(c[0] + 2 * sum(c[k] * np.cos(k * t) for k in (1, 2, 3))) / h ** 2
+
+
+def test_rpa_coupling_closed_form():
+    """Pair densities cos(2 pi m i0 / N) on a periodic grid are eigenmodes of
+    the solver's operator -1/(4 pi) L: phi = 4 pi / mu(t) rho, so the
+    coupling matrix is diagonal with K[m, m] = 4 pi / mu(t) * V / 2.  With
+    the K=6 restriction (finegrid == 1) of phi to the coarse grid and coarse
+    cos modes, the diagonal carries the restriction's transfer factor
+    0.5 (1 + 2 sum_j w_j cos(j t)), w = (0.5859375, -0.09765625, 0.01171875)."""
+    N, h, ms = 16, 0.3, (1, 2, 3)
+    g = _grid(N, True, h)
+    V = (N * h) ** 3
+    ps = O.PoissonSolver(nn=3, relax='J', eps=1e-24)
+    ps.set_grid_descriptor(g)
+    ps.initialize()
+    rho = _cos_modes(g, ms)
+    K, niter = O.rpa_coupling(ps, rho)
+    want = np.diag([4 * np.pi / _mu(2 * np.pi * m / N, h) * V / 2 for m in ms])
+    assert np.abs(K - want).max() <= 1e-12 * np.abs(want).max()
+    assert all(n > 0 for n in niter)
+    # finegrid == 1: solve on the fine grid, restrict, integrate on the coarse
+    c = g.coarsen()
+    R = O.Transformer(g, c, 3)
+    rho_c = _cos_modes(c, ms)
+    Kc, _ = O.rpa_coupling(ps, rho, gd=c, restrictor=R, rhot_pG=rho_c)
+    w = (0.5859375, -0.09765625, 0.01171875)
+    r = [0.5 * (1 + 2 * sum(wj * np.cos(j * 2 * np.pi * m / N) for wj, j in zip(w, (1, 3, 5))))
+         for m in ms]
+    want_c = np.diag([4 * np.pi / _mu(2 * np.pi * m / N, h) * r[i] * V / 2
+                      for i, m in enumerate(ms)])
+    assert np.abs(Kc - want_c).max() <= 1e-12 * np.abs(want_c).max()
