@@ -269,7 +269,10 @@ def callers_c3(torch, np):
     * potential restriction: 2 spins, vt_g += vHt_g (24 B per fine point)
       and nn=3 restriction to 64^3 (8 B per fine and per coarse point);
     * pair potentials: 8 pair-density solves on a 64^3 zero-boundary grid
-      (63^3 arrays), eps = 1e-12, half of them charged."""
+      (63^3 arrays), eps = 1e-12, half of them charged;
+    * RPA coupling (finegrid == 1): 12 pair densities solved on the 128^3
+      zero-boundary fine grid (127^3, eps = 2e-10, charge=None), potentials
+      restricted (nn=3) to 64^3, upper triangle of the coupling integrals."""
     from gpaw_amd import callers
     from gpaw_amd.fd_operators import Laplace
     from gpaw_amd.grid_descriptor import GridDescriptor
@@ -341,6 +344,31 @@ def callers_c3(torch, np):
                                         'eps=1e-12, 4 charged',
                               'solves_per_s': round(8 / dt, 2),
                               'iterations': niter}
+    del rho_d
+    # RPA coupling on the fine grid, integrals on the coarse one
+    fz = cg.refine()
+    rng = np.random.default_rng(128)
+    rho_f, rho_c = [], []
+    for p in range(12):
+        c, w, q = random_gaussian_charges(fz, rng, 2, 0.5 / BOHR, 1.0 / BOHR, 2.0 / BOHR)
+        rho_f.append(gaussian_density(fz, c, w, q))
+        rho_c.append(gaussian_density(cg, c, w, q))
+    rf = torch.from_numpy(np.stack(rho_f)).to('cuda')
+    rc = torch.from_numpy(np.stack(rho_c)).to('cuda')
+    ps = PoissonSolver(nn=3, relax='J')
+    ps.set_grid_descriptor(fz)
+    ps.initialize()
+    rst = Transformer(fz, cg, 3)
+    callers.rpa_coupling(ps, rf[:2], gd=cg, restrictor=rst, rhot_pG=rc[:2])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, niter = callers.rpa_coupling(ps, rf, gd=cg, restrictor=rst, rhot_pG=rc)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out['rpa_coupling'] = {'config': '12 pair densities, solved on 128^3 zero boundary (127^3), '
+                                     'restricted to 64^3, coupling integrals',
+                           'pairs_per_s': round(12 / dt, 2),
+                           'iterations': niter}
     return out
 
 
