@@ -51,6 +51,17 @@ hipError_t relax(gpaw_poisson* ps, int l, int k, double*& cur, double*& alt, hip
                    ? hipSuccess
                    : hipErrorLaunchFailure;
     }
+    if (l == 0 && ps->rot.total) {   // chunked solve: the cycle's input is kept
+        for (int it = 0; it < k; it++) {
+            double* out = ++ps->rot.done == ps->rot.total ? ps->rot.dst
+                                                          : ps->rot.t[(ps->rot.done - 1) & 1];
+            hipError_t e = launch_stencil(op->st, DT_F64, ST_JACOBI, cur, out, ps->rho[l], op->g,
+                                          ph, 1, ps->w[l], nullptr, nullptr, true, st);
+            if (e != hipSuccess) return e;
+            cur = out;
+        }
+        return hipSuccess;
+    }
     if (l > 0 && ps->fuse_jacobi && jacobi_fused_ok(op->st, op->g)) {
         for (; k >= JF_SWEEPS; k -= JF_SWEEPS) {
             hipError_t e = launch_jacobi_fused(op->st, DT_F64, cur, alt, ps->rho[l], op->g,
@@ -143,11 +154,41 @@ int capture(gpaw_poisson* ps)
     return GPAW_OK;
 }
 
-// k V-cycles from `state`; before cycle i, phi is copied to snap[i]; the
-// error after cycle i goes to d_errs[i]
+// k V-cycles; the error after cycle i goes to d_errs[i].
+// Jacobi: from phiA[0]; cycle i reads S_i and leaves its result in
+// S_{i+1} = snap[i] (S_0 = phiA[0]) without writing S_i, its other sweeps
+// going to phiB[0] and tmp2; then S_k is copied to phiA[0] for the next
+// chunk.  Phi after every cycle of the chunk thus stays available with one
+// copy per chunk.  Gauss-Seidel (in place): from `state`, phi copied to
+// snap[i] before cycle i.
+bool chunk_rotates(const gpaw_poisson* ps)
+{
+    // Jacobi with level 0 swept by single launches (not the fused coarse kernel)
+    return ps->method == 2 && ps->lc != 0;
+}
+
 hipError_t emit_chunk(gpaw_poisson* ps, int k, int state, hipStream_t st)
 {
     hipError_t e = hipSuccess;
+    if (chunk_rotates(ps)) {
+        const int n = (ps->L > 0 ? ps->pre[0] : 0) + ps->post[0];
+        for (int i = 0; i < k && e == hipSuccess; i++) {
+            double* cur = i ? ps->snap[i - 1] : ps->phiA[0];
+            double* alt = ps->phiB[0];
+            ps->rot.dst = ps->snap[i];
+            ps->rot.t[0] = ps->phiB[0];
+            ps->rot.t[1] = ps->tmp2;
+            ps->rot.done = 0;
+            ps->rot.total = n;
+            ps->err_dst = ps->d_errs + i;
+            e = emit_level(ps, 0, cur, alt, st);
+            ps->err_dst = nullptr;
+            ps->rot.total = 0;
+            if (e == hipSuccess && cur != ps->snap[i]) e = hipErrorInvalidValue;
+        }
+        if (e == hipSuccess) e = launch_copy(ps->phiA[0], ps->snap[k - 1], DT_F64, ps->npts[0], st);
+        return e;
+    }
     for (int i = 0; i < k && e == hipSuccess; i++) {
         e = launch_copy(ps->snap[i], state ? ps->phiB[0] : ps->phiA[0], DT_F64, ps->npts[0], st);
         ps->err_dst = ps->d_errs + i;
@@ -164,6 +205,7 @@ int ensure_chunks(gpaw_poisson* ps)
     int rc = GPAW_OK;
     ps->snap.assign(GPAW_CHUNK_MAX, nullptr);
     for (int i = 0; i < GPAW_CHUNK_MAX && !rc; i++) rc = alloc(&ps->snap[i], ps->npts[0]);
+    if (!rc && chunk_rotates(ps)) rc = alloc(&ps->tmp2, ps->npts[0]);
     if (!rc) rc = alloc(&ps->d_errs, GPAW_CHUNK_MAX);
     if (!rc)
         rc = gpaw_check(hipHostMalloc((void**)&ps->h_errs, sizeof(double) * GPAW_CHUNK_MAX),
@@ -173,6 +215,8 @@ int ensure_chunks(gpaw_poisson* ps)
         for (double* p : ps->snap)
             if (p) hipFree(p);
         ps->snap.clear();
+        if (ps->tmp2) hipFree(ps->tmp2);
+        ps->tmp2 = nullptr;
         if (ps->d_errs) hipFree(ps->d_errs);
         ps->d_errs = nullptr;
     }
@@ -439,12 +483,14 @@ extern "C" int gpaw_poisson_solve_neutral(gpaw_poisson* ps, double* phi, const d
     if (rc) return rc;
     // The reference runs `while iterate2() > eps and niter < maxiter:
     // niter += 1` (gpaw/poisson.py:203-209), one V-cycle and one norm per
-    // step.  Here V-cycles run in chunks of k with phi saved before each;
-    // the host reads the k norms once, finds the first cycle that meets the
-    // stopping rule and, if later cycles of the chunk ran, restores phi as
-    // it was after that cycle.  niter and phi are exactly the reference's.
-    // k follows the observed convergence rate so that a chunk usually ends
-    // at the stopping cycle.
+    // step.  Here V-cycles run in chunks of k with phi kept after each (see
+    // emit_chunk); the host reads the k norms once, finds the first cycle
+    // that meets the stopping rule and takes phi as it was after that
+    // cycle.  niter and phi are exactly the reference's.  k follows the
+    // observed convergence rate so that a chunk usually ends at the
+    // stopping cycle.
+    const bool jac = chunk_rotates(ps);
+    const double* result = nullptr;
     int state = 0;
     int n = 0;         // V-cycles accepted so far
     double err = 0.0;
@@ -469,6 +515,10 @@ extern "C" int gpaw_poisson_solve_neutral(gpaw_poisson* ps, double* phi, const d
                 break;
             }
         }
+        if (stop >= 0 && jac) {
+            result = ps->snap[stop];
+            break;
+        }
         if (stop >= 0) {
             const int after = state ^ (ps->swaps & ((stop + 1) & 1));
             if (stop + 1 < k)   // cycles past the stopping one ran: take phi back
@@ -478,7 +528,7 @@ extern "C" int gpaw_poisson_solve_neutral(gpaw_poisson* ps, double* phi, const d
             state = after;
             break;
         }
-        state ^= ps->swaps & (k & 1);
+        if (!jac) state ^= ps->swaps & (k & 1);
         // cycles still needed at the observed rate
         k = GPAW_CHUNK_MAX;
         if (rate > 0 && rate < 1 && err > eps) {
@@ -486,8 +536,8 @@ extern "C" int gpaw_poisson_solve_neutral(gpaw_poisson* ps, double* phi, const d
             if (m >= 1 && m < GPAW_CHUNK_MAX) k = (int)m;
         }
     }
-    rc = gpaw_check(launch_copy(phi, state ? ps->phiB[0] : ps->phiA[0], DT_F64, ps->npts[0], st),
-                    "poisson: store");
+    if (!result) result = state ? ps->phiB[0] : ps->phiA[0];
+    rc = gpaw_check(launch_copy(phi, result, DT_F64, ps->npts[0], st), "poisson: store");
     if (niter) *niter = n;
     if (error) *error = err;
     if (rc) return rc;
@@ -510,6 +560,7 @@ extern "C" void gpaw_poisson_free(gpaw_poisson* ps)
         if (g) hipGraphExecDestroy(g);
     for (double* p : ps->snap)
         if (p) hipFree(p);
+    if (ps->tmp2) hipFree(ps->tmp2);
     if (ps->d_errs) hipFree(ps->d_errs);
     if (ps->h_errs) hipHostFree(ps->h_errs);
     if (ps->d_coarse) hipFree(ps->d_coarse);

@@ -3,7 +3,7 @@
 #pragma once
 #include <vector>
 
-#define GPAW_CHUNK_MAX 8   // V-cycles per chunk
+#define GPAW_CHUNK_MAX 16   // V-cycles per chunk
 
 #include "capi_internal.h"
 #include "coarse.h"
@@ -24,9 +24,19 @@ struct gpaw_poisson {
     double* h_err = nullptr;
     double* err_dst = nullptr;  // where the level-0 norm of the next emitted cycle goes
     hipGraphExec_t exec[2] = {nullptr, nullptr};
-    // chunks of V-cycles (solve_neutral): snapshots of phi before each cycle
-    // of a chunk, one error slot per cycle, graphs by (length, start state)
+    // chunks of V-cycles (solve_neutral): phi after each cycle of a chunk
+    // (Jacobi: each cycle writes its result into snap[i] and never writes
+    // its input; Gauss-Seidel: phi copied to snap[i] before cycle i), one
+    // error slot per cycle, graphs by (length, start state)
     std::vector<double*> snap;
+    double* tmp2 = nullptr;    // second level-0 sweep buffer of the Jacobi chunks
+    // level-0 sweep buffers of the cycle being emitted (total = 0: phiA / phiB
+    // ping-pong): sweep j of `total` writes t[(j - 1) % 2], the last writes dst
+    struct {
+        double* dst = nullptr;
+        double* t[2] = {nullptr, nullptr};
+        int done = 0, total = 0;
+    } rot;
     double* d_errs = nullptr;
     double* h_errs = nullptr;
     std::vector<hipGraphExec_t> chunk;   // index 2 * (k - 1) + state

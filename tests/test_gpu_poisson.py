@@ -76,6 +76,28 @@ def test_solve_neutral_exact(N, pbc):
         assert np.array_equal(got, phi_ref)
 
 
+def test_every_stopping_cycle():
+    """One plan, a warm start and eps from 1e-3 down to 1e-21: the stopping
+    cycle falls at every position of the solver's chunks of V-cycles (first,
+    middle and last cycle of a chunk, and chunks that overshoot), and each
+    solve returns the reference's iteration count and potential."""
+    print('case every stopping cycle', flush=True)
+    og, gd, ops, gps = _pair(32, 0.2 / BOHR, False)
+    rho = _density(og, 3, 4, 0.5 / BOHR, 1.5 / BOHR, 2.0 / BOHR)
+    rho -= og.integrate(rho) / og.dv / og.size_of_global_array().prod()
+    start = 1e-3 * np.random.default_rng(0).standard_normal(tuple(og.n_c))
+    seen = set()
+    for eps in [10.0 ** -e for e in range(3, 22)]:
+        phi_ref = start.copy()
+        n_ref = ops.solve_neutral(phi_ref, rho.copy(), eps=eps)
+        phi = dev(start)
+        n = gps.solve_neutral(phi, dev(rho), eps=eps)
+        assert n == n_ref, eps
+        assert np.array_equal(phi.cpu().numpy(), phi_ref), eps
+        seen.add(n)
+    assert len(seen) >= 10
+
+
 def test_c2_config_solve():
     """96^3 non-periodic (95^3 arrays), h = 0.2 A, nn=3 Jacobi, eps=2e-10,
     eight seeded neutral Gaussians: the plain FP64 configuration."""
