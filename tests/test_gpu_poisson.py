@@ -98,6 +98,28 @@ def test_every_stopping_cycle():
     assert len(seen) >= 10
 
 
+@pytest.mark.parametrize('N,pbc', [(95, False), (63, False), (33, False)])
+def test_error_equals_separate_norm(N, pbc):
+    """The convergence error the V-cycle's last stencil pass sums itself
+    (its last block adds the per-block partials) equals, bit for bit, the
+    same norm taken by a separate residual pass and reduction launch.
+    (Zero-boundary cells: on periodic ones the solver removes the mean of
+    phi after the error was taken.)"""
+    print('case error vs separate norm', N, pbc, flush=True)
+    t = _torch()
+    h = 0.2 / BOHR
+    og, gd, ops, gps = _pair(N + (0 if pbc else 1), h, pbc)
+    rho = _density(og, 7, 4, 0.5 / BOHR, 1.5 / BOHR, 2.0 / BOHR)
+    rho -= og.integrate(rho) / og.dv / og.size_of_global_array().prod()
+    rho_d = dev(rho)
+    for eps in (1e-6, 1e-12):
+        phi = dev(np.zeros(tuple(og.n_c)))
+        gps.solve_neutral(phi, rho_d, eps=eps)
+        norm = t.zeros(1, dtype=t.float64, device='cuda')
+        gps.operators[0].operator.residual(phi, rho_d, None, norm, og.dv)
+        assert float(norm.cpu()[0]) == gps.error
+
+
 def test_c2_config_solve():
     """96^3 non-periodic (95^3 arrays), h = 0.2 A, nn=3 Jacobi, eps=2e-10,
     eight seeded neutral Gaussians: the plain FP64 configuration."""
