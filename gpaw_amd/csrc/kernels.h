@@ -135,6 +135,10 @@ hipError_t launch_transfer(const Transfer& t, int dtype, int mode, const void* i
 // Reductions and element-wise helpers (FP64 real unless stated).
 hipError_t launch_reduce_partials(const double* partial, int n, double scale, double* result,
                                   hipStream_t st);
+// result[i] = scale * sum_j partial[i * ncols + j], each row exactly as
+// launch_reduce_partials sums it
+hipError_t launch_reduce_rows(const double* partial, int nrows, int ncols, double scale,
+                              double* result, hipStream_t st);
 hipError_t launch_sum(const double* a, long long n, double* partial, int* npartial,
                       hipStream_t st);
 // result[j] = scale * sum_g a[g] * b[j n + g] for j < nb (nb <= 65535);

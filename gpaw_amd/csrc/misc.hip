@@ -48,7 +48,8 @@ __global__ __launch_bounds__(256) void k_dots(const double* __restrict__ a,
     if (threadIdx.x == 0) partial[(i64)blockIdx.y * nchunk + blockIdx.x] = s;
 }
 
-// out[j] = scale * (sum of row j's partials, in index order)
+// out[j] = scale * (sum of row j's partials, in index order); row by row
+// the same order and tree as k_reduce_partials
 __global__ __launch_bounds__(256) void k_reduce_rows(const double* __restrict__ p, int nchunk,
                                                      double scale, double* __restrict__ out)
 {
@@ -156,6 +157,14 @@ int dots_chunks(long long n)
 {
     long long c = (n + 1023) / 1024;
     return (int)(c < 1 ? 1 : c > 256 ? 256 : c);
+}
+
+hipError_t launch_reduce_rows(const double* partial, int nrows, int ncols, double scale,
+                              double* result, hipStream_t st)
+{
+    if (nrows <= 0) return hipSuccess;
+    launch_kernel(k_reduce_rows, dim3(nrows), dim3(256), 0, st, partial, ncols, scale, result);
+    return launch_error();
 }
 
 hipError_t launch_dots(const double* a, const double* b, int nb, long long n, double scale,

@@ -119,11 +119,15 @@ hipError_t emit_level(gpaw_poisson* ps, int l, double*& cur0, double*& alt0, hip
     if (e != hipSuccess) return e;
     if (l == 0) {
         int np = 0;
+        double* part = ps->defer_partial ? ps->defer_partial : op->partial;
         e = launch_stencil(op->st, DT_F64, ST_RESID, cur, nullptr, ps->rho[0], op->g, ph, 1, 0.0,
-                           op->partial, &np, false, st);
+                           part, &np, false, st);
         if (e != hipSuccess) return e;
-        e = launch_reduce_partials(op->partial, np, ps->dv, ps->err_dst ? ps->err_dst : ps->d_err,
-                                   st);
+        if (ps->defer_partial)   // summed with the chunk's other norms
+            e = np == ps->np0 ? hipSuccess : hipErrorInvalidValue;
+        else
+            e = launch_reduce_partials(op->partial, np, ps->dv,
+                                       ps->err_dst ? ps->err_dst : ps->d_err, st);
     }
     cur0 = cur;
     alt0 = alt;
@@ -180,12 +184,16 @@ hipError_t emit_chunk(gpaw_poisson* ps, int k, int state, hipStream_t st)
             ps->rot.t[1] = ps->tmp2;
             ps->rot.done = 0;
             ps->rot.total = n;
-            ps->err_dst = ps->d_errs + i;
+            ps->defer_partial = ps->d_chunk_partials + (long long)i * ps->np0;
             e = emit_level(ps, 0, cur, alt, st);
-            ps->err_dst = nullptr;
+            ps->defer_partial = nullptr;
             ps->rot.total = 0;
             if (e == hipSuccess && cur != ps->snap[i]) e = hipErrorInvalidValue;
         }
+        // the k norms: row i of the partials, in the order of one reduction
+        // launch per cycle
+        if (e == hipSuccess)
+            e = launch_reduce_rows(ps->d_chunk_partials, k, ps->np0, ps->dv, ps->d_errs, st);
         if (e == hipSuccess) e = launch_copy(ps->phiA[0], ps->snap[k - 1], DT_F64, ps->npts[0], st);
         return e;
     }
@@ -205,7 +213,11 @@ int ensure_chunks(gpaw_poisson* ps)
     int rc = GPAW_OK;
     ps->snap.assign(GPAW_CHUNK_MAX, nullptr);
     for (int i = 0; i < GPAW_CHUNK_MAX && !rc; i++) rc = alloc(&ps->snap[i], ps->npts[0]);
-    if (!rc && chunk_rotates(ps)) rc = alloc(&ps->tmp2, ps->npts[0]);
+    if (!rc && chunk_rotates(ps)) {
+        ps->np0 = stencil_partials(ps->ops[0]->st, ps->ops[0]->g, 1);
+        rc = alloc(&ps->tmp2, ps->npts[0]);
+        if (!rc) rc = alloc(&ps->d_chunk_partials, (long long)GPAW_CHUNK_MAX * ps->np0);
+    }
     if (!rc) rc = alloc(&ps->d_errs, GPAW_CHUNK_MAX);
     if (!rc)
         rc = gpaw_check(hipHostMalloc((void**)&ps->h_errs, sizeof(double) * GPAW_CHUNK_MAX),
@@ -217,6 +229,8 @@ int ensure_chunks(gpaw_poisson* ps)
         ps->snap.clear();
         if (ps->tmp2) hipFree(ps->tmp2);
         ps->tmp2 = nullptr;
+        if (ps->d_chunk_partials) hipFree(ps->d_chunk_partials);
+        ps->d_chunk_partials = nullptr;
         if (ps->d_errs) hipFree(ps->d_errs);
         ps->d_errs = nullptr;
     }
@@ -561,6 +575,7 @@ extern "C" void gpaw_poisson_free(gpaw_poisson* ps)
     for (double* p : ps->snap)
         if (p) hipFree(p);
     if (ps->tmp2) hipFree(ps->tmp2);
+    if (ps->d_chunk_partials) hipFree(ps->d_chunk_partials);
     if (ps->d_errs) hipFree(ps->d_errs);
     if (ps->h_errs) hipHostFree(ps->h_errs);
     if (ps->d_coarse) hipFree(ps->d_coarse);

@@ -23,6 +23,11 @@ struct gpaw_poisson {
     double* d_err = nullptr;
     double* h_err = nullptr;
     double* err_dst = nullptr;  // where the level-0 norm of the next emitted cycle goes
+    // when set: the level-0 norm partials of the next emitted cycle go here
+    // and are summed later (one launch per chunk); np0 = their count
+    double* defer_partial = nullptr;
+    double* d_chunk_partials = nullptr;   // GPAW_CHUNK_MAX x np0
+    int np0 = 0;
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     // chunks of V-cycles (solve_neutral): phi after each cycle of a chunk
     // (Jacobi: each cycle writes its result into snap[i] and never writes
