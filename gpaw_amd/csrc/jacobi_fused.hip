@@ -19,9 +19,18 @@
 // zero-boundary axis stay zero; periodic axes wrap (any number of periods:
 // the halo may exceed a small level's extent).  Real data only (the
 // Poisson hierarchy); phases are 1.
+//
+// Optionally the first launch of a level's post-smoothing also applies the
+// coarse-grid correction phi -= I e (K = 2 interpolation of the coarser
+// level's result, gpaw/poisson.py:266-267) to every cell it loads, with
+// interp2_point's arithmetic: the same values as the interpolate-and-
+// subtract launch it replaces.
 #include "kernels.h"
+#include "transfer_common.h"
 
 namespace gpaw {
+
+TrArgs make_trargs(const Transfer& tr);
 
 namespace {
 
@@ -46,7 +55,8 @@ constexpr int JF_NT = 1024;
 template <class V, int S>
 __global__ __launch_bounds__(JF_NT) void k_jacobi_fused(const V* __restrict__ in, V* __restrict__ out,
                                                         const V* __restrict__ rho, Geom g,
-                                                        StarCoefs7 c, double w, int nb1, int nb2)
+                                                        StarCoefs7 c, double w, int nb1, int nb2,
+                                                        const V* __restrict__ corr, TrArgs ct)
 {
     constexpr int E = JF_B + 2 * S;   // box edge with halo
     static_assert(E * E == 256, "one thread per (y, z) column of the box and plane run");
@@ -75,8 +85,12 @@ __global__ __launch_bounds__(JF_NT) void k_jacobi_fused(const V* __restrict__ in
         const int j0 = jf_wrap(o0 + x0 + p, n0, g.per[0], o);
         if (o) outside |= 1u << p;
         const long idx = (long)j0 * plane + col;
-        const V v = in[o ? 0 : idx];
+        V v = in[o ? 0 : idx];
         r[p] = rho[o ? 0 : idx];
+        if (corr && !o) {
+            const Phases one{};   // real data: phases unused
+            v = vsub(v, interp2_point<V>(corr, ct, j0, j1, j2, one));
+        }
         P[0][((x0 + p) * E + ty) * E + tz] = o ? V(0) : v;
     }
     __syncthreads();
@@ -124,9 +138,18 @@ bool jacobi_fused_ok(const Stencil& s, const Geom& g)
 }
 
 hipError_t launch_jacobi_fused(const Stencil& s, int dtype, const void* in, void* out,
-                               const void* rho, const Geom& g, double w, hipStream_t st)
+                               const void* rho, const Geom& g, double w, hipStream_t st,
+                               const void* corr, const Transfer* ctr)
 {
     if (!jacobi_fused_ok(s, g)) return hipErrorInvalidValue;
+    TrArgs ct{};
+    if (corr) {
+        // the correction is a K = 2 interpolation onto exactly this grid
+        if (!ctr || ctr->K != 2 || !ctr->interpolate) return hipErrorInvalidValue;
+        ct = make_trargs(*ctr);
+        for (int a = 0; a < 3; a++)
+            if (ct.nout[a] != g.n[a]) return hipErrorInvalidValue;
+    }
     StarCoefs7 c;
     for (int k = 0; k < 7; k++) c.c[k] = s.coefs[k];
     const int nb0 = (g.n[0] + JF_B - 1) / JF_B, nb1 = (g.n[1] + JF_B - 1) / JF_B,
@@ -134,10 +157,12 @@ hipError_t launch_jacobi_fused(const Stencil& s, int dtype, const void* in, void
     const dim3 grid((unsigned)(nb0 * nb1 * nb2));
     if (dtype == DT_F64)
         launch_kernel(k_jacobi_fused<double, JF_SWEEPS>, grid, dim3(JF_NT), 0, st,
-                      (const double*)in, (double*)out, (const double*)rho, g, c, w, nb1, nb2);
+                      (const double*)in, (double*)out, (const double*)rho, g, c, w, nb1, nb2,
+                      (const double*)corr, ct);
     else if (dtype == DT_F32)
         launch_kernel(k_jacobi_fused<float, JF_SWEEPS>, grid, dim3(JF_NT), 0, st,
-                      (const float*)in, (float*)out, (const float*)rho, g, c, w, nb1, nb2);
+                      (const float*)in, (float*)out, (const float*)rho, g, c, w, nb1, nb2,
+                      (const float*)corr, ct);
     else
         return hipErrorInvalidValue;
     return launch_error();

@@ -103,9 +103,14 @@ struct StarCoefs7 {
     double c[7];
 };
 #define JF_SWEEPS 4
+// With `corr` (the coarser level's result) and `ctr` (the K = 2
+// interpolator from it onto this grid), the loaded field is first corrected,
+// in -= I corr, as a separate interpolate-and-subtract launch would.
+struct Transfer;
 bool jacobi_fused_ok(const Stencil& s, const Geom& g);
 hipError_t launch_jacobi_fused(const Stencil& s, int dtype, const void* in, void* out,
-                               const void* rho, const Geom& g, double w, hipStream_t st);
+                               const void* rho, const Geom& g, double w, hipStream_t st,
+                               const void* corr = nullptr, const Transfer* ctr = nullptr);
 
 // Launch one stencil pass.  `partial` (optional) receives one FP64 partial
 // sum of |result|^2 per thread block; `write` = false skips the store.

@@ -36,11 +36,23 @@ int alloc(double** p, long long n)
     return gpaw_check(hipMalloc((void**)p, sizeof(double) * (n > 0 ? n : 1)), "poisson: level array");
 }
 
+// Whether level l's post-smoothing starts with a fused Jacobi launch that
+// can apply the coarse-grid correction itself.
+bool folds_correction(const gpaw_poisson* ps, int l)
+{
+    return ps->fold_correction && ps->method == 2 && l > 0 && l < ps->L && ps->fuse_jacobi &&
+           ps->post[l] >= JF_SWEEPS && jacobi_fused_ok(ps->ops[l]->st, ps->ops[l]->g) &&
+           ps->itp[l]->t.K == 2;
+}
+
 // relax k sweeps on level l, ping-ponging between phi[l] and alt; returns via
 // `cur` which buffer holds the result.  On the levels below 0, runs of
 // JF_SWEEPS Jacobi sweeps are one fused launch (the result then sits in the
 // other buffer than after as many single sweeps; callers follow `cur`).
-hipError_t relax(gpaw_poisson* ps, int l, int k, double*& cur, double*& alt, hipStream_t st)
+// `corr` (with folds_correction): the first fused launch first applies
+// phi -= I corr.
+hipError_t relax(gpaw_poisson* ps, int l, int k, double*& cur, double*& alt, hipStream_t st,
+                 const double* corr = nullptr)
 {
     Phases ph;
     gpaw_phases(nullptr, ph);
@@ -65,7 +77,8 @@ hipError_t relax(gpaw_poisson* ps, int l, int k, double*& cur, double*& alt, hip
     if (l > 0 && ps->fuse_jacobi && jacobi_fused_ok(op->st, op->g)) {
         for (; k >= JF_SWEEPS; k -= JF_SWEEPS) {
             hipError_t e = launch_jacobi_fused(op->st, DT_F64, cur, alt, ps->rho[l], op->g,
-                                               ps->w[l], st);
+                                               ps->w[l], st, corr, corr ? &ps->itp[l]->t : nullptr);
+            corr = nullptr;
             if (e != hipSuccess) return e;
             double* t = cur;
             cur = alt;
@@ -112,6 +125,13 @@ hipError_t emit_level(gpaw_poisson* ps, int l, double*& cur0, double*& alt0, hip
         double* calt = nullptr;
         e = emit_level(ps, l + 1, ccur, calt, st);
         if (e != hipSuccess) return e;
+        if (folds_correction(ps, l)) {   // phi_l -= I phi_{l+1} inside the first sweeps
+            e = relax(ps, l, ps->post[l], cur, alt, st, ccur);
+            if (e != hipSuccess) return e;
+            cur0 = cur;
+            alt0 = alt;
+            return e;
+        }
         e = launch_transfer(ps->itp[l]->t, DT_F64, TR_SUBTRACT, ccur, cur, nullptr, ph, 1, st);
         if (e != hipSuccess) return e;
     }
@@ -317,6 +337,8 @@ extern "C" int gpaw_poisson_new(int levels, gpaw_operator* const* ops,
     ps->use_graph = !(env && env[0] == '0');
     env = getenv("GPAW_AMD_FUSED_JACOBI");
     ps->fuse_jacobi = !(env && env[0] == '0');
+    env = getenv("GPAW_AMD_FOLD_CORRECTION");
+    ps->fold_correction = !(env && env[0] == '0');
     int rc = GPAW_OK;
     for (int l = 0; l <= levels && !rc; l++) {
         const Geom& g = ps->ops[l]->g;

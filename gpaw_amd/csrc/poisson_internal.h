@@ -53,6 +53,9 @@ struct gpaw_poisson {
     short* d_coarse_nb = nullptr;     // neighbour tables of the fused coarse levels
     int coarse_lds = 0;        // dynamic LDS bytes of the fused kernel
     bool fuse_jacobi = true;   // fused Jacobi sweeps on levels > 0 (GPAW_AMD_FUSED_JACOBI=0: off)
+    // coarse-grid correction applied by the first fused post-smoothing launch
+    // (GPAW_AMD_FOLD_CORRECTION=0: a separate interpolate-and-subtract launch)
+    bool fold_correction = true;
     gpaw::CoarsePlan h_coarse; // host copy of the coarse plan
     MixedState* mixed = nullptr;
 };

@@ -247,11 +247,14 @@ def test_reference_symmetry_case(relax):
     assert rel_maxnorm(b, ref) < 1e-13
 
 
-@pytest.mark.parametrize('N,pbc', [(40, True), (48, False), (64, True)])
+@pytest.mark.parametrize('N,pbc', [(40, True), (48, False), (64, True), (96, False)])
 def test_fused_jacobi_sweeps_identical(N, pbc, monkeypatch):
     """Levels below 0 run their Jacobi sweeps four at a time in one fused
-    launch; the potential and niter must be bit-identical to single sweeps
-    (GPAW_AMD_FUSED_JACOBI=0, read when the plan is built)."""
+    launch, whose first post-smoothing launch also applies the coarse-grid
+    correction; the potential and niter must be bit-identical to single
+    sweeps (GPAW_AMD_FUSED_JACOBI=0) and to fused sweeps after a separate
+    correction launch (GPAW_AMD_FOLD_CORRECTION=0), both read when the plan
+    is built."""
     print('case fused Jacobi', N, pbc, flush=True)
     torch = _torch()
     from gpaw_amd.grid_descriptor import GridDescriptor
@@ -261,8 +264,9 @@ def test_fused_jacobi_sweeps_identical(N, pbc, monkeypatch):
     rho = _density(og, N + 1, 4, 0.5 / BOHR, 1.5 / BOHR, 2.0 / BOHR)
     rho -= og.integrate(rho) / og.dv / og.size_of_global_array().prod()
     out = []
-    for fused in ('1', '0'):
+    for fused, fold in (('1', '1'), ('1', '0'), ('0', '1')):
         monkeypatch.setenv('GPAW_AMD_FUSED_JACOBI', fused)
+        monkeypatch.setenv('GPAW_AMD_FOLD_CORRECTION', fold)
         gd = GridDescriptor((N,) * 3, (N * h,) * 3, pbc)
         gps = PoissonSolver(nn=3, relax='J')
         gps.set_grid_descriptor(gd)
@@ -270,5 +274,6 @@ def test_fused_jacobi_sweeps_identical(N, pbc, monkeypatch):
         phi = dev(np.zeros(tuple(og.n_c)))
         n = gps.solve_neutral(phi, dev(rho), eps=2e-10)
         out.append((n, phi))
-    assert out[0][0] == out[1][0]
-    assert torch.equal(out[0][1], out[1][1])
+    for n, phi in out[1:]:
+        assert n == out[0][0]
+        assert torch.equal(phi, out[0][1])
