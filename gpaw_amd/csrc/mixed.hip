@@ -168,8 +168,10 @@ dim3 gridn(i64 n)
     return dim3((unsigned)(b < NB ? b : NB));
 }
 
+// `corr`: the first fused launch first applies phi -= I corr (the
+// coarse-grid correction, same values as the separate launch)
 hipError_t relax32(gpaw_poisson* ps, MixedState* m, int l, int k, float*& cur, float*& alt,
-                   hipStream_t st)
+                   hipStream_t st, const float* corr = nullptr)
 {
     Phases ph;
     gpaw_phases(nullptr, ph);
@@ -179,7 +181,9 @@ hipError_t relax32(gpaw_poisson* ps, MixedState* m, int l, int k, float*& cur, f
     if (l > 0 && ps->fuse_jacobi && jacobi_fused_ok(op->st, op->g)) {
         for (; k >= JF_SWEEPS; k -= JF_SWEEPS) {
             hipError_t e = launch_jacobi_fused(op->st, DT_F32, cur, alt, m->rho[l], op->g,
-                                               ps->w[l], st);
+                                               ps->w[l], st, corr,
+                                               corr ? &ps->itp[l]->t : nullptr);
+            corr = nullptr;
             if (e != hipSuccess) return e;
             float* t = cur;
             cur = alt;
@@ -227,6 +231,12 @@ hipError_t emit32(gpaw_poisson* ps, MixedState* m, int l, bool zero0, float*& re
         float* coarse = nullptr;
         e = emit32(ps, m, l + 1, false, coarse, st);
         if (e != hipSuccess) return e;
+        if (l > 0 && ps->fold_correction && ps->fuse_jacobi && m->post[l] >= JF_SWEEPS &&
+            jacobi_fused_ok(op->st, op->g) && ps->itp[l]->t.K == 2) {
+            e = relax32(ps, m, l, m->post[l], cur, alt, st, coarse);
+            res = cur;
+            return e;
+        }
         e = launch_transfer(ps->itp[l]->t, DT_F32, TR_SUBTRACT, coarse, cur, nullptr, ph, 1, st);
         if (e != hipSuccess) return e;
     }

@@ -102,14 +102,17 @@ def test_mixed_fewer_cycles_than_fp64():
 @pytest.mark.parametrize('N,pbc', [(64, False), (40, True)])
 def test_mixed_fused_sweeps_identical(N, pbc, monkeypatch):
     """The FP32 V-cycle's levels below 0 run four Jacobi sweeps per fused
-    launch; the mixed solution must be bit-identical to single sweeps
-    (GPAW_AMD_FUSED_JACOBI=0, read when the plan is built)."""
+    launch, the first post-smoothing launch with the coarse-grid correction;
+    the mixed solution must be bit-identical to single sweeps
+    (GPAW_AMD_FUSED_JACOBI=0) and to a separate correction launch
+    (GPAW_AMD_FOLD_CORRECTION=0), both read when the plan is built."""
     print('case mixed fused sweeps', N, pbc, flush=True)
     t = _torch()
     from gpaw_amd.poisson import MixedPoissonSolver
     out = []
-    for fused in ('1', '0'):
+    for fused, fold in (('1', '1'), ('1', '0'), ('0', '1')):
         monkeypatch.setenv('GPAW_AMD_FUSED_JACOBI', fused)
+        monkeypatch.setenv('GPAW_AMD_FOLD_CORRECTION', fold)
         gd, rho = _setup(N, 0.18 / BOHR, pbc, 6, N + 3)
         ms = MixedPoissonSolver(nn=3, rel_tol=1e-10, outer='pcg', inner=4)
         ms.set_grid_descriptor(gd)
@@ -117,5 +120,6 @@ def test_mixed_fused_sweeps_identical(N, pbc, monkeypatch):
         phi = t.zeros(tuple(gd.n_c), dtype=t.float64, device='cuda')
         n = ms.solve(phi, t.from_numpy(rho).to('cuda'))
         out.append((n, phi))
-    assert out[0][0] == out[1][0]
-    assert t.equal(out[0][1], out[1][1])
+    for n, phi in out[1:]:
+        assert n == out[0][0]
+        assert t.equal(phi, out[0][1])
