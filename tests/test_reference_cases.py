@@ -93,3 +93,119 @@ def test_bloch_gaussian_laplacian_device():
         return got
 
     _check(_errors(device_apply))
+
+
+# --- the reference's other test scripts, through the HIP operators --------
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip('no GPU')
+    return torch
+
+
+@pytest.mark.gpu
+def test_laplace_gradient_polynomial_exactness_device():
+    """gpaw/test/laplace.py through the device Laplace / Gradient: on the
+    eleven cells, every monomial of degree <= 2n has the exact Laplacian
+    (2 for x^2-type terms, else 0) and gradient at the central point, to
+    2e-12."""
+    print('case laplace.py', flush=True)
+    t = _torch()
+    from test_oracle_math import CELLS as POLY_CELLS
+    from gpaw_amd.fd_operators import Gradient, Laplace
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    for name, D, cell in POLY_CELLS:
+        for n in range(1, 4):
+            Nn = 2 * n + 2
+            g = O.Grid((Nn,) * 3, cell)
+            gd = GridDescriptor((Nn,) * 3, cell)
+            r_gv = g.grid_point_coordinates().transpose((1, 2, 3, 0)) - g.cell_cv.sum(0) / 2
+            lap = Laplace(gd, 1.0, n)
+            grads = [Gradient(gd, v, 1.0, n) for v in range(3)]
+            b = t.empty((Nn,) * 3, dtype=t.float64, device='cuda')
+            for m in range(0, 2 * n + 1):
+                for ix in range(m + 1):
+                    for iy in range(m - ix + 1):
+                        iz = m - ix - iy
+                        a = t.from_numpy(np.ascontiguousarray(
+                            (r_gv ** (ix, iy, iz)).prod(3))).to('cuda')
+                        r = 2.0 if (m == 2 and max(ix, iy, iz) == 2) else 0.0
+                        lap.apply(a, b)
+                        assert abs(float(b[n + 1, n + 1, n + 1]) - r) < 2e-12, (name, n, m)
+                        for v in range(3):
+                            grads[v].apply(a, b)
+                            r = 1.0 if (m == 1 and (ix, iy, iz)[v] == 1) else 0.0
+                            assert abs(float(b[n + 1, n + 1, n + 1]) - r) < 2e-12, (name, n, v)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('dtype', [float, complex])
+def test_transformations_device(dtype):
+    """gpaw/test/transformations.py through the device Transformer: K = 2,
+    4, 6, 8 interpolation and restriction of random data on a periodic 20^3
+    grid keep the integral to 1e-14."""
+    print('case transformations.py', dtype, flush=True)
+    t = _torch()
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.transformers import Transformer
+    n = 20
+    gd = GridDescriptor((n, n, n))
+    rng = np.random.RandomState(8)
+    a = rng.random_sample((n, n, n)).astype(dtype)
+    if dtype == complex:
+        a.imag = rng.random_sample((n, n, n))
+    phase = np.ones((3, 2), complex)
+    for g2 in (gd.refine(), gd.coarsen()):
+        for k in (2, 4, 6, 8):
+            b = t.empty(tuple(g2.n_c), dtype=t.float64 if dtype == float else t.complex128,
+                        device='cuda')
+            Transformer(gd, g2, k // 2, dtype).apply(t.from_numpy(a).to('cuda'), b, phase)
+            ia = gd.dv * a.sum()
+            ib = g2.dv * b.cpu().numpy().sum()
+            assert abs(ia - ib) < 1e-14, (k, g2.N_c)
+
+
+@pytest.mark.gpu
+def test_non_periodic_device():
+    """gpaw/test/non_periodic.py through the device Transformer: zero-boundary
+    interpolation and restriction keep the integral to 1e-10."""
+    print('case non_periodic.py', flush=True)
+    t = _torch()
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.transformers import Transformer
+    n = 20
+    g1 = GridDescriptor((n, n, n), (8.0, 8.0, 8.0), pbc_c=False)
+    a1 = np.random.RandomState(8).random_sample(tuple(g1.n_c))
+    g2 = g1.refine()
+    a2 = t.empty(tuple(g2.n_c), dtype=t.float64, device='cuda')
+    Transformer(g1, g2).apply(t.from_numpy(a1).to('cuda'), a2)
+    h2 = a2.cpu().numpy()
+    assert abs(g1.dv * a1.sum() - g2.dv * h2.sum()) < 1e-10
+    for sl in (0, -1):
+        h2[sl] = 0.0
+        h2[:, sl] = 0.0
+        h2[:, :, sl] = 0.0
+    back = t.empty(tuple(g1.n_c), dtype=t.float64, device='cuda')
+    Transformer(g2, g1).apply(t.from_numpy(h2).to('cuda'), back)
+    assert abs(g1.dv * back.cpu().numpy().sum() - g2.dv * h2.sum()) < 1e-10
+
+
+@pytest.mark.gpu
+def test_gauss_func_device():
+    """gpaw/test/gauss_func.py (L = 0) through the device PoissonSolver: the
+    potential of the generalized Gaussian on a 48^3 zero-boundary cell is
+    within 0.6 (L2 norm) of the analytic erf potential."""
+    print('case gauss_func.py', flush=True)
+    t = _torch()
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.poisson import PoissonSolver
+    g = O.Grid((48, 48, 48), (20.0, 20.0, 20.0), 0)
+    gd = GridDescriptor((48, 48, 48), (20.0, 20.0, 20.0), False)
+    ng, vg = O.gpaw_oracle.gaussian_monopole(g)
+    ps = PoissonSolver(nn=3)
+    ps.set_grid_descriptor(gd)
+    ps.initialize()
+    pot = t.zeros(tuple(gd.n_c), dtype=t.float64, device='cuda')
+    ps.solve(pot, t.from_numpy(ng).to('cuda'), charge=None, zero_initial_phi=True)
+    p = pot.cpu().numpy()
+    assert (g.dv * ((p - vg) ** 2).sum()) ** 0.5 < 0.6
