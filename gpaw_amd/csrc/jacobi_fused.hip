@@ -23,8 +23,12 @@
 // Optionally the first launch of a level's post-smoothing also applies the
 // coarse-grid correction phi -= I e (K = 2 interpolation of the coarser
 // level's result, gpaw/poisson.py:266-267) to every cell it loads, with
-// interp2_point's arithmetic: the same values as the interpolate-and-
-// subtract launch it replaces.
+// interp2_combine's arithmetic: the same values as the interpolate-and-
+// subtract launch it replaces.  The block first stages the coarse values
+// its cells interpolate from (at most 10 per axis) in LDS.  Cells are
+// interpolated at their unwrapped coordinates: on a periodic axis a shift
+// by the fine period shifts the coarse index by the coarse period, so the
+// value equals the one at the wrapped point.
 #include "kernels.h"
 #include "transfer_common.h"
 
@@ -51,6 +55,14 @@ __device__ __forceinline__ int jf_wrap(int j, int n, int per, bool& out)
 // planes split into four runs of PX planes (four waves per SIMD hide the
 // LDS and division latencies of the sweeps).
 constexpr int JF_NT = 1024;
+constexpr int JF_CB = 10;   // staged coarse values per axis (16 fine cells span <= 9, +1)
+
+// index j of a coarse axis of extent n for the staging: wrapped (any number
+// of periods) or flagged outside a zero-boundary axis
+__device__ __forceinline__ int jf_cwrap(int j, int n, int per, bool& out)
+{
+    return jf_wrap(j, n, per, out);
+}
 
 template <class V, int S>
 __global__ __launch_bounds__(JF_NT) void k_jacobi_fused(const V* __restrict__ in, V* __restrict__ out,
@@ -62,6 +74,7 @@ __global__ __launch_bounds__(JF_NT) void k_jacobi_fused(const V* __restrict__ in
     static_assert(E * E == 256, "one thread per (y, z) column of the box and plane run");
     constexpr int PX = E * E * E / JF_NT;   // planes per thread
     __shared__ V P[2][E * E * E];
+    __shared__ V C[JF_CB * JF_CB * JF_CB];
     const int b = blockIdx.x;
     const int bz = b % nb2, by = (b / nb2) % nb1, bx = b / (nb2 * nb1);
     const int o0 = bx * JF_B - S, o1 = by * JF_B - S, o2 = bz * JF_B - S;
@@ -75,6 +88,23 @@ __global__ __launch_bounds__(JF_NT) void k_jacobi_fused(const V* __restrict__ in
     const long plane = (long)n1 * n2;
     typedef typename VT<V>::R RT;
     const RT wr = (RT)w;
+    // the coarse values of the correction: unwrapped coarse indices from cm
+    int cm0 = 0, cm1 = 0, cm2 = 0;
+    if (corr) {
+        static_assert(E == 16, "the staging covers 16 fine cells per axis");
+        cm0 = ((o0 + ct.skip[0][0]) >> 1) - ct.lo[0];
+        cm1 = ((o1 + ct.skip[1][0]) >> 1) - ct.lo[1];
+        cm2 = ((o2 + ct.skip[2][0]) >> 1) - ct.lo[2];
+        for (int i = threadIdx.x; i < JF_CB * JF_CB * JF_CB; i += JF_NT) {
+            bool o = false;
+            const int c0 = jf_cwrap(cm0 + i / (JF_CB * JF_CB), ct.gin.n[0], ct.gin.per[0], o);
+            const int c1 = jf_cwrap(cm1 + (i / JF_CB) % JF_CB, ct.gin.n[1], ct.gin.per[1], o);
+            const int c2 = jf_cwrap(cm2 + i % JF_CB, ct.gin.n[2], ct.gin.per[2], o);
+            const V v = corr[o ? 0 : ((long)c0 * ct.gin.n[1] + c1) * ct.gin.n[2] + c2];
+            C[i] = o ? V(0) : v;
+        }
+        __syncthreads();
+    }
     // load the box (zero outside a zero-boundary axis) and rho of this
     // thread's cells
     V r[PX];
@@ -88,8 +118,15 @@ __global__ __launch_bounds__(JF_NT) void k_jacobi_fused(const V* __restrict__ in
         V v = in[o ? 0 : idx];
         r[p] = rho[o ? 0 : idx];
         if (corr && !o) {
-            const Phases one{};   // real data: phases unused
-            v = vsub(v, interp2_point<V>(corr, ct, j0, j1, j2, one));
+            const int q0 = o0 + x0 + p + ct.skip[0][0], q1 = o1 + ty + ct.skip[1][0],
+                      q2 = o2 + tz + ct.skip[2][0];
+            const int l0 = (q0 >> 1) - ct.lo[0] - cm0, l1 = (q1 >> 1) - ct.lo[1] - cm1,
+                      l2 = (q2 >> 1) - ct.lo[2] - cm2;
+            v = vsub(v, interp2_combine<V>(
+                            [&](int d0, int d1, int d2) {
+                                return C[((l0 + d0) * JF_CB + l1 + d1) * JF_CB + l2 + d2];
+                            },
+                            q0 & 1, q1 & 1, q2 & 1));
         }
         P[0][((x0 + p) * E + ty) * E + tz] = o ? V(0) : v;
     }

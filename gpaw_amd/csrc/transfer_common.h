@@ -249,19 +249,14 @@ __device__ __forceinline__ void interp2_cell(const V* __restrict__ a, V* __restr
     }
 }
 
-// The K = 2 interpolation at one fine output point (o0, o1, o2): the value
-// interp2_cell gives that point (same coarse loads, z, y, x pass order and
-// rounding), for kernels that apply the coarse-grid correction while
-// loading the fine field.
-template <class V>
-__device__ __forceinline__ V interp2_point(const V* __restrict__ a, const TrArgs& t, int o0,
-                                           int o1, int o2, const Phases& ph)
+// The K = 2 interpolation at a fine point of parities (px, py, pz) from its
+// coarse corner values f(d0, d1, d2), d in {0, 1}: the value interp2_cell
+// gives that point (z, y, x passes, same rounding).
+template <class V, class F>
+__device__ __forceinline__ V interp2_combine(F&& f, bool px, bool py, bool pz)
 {
     typedef typename VT<V>::R RT;
     const RT h = (RT)0.5;
-    const int p0 = o0 + t.skip[0][0], p1 = o1 + t.skip[1][0], p2 = o2 + t.skip[2][0];
-    const int c0 = (p0 >> 1) - t.lo[0], c1 = (p1 >> 1) - t.lo[1], c2 = (p2 >> 1) - t.lo[2];
-    const bool px = p0 & 1, py = p1 & 1, pz = p2 & 1;
     V y[2];
 #pragma unroll
     for (int d0 = 0; d0 < 2; d0++) {
@@ -270,9 +265,8 @@ __device__ __forceinline__ V interp2_point(const V* __restrict__ a, const TrArgs
 #pragma unroll
         for (int d1 = 0; d1 < 2; d1++) {
             if (d1 && !py) break;
-            const V v0 = ghost_fetch(a, c0 + d0, c1 + d1, c2, t.gin, ph);
-            z[d1] = pz ? vmul(vadd(v0, ghost_fetch(a, c0 + d0, c1 + d1, c2 + 1, t.gin, ph)), h)
-                       : v0;
+            const V v0 = f(d0, d1, 0);
+            z[d1] = pz ? vmul(vadd(v0, f(d0, d1, 1)), h) : v0;
         }
         y[d0] = py ? vmul(vadd(z[0], z[1]), h) : z[0];
     }
