@@ -50,17 +50,21 @@ def _l(a):
 class Kernels:
     """ctypes front end to the oracle kernels.
 
-    kind='port'       our C restatement (liboracle.so)
-    kind='reference'  the reference's own C, built in place (_ref/libgpawref.so)
+    kind='port'          our C restatement (liboracle.so)
+    kind='reference'     the reference's own C, built in place (_ref/libgpawref.so)
+    kind='reference_mt'  the same with its GPAW_OMP_MONLY threads
+                         (_ref/libgpawref_mt.so; OMP_NUM_THREADS threads per
+                         bmgs_fd / restrict / interpolate call)
     """
 
+    _FILES = {'port': ('liboracle.so', 'orc_'),
+              'reference': (os.path.join('_ref', 'libgpawref.so'), 'ref_'),
+              'reference_mt': (os.path.join('_ref', 'libgpawref_mt.so'), 'ref_')}
+
     def __init__(self, kind='port'):
-        if kind == 'port':
-            path, prefix = os.path.join(_HERE, 'liboracle.so'), 'orc_'
-        elif kind == 'reference':
-            path, prefix = os.path.join(_HERE, '_ref', 'libgpawref.so'), 'ref_'
-        else:
+        if kind not in self._FILES:
             raise ValueError(kind)
+        path, prefix = os.path.join(_HERE, self._FILES[kind][0]), self._FILES[kind][1]
         if not os.path.exists(path):
             raise FileNotFoundError(
                 '%s not built (run make -C oracle)' % path)
@@ -82,8 +86,7 @@ class Kernels:
 
     @staticmethod
     def available(kind):
-        name = 'liboracle.so' if kind == 'port' else '_ref/libgpawref.so'
-        return os.path.exists(os.path.join(_HERE, name))
+        return os.path.exists(os.path.join(_HERE, Kernels._FILES[kind][0]))
 
     def operator_apply(self, coefs, offsets, size, rng, nb, real, cfd,
                        a, b, nin, phases):

@@ -84,3 +84,36 @@ def test_transformer_random(seed):
                 Transformer(gin, gout, nn, dtype, kernels=kind).apply(a, b, ph)
                 outs.append(b)
             assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.skipif(not O.Kernels.available('reference_mt'),
+                    reason='threaded reference build absent')
+def test_threaded_reference_is_identical(monkeypatch):
+    """The reference's GPAW_OMP_MONLY build (bmgs_fd / restrict /
+    interpolate over OMP_NUM_THREADS pthreads, the all-cores CPU baseline)
+    gives the serial build's bits: stencil applies, K = 2..8 transfers and a
+    Poisson solve with its iteration count."""
+    monkeypatch.setenv('OMP_NUM_THREADS', '4')
+    rng = np.random.default_rng(31)
+    g = O.Grid((20, 18, 22), (7.0, 6.5, 8.0), (True, False, True))
+    a = rng.standard_normal((2,) + tuple(g.n_c))
+    out = {}
+    for kind in ('reference', 'reference_mt'):
+        b = np.zeros_like(a)
+        O.laplace(g, 1.0, 3, kernels=kind).apply(a, b)
+        c = g.coarsen()
+        r = np.zeros((2,) + tuple(c.n_c))
+        O.Transformer(g, c, 3, kernels=kind).apply(a, r)
+        f = g.refine()
+        i = np.zeros((2,) + tuple(f.n_c))
+        O.Transformer(g, f, 2, kernels=kind).apply(a, i)
+        pg = O.Grid((32,) * 3, (12.0,) * 3, False)
+        ps = O.PoissonSolver(nn=3, relax='J', kernels=kind)
+        ps.set_grid_descriptor(pg)
+        ps.initialize()
+        rho = O.gaussian_charges(pg, [(5.0, 6.0, 6.5), (7.0, 6.0, 5.5)], [1.0, 1.1], [1.0, -1.0])
+        phi = pg.zeros()
+        n = ps.solve(phi, rho, eps=1e-14)
+        out[kind] = (b, r, i, phi, n)
+    for x, y in zip(out['reference'], out['reference_mt']):
+        assert np.array_equal(x, y)
