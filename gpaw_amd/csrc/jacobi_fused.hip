@@ -68,7 +68,8 @@ template <class V, int S>
 __global__ __launch_bounds__(JF_NT) void k_jacobi_fused(const V* __restrict__ in, V* __restrict__ out,
                                                         const V* __restrict__ rho, Geom g,
                                                         StarCoefs7 c, double w, int nb1, int nb2,
-                                                        const V* __restrict__ corr, TrArgs ct)
+                                                        const V* __restrict__ corr, TrArgs ct,
+                                                        int zero_in)
 {
     constexpr int E = JF_B + 2 * S;   // box edge with halo
     static_assert(E * E == 256, "one thread per (y, z) column of the box and plane run");
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(JF_NT) void k_jacobi_fused(const V* __restrict__ in
         const int j0 = jf_wrap(o0 + x0 + p, n0, g.per[0], o);
         if (o) outside |= 1u << p;
         const long idx = (long)j0 * plane + col;
-        V v = in[o ? 0 : idx];
+        V v = zero_in ? V(0) : in[o ? 0 : idx];
         r[p] = rho[o ? 0 : idx];
         if (corr && !o) {
             const int q0 = o0 + x0 + p + ct.skip[0][0], q1 = o1 + ty + ct.skip[1][0],
@@ -176,7 +177,7 @@ bool jacobi_fused_ok(const Stencil& s, const Geom& g)
 
 hipError_t launch_jacobi_fused(const Stencil& s, int dtype, const void* in, void* out,
                                const void* rho, const Geom& g, double w, hipStream_t st,
-                               const void* corr, const Transfer* ctr)
+                               const void* corr, const Transfer* ctr, bool zero_in)
 {
     if (!jacobi_fused_ok(s, g)) return hipErrorInvalidValue;
     TrArgs ct{};
@@ -195,11 +196,11 @@ hipError_t launch_jacobi_fused(const Stencil& s, int dtype, const void* in, void
     if (dtype == DT_F64)
         launch_kernel(k_jacobi_fused<double, JF_SWEEPS>, grid, dim3(JF_NT), 0, st,
                       (const double*)in, (double*)out, (const double*)rho, g, c, w, nb1, nb2,
-                      (const double*)corr, ct);
+                      (const double*)corr, ct, (int)zero_in);
     else if (dtype == DT_F32)
         launch_kernel(k_jacobi_fused<float, JF_SWEEPS>, grid, dim3(JF_NT), 0, st,
                       (const float*)in, (float*)out, (const float*)rho, g, c, w, nb1, nb2,
-                      (const float*)corr, ct);
+                      (const float*)corr, ct, (int)zero_in);
     else
         return hipErrorInvalidValue;
     return launch_error();

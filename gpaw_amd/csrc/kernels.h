@@ -106,11 +106,14 @@ struct StarCoefs7 {
 // With `corr` (the coarser level's result) and `ctr` (the K = 2
 // interpolator from it onto this grid), the loaded field is first corrected,
 // in -= I corr, as a separate interpolate-and-subtract launch would.
+// `zero_in`: the caller guarantees in == 0 everywhere (a level's first
+// sweeps after the restriction zeroed it), so it is not read.
 struct Transfer;
 bool jacobi_fused_ok(const Stencil& s, const Geom& g);
 hipError_t launch_jacobi_fused(const Stencil& s, int dtype, const void* in, void* out,
                                const void* rho, const Geom& g, double w, hipStream_t st,
-                               const void* corr = nullptr, const Transfer* ctr = nullptr);
+                               const void* corr = nullptr, const Transfer* ctr = nullptr,
+                               bool zero_in = false);
 
 // Launch one stencil pass.  `partial` (optional) receives one FP64 partial
 // sum of |result|^2 per thread block; `write` = false skips the store.

@@ -171,7 +171,7 @@ dim3 gridn(i64 n)
 // `corr`: the first fused launch first applies phi -= I corr (the
 // coarse-grid correction, same values as the separate launch)
 hipError_t relax32(gpaw_poisson* ps, MixedState* m, int l, int k, float*& cur, float*& alt,
-                   hipStream_t st, const float* corr = nullptr)
+                   hipStream_t st, const float* corr = nullptr, bool zero_in = false)
 {
     Phases ph;
     gpaw_phases(nullptr, ph);
@@ -182,8 +182,9 @@ hipError_t relax32(gpaw_poisson* ps, MixedState* m, int l, int k, float*& cur, f
         for (; k >= JF_SWEEPS; k -= JF_SWEEPS) {
             hipError_t e = launch_jacobi_fused(op->st, DT_F32, cur, alt, m->rho[l], op->g,
                                                ps->w[l], st, corr,
-                                               corr ? &ps->itp[l]->t : nullptr);
+                                               corr ? &ps->itp[l]->t : nullptr, zero_in);
             corr = nullptr;
+            zero_in = false;
             if (e != hipSuccess) return e;
             float* t = cur;
             cur = alt;
@@ -220,7 +221,8 @@ hipError_t emit32(gpaw_poisson* ps, MixedState* m, int l, bool zero0, float*& re
     if (l == 0 && zero0) e = launch_fill(cur, DT_F32, ps->npts[0], st);
     if (e != hipSuccess) return e;
     if (l < m->L) {
-        e = relax32(ps, m, l, m->pre[l], cur, alt, st);
+        // below level 0, phi_l was zeroed by the restriction onto it
+        e = relax32(ps, m, l, m->pre[l], cur, alt, st, nullptr, l > 0);
         if (e != hipSuccess) return e;
         e = launch_stencil(op->st, DT_F32, ST_RESID, cur, m->res[l], m->rho[l], op->g, ph, 1, 0.0,
                            nullptr, nullptr, true, st);

@@ -50,9 +50,10 @@ bool folds_correction(const gpaw_poisson* ps, int l)
 // JF_SWEEPS Jacobi sweeps are one fused launch (the result then sits in the
 // other buffer than after as many single sweeps; callers follow `cur`).
 // `corr` (with folds_correction): the first fused launch first applies
-// phi -= I corr.
+// phi -= I corr.  `zero_in`: phi is zero on entry (the first fused launch
+// does not read it).
 hipError_t relax(gpaw_poisson* ps, int l, int k, double*& cur, double*& alt, hipStream_t st,
-                 const double* corr = nullptr)
+                 const double* corr = nullptr, bool zero_in = false)
 {
     Phases ph;
     gpaw_phases(nullptr, ph);
@@ -77,8 +78,10 @@ hipError_t relax(gpaw_poisson* ps, int l, int k, double*& cur, double*& alt, hip
     if (l > 0 && ps->fuse_jacobi && jacobi_fused_ok(op->st, op->g)) {
         for (; k >= JF_SWEEPS; k -= JF_SWEEPS) {
             hipError_t e = launch_jacobi_fused(op->st, DT_F64, cur, alt, ps->rho[l], op->g,
-                                               ps->w[l], st, corr, corr ? &ps->itp[l]->t : nullptr);
+                                               ps->w[l], st, corr, corr ? &ps->itp[l]->t : nullptr,
+                                               zero_in);
             corr = nullptr;
+            zero_in = false;
             if (e != hipSuccess) return e;
             double* t = cur;
             cur = alt;
@@ -111,7 +114,8 @@ hipError_t emit_level(gpaw_poisson* ps, int l, double*& cur0, double*& alt0, hip
     hipError_t e = hipSuccess;
     gpaw_operator* op = ps->ops[l];
     if (l < ps->L) {
-        e = relax(ps, l, ps->pre[l], cur, alt, st);
+        // below level 0, phi_l was zeroed by the restriction onto it
+        e = relax(ps, l, ps->pre[l], cur, alt, st, nullptr, l > 0);
         if (e != hipSuccess) return e;
         e = launch_stencil(op->st, DT_F64, ST_RESID, cur, ps->res[l], ps->rho[l], op->g, ph, 1,
                            0.0, nullptr, nullptr, true, st);
