@@ -238,6 +238,8 @@ class PoissonSolver:
 
     def estimate_memory(self, mem):
         gdbytes = self.gd.bytecount()
+        mem.subnode('V-cycle chunk buffers (phi after each of up to 16 cycles '
+                    'and a sweep buffer)', 17 * gdbytes)
         nbytes = 0
         for level in range(self.levels + 1):
             nbytes += 4 * gdbytes        # phi pair, rho, residual
