@@ -461,11 +461,12 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
     }
     if (e != hipSuccess) return gpaw_check(e, "mixed: load");
     double rr = 0.0, dd = 0.0;
-    rc = read_scalar(m, S_DZOLD, st, &rr);
-    if (rc) return rc;
-    const double target = rel_tol * rel_tol * rr;
+    // |rho|^2 and the first |d|^2 come back in one readback
     rc = residual64(ps, m, st);
     if (!rc) rc = read_scalar(m, S_DD, st, &dd);
+    if (rc) return rc;
+    rr = m->h_S[S_DZOLD];
+    const double target = rel_tol * rel_tol * rr;
     int it = 0;
     if (!rc && mode == 1 && dd > target) {
         // z = V32(d); dz = d.z; p = z
