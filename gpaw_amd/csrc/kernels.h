@@ -94,6 +94,10 @@ long autotune_report(char* buf, long len);
 
 // Work estimate used to size partial-sum buffers.
 int stencil_partials(const Stencil& s, const Geom& g, int nin);
+// Whether an ST_APPLY pass of this stencil and grid can also produce the
+// partial sums of in . out (real data on the tile kernel); launch_stencil
+// refuses ST_APPLY with partials otherwise.
+bool stencil_dot_ok(const Stencil& s, int dtype, const Geom& g, int nin);
 
 // S = JF_SWEEPS weighted-Jacobi sweeps of a 7-point star stencil in one
 // launch (temporal blocking, jacobi_fused.hip): real data, in != out;

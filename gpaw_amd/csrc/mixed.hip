@@ -330,12 +330,21 @@ hipError_t emit_a(gpaw_poisson* ps, MixedState* m, hipStream_t st)
     gpaw_operator* op = ps->ops[0];
     Phases ph;
     gpaw_phases(nullptr, ph);
-    hipError_t e = launch_stencil(op->st, DT_F64, ST_APPLY, m->p, m->q, nullptr, op->g, ph, 1,
-                                  0.0, nullptr, nullptr, true, st);
-    if (e != hipSuccess) return e;
-    launch_kernel(k_dot, gridn(n), dim3(256), 0, st, m->p, m->q, n, m->partial);
-    launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x, m->S,
-                  (int)S_PQ);
+    hipError_t e;
+    if (stencil_dot_ok(op->st, DT_F64, op->g, 1)) {   // q = L p with the partials of p.q
+        int np = 0;
+        e = launch_stencil(op->st, DT_F64, ST_APPLY, m->p, m->q, nullptr, op->g, ph, 1, 0.0,
+                           op->partial, &np, true, st);
+        if (e != hipSuccess) return e;
+        launch_kernel(k_red1, dim3(1), dim3(256), 0, st, op->partial, np, m->S, (int)S_PQ);
+    } else {
+        e = launch_stencil(op->st, DT_F64, ST_APPLY, m->p, m->q, nullptr, op->g, ph, 1, 0.0,
+                           nullptr, nullptr, true, st);
+        if (e != hipSuccess) return e;
+        launch_kernel(k_dot, gridn(n), dim3(256), 0, st, m->p, m->q, n, m->partial);
+        launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x, m->S,
+                      (int)S_PQ);
+    }
     launch_kernel(k_pcg_update, gridn(n), dim3(256), 0, st, ps->phiA[0], ps->res[0], m->p, m->q,
                   m->rho[0], m->S, n, m->partial);
     launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x, m->S,

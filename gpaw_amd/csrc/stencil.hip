@@ -632,7 +632,17 @@ hipError_t launch_stencil(const Stencil& s, int dtype, int mode, const void* in,
             }
         }
     }
+    // in . out partials of an apply come from the tile kernel only
+    if (partial && mode == ST_APPLY && c.kind != K_TILE) return hipErrorInvalidValue;
     return launch_cfg(c, s, dtype, mode, in, out, aux, g, ph, nin, w, partial, write, st);
+}
+
+bool stencil_dot_ok(const Stencil& s, int dtype, const Geom& g, int nin)
+{
+    int TZ;
+    TileShape sh;
+    return (dtype == DT_F64 || dtype == DT_F32) &&
+           stencil_kind(s, g, dtype_size(dtype), nin, TZ, sh) == K_TILE;
 }
 
 }  // namespace gpaw

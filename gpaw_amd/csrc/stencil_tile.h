@@ -47,6 +47,11 @@ struct TileShape {
 
 template <class V>
 __device__ __forceinline__ double tnorm2(V v);
+// in . result for real data (norm passes of ST_APPLY)
+__device__ __forceinline__ double tdot(double a, double b) { return a * b; }
+__device__ __forceinline__ double tdot(float a, float b) { return (double)a * (double)b; }
+template <class V>
+__device__ __forceinline__ double tdot(V, V) { return 0.0; }
 template <> __device__ __forceinline__ double tnorm2<double>(double v) { return v * v; }
 template <> __device__ __forceinline__ double tnorm2<float>(float v) { return (double)v * (double)v; }
 template <> __device__ __forceinline__ double tnorm2<double2>(double2 v) { return v.x * v.x + v.y * v.y; }
@@ -256,7 +261,9 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
             for (int k = 1; k <= R; k++) x = vmac(x, q[j][R - k], (RT)sc.c[5 * R + k]);
             rp[j] = st_epilogue<MODE>(x, q[j][R], MODE == ST_APPLY ? vzero<V>() : s[j], (RT)w,
                                       (RT)sc.c[0]);
-            if (NORM && inr[j]) acc += tnorm2(rp[j]);
+            // NORM: |result|^2, or for ST_APPLY in . result (a CG step's p.Lp)
+            if (NORM && inr[j])
+                acc += MODE == ST_APPLY ? tdot(q[j][R], rp[j]) : tnorm2(rp[j]);
         }
 #pragma unroll
         for (int j = 0; j < ROWS; j++)

@@ -58,11 +58,11 @@ hipError_t tile_modes(const Stencil& s, int mode, const void* in, void* out, con
                       const Geom& g, const Phases& ph, double w, const TileShape& sh,
                       double* partial, bool write, hipStream_t st)
 {
-    // residual norms are only taken in the Poisson loop (RESID); other modes
-    // never carry partials there
-    if (partial && mode != ST_RESID) return hipErrorInvalidValue;
+    // partials: residual norms (RESID, the Poisson loop) or in . out (APPLY,
+    // the mixed solver's CG step); other modes never carry them
+    if (partial && mode != ST_RESID && mode != ST_APPLY) return hipErrorInvalidValue;
     switch (mode) {
-    case ST_APPLY: return tile_mode<V, R, TZ, ST_APPLY>(s, in, out, aux, g, ph, w, sh, nullptr, true, st);
+    case ST_APPLY: return tile_mode<V, R, TZ, ST_APPLY>(s, in, out, aux, g, ph, w, sh, partial, true, st);
     case ST_RESID: return tile_mode<V, R, TZ, ST_RESID>(s, in, out, aux, g, ph, w, sh, partial, write, st);
     case ST_JACOBI: return tile_mode<V, R, TZ, ST_JACOBI>(s, in, out, aux, g, ph, w, sh, nullptr, true, st);
     case ST_PCUPD: return tile_mode<V, R, TZ, ST_PCUPD>(s, in, out, aux, g, ph, w, sh, nullptr, true, st);
