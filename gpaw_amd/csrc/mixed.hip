@@ -102,15 +102,26 @@ __global__ __launch_bounds__(256) void k_red2(const double* __restrict__ p, int 
     }
 }
 
-// alpha = dz / pq ; phi -= alpha p ; d -= alpha q ; d32 = d ; partial |d|^2
+// alpha = dz / pq ; phi -= alpha p ; d -= alpha q ; d32 = d ; partial |d|^2.
+// With npq > 0, pq is summed by every block from the npq partials at pqp,
+// in k_red1's order (the same value in every block); else it is S[PQ].
 __global__ __launch_bounds__(256) void k_pcg_update(double* __restrict__ phi, double* __restrict__ d,
                                                     const double* __restrict__ p,
                                                     const double* __restrict__ q,
                                                     float* __restrict__ d32, double* __restrict__ S,
-                                                    i64 n, double* __restrict__ partial)
+                                                    i64 n, double* __restrict__ partial,
+                                                    const double* __restrict__ pqp, int npq)
 {
     __shared__ double red[256];
-    const double alpha = S[S_DZ] / S[S_PQ];
+    double pq;
+    if (npq > 0) {
+        double t = 0.0;
+        for (int i = threadIdx.x; i < npq; i += 256) t += pqp[i];
+        pq = block_sum<256>(t, red);
+    } else {
+        pq = S[S_PQ];
+    }
+    const double alpha = S[S_DZ] / pq;
     double s = 0.0;
     for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256) {
         phi[i] = phi[i] - alpha * p[i];
@@ -331,12 +342,15 @@ hipError_t emit_a(gpaw_poisson* ps, MixedState* m, hipStream_t st)
     Phases ph;
     gpaw_phases(nullptr, ph);
     hipError_t e;
+    const double* pqp = nullptr;
+    int npq = 0;
     if (stencil_dot_ok(op->st, DT_F64, op->g, 1)) {   // q = L p with the partials of p.q
         int np = 0;
         e = launch_stencil(op->st, DT_F64, ST_APPLY, m->p, m->q, nullptr, op->g, ph, 1, 0.0,
                            op->partial, &np, true, st);
         if (e != hipSuccess) return e;
-        launch_kernel(k_red1, dim3(1), dim3(256), 0, st, op->partial, np, m->S, (int)S_PQ);
+        pqp = op->partial;   // summed inside the update
+        npq = np;
     } else {
         e = launch_stencil(op->st, DT_F64, ST_APPLY, m->p, m->q, nullptr, op->g, ph, 1, 0.0,
                            nullptr, nullptr, true, st);
@@ -346,7 +360,7 @@ hipError_t emit_a(gpaw_poisson* ps, MixedState* m, hipStream_t st)
                       (int)S_PQ);
     }
     launch_kernel(k_pcg_update, gridn(n), dim3(256), 0, st, ps->phiA[0], ps->res[0], m->p, m->q,
-                  m->rho[0], m->S, n, m->partial);
+                  m->rho[0], m->S, n, m->partial, pqp, npq);
     launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x, m->S,
                   (int)S_DD);
     return launch_error();
