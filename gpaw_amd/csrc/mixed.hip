@@ -334,8 +334,10 @@ void destroy_graphs(MixedState* m)
     }
 }
 
-// PCG step part a: q = L p, alpha, phi += alpha p, d -= alpha q, |d|^2
-hipError_t emit_a(gpaw_poisson* ps, MixedState* m, hipStream_t st)
+// PCG step part a: q = L p, alpha, phi += alpha p, d -= alpha q, |d|^2.
+// |d|^2 (S[DD]) is only read by the host after a launch, so only the last
+// step of a launch sums its partials (`last`).
+hipError_t emit_a(gpaw_poisson* ps, MixedState* m, hipStream_t st, bool last = true)
 {
     const i64 n = ps->npts[0];
     gpaw_operator* op = ps->ops[0];
@@ -361,8 +363,9 @@ hipError_t emit_a(gpaw_poisson* ps, MixedState* m, hipStream_t st)
     }
     launch_kernel(k_pcg_update, gridn(n), dim3(256), 0, st, ps->phiA[0], ps->res[0], m->p, m->q,
                   m->rho[0], m->S, n, m->partial, pqp, npq);
-    launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x, m->S,
-                  (int)S_DD);
+    if (last)
+        launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x, m->S,
+                      (int)S_DD);
     return launch_error();
 }
 
@@ -404,7 +407,7 @@ int run_ba(gpaw_poisson* ps, MixedState* m, int k, hipStream_t st)
             hipError_t e = hipSuccess;
             for (int i = 0; i < k && e == hipSuccess; i++) {
                 e = emit_b(ps, m, s);
-                if (e == hipSuccess) e = emit_a(ps, m, s);
+                if (e == hipSuccess) e = emit_a(ps, m, s, i == k - 1);
             }
             return e;
         });
