@@ -165,6 +165,17 @@ __global__ __launch_bounds__(256) void k_p_update(double* __restrict__ p, const 
         p[i] = first ? (double)z[i] : (double)z[i] + beta * p[i];
 }
 
+// The first Jacobi sweep from phi = 0: every stencil term is zero, so the
+// sweep is element-wise: the tile kernel's epilogue with a zero sum and a
+// zero centre value (same operations, same bits).
+__global__ __launch_bounds__(256) void k_jacobi_from_zero(float* __restrict__ out,
+                                                          const float* __restrict__ rho, i64 n,
+                                                          double w, double c0)
+{
+    for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256)
+        out[i] = st_epilogue<ST_JACOBI>(0.0f, 0.0f, rho[i], (float)w, (float)c0);
+}
+
 // phi -= e (FP64 accumulation of an FP32 correction)
 __global__ __launch_bounds__(256) void k_sub_f32(double* __restrict__ phi, const float* __restrict__ e,
                                                  i64 n)
@@ -229,11 +240,21 @@ hipError_t emit32(gpaw_poisson* ps, MixedState* m, int l, bool zero0, float*& re
     float* cur = m->phiA[l];
     float* alt = m->phiB[l];
     hipError_t e = hipSuccess;
-    if (l == 0 && zero0) e = launch_fill(cur, DT_F32, ps->npts[0], st);
+    int pre = l < m->L ? m->pre[l] : 0;
+    if (l == 0 && zero0 && pre > 0 && ps->fuse_jacobi) {
+        // the first sweep from zero, element-wise, into alt (as a sweep would)
+        launch_kernel(k_jacobi_from_zero, gridn(ps->npts[0]), dim3(256), 0, st, alt, m->rho[0],
+                      (i64)ps->npts[0], ps->w[0], op->st.coefs[0]);
+        e = launch_error();
+        std::swap(cur, alt);
+        pre--;
+    } else if (l == 0 && zero0) {
+        e = launch_fill(cur, DT_F32, ps->npts[0], st);
+    }
     if (e != hipSuccess) return e;
     if (l < m->L) {
         // below level 0, phi_l was zeroed by the restriction onto it
-        e = relax32(ps, m, l, m->pre[l], cur, alt, st, nullptr, l > 0);
+        e = relax32(ps, m, l, pre, cur, alt, st, nullptr, l > 0);
         if (e != hipSuccess) return e;
         e = launch_stencil(op->st, DT_F32, ST_RESID, cur, m->res[l], m->rho[l], op->g, ph, 1, 0.0,
                            nullptr, nullptr, true, st);
