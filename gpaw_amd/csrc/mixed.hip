@@ -54,11 +54,14 @@ struct MixedState {
 #define MIXED_CHUNK_MAX 8
 
 // device scalars
-enum { S_DZ = 0, S_PQ = 1, S_DD = 2, S_ZQ = 3, S_DZOLD = 4, S_ALPHA = 5, S_N = 8 };
+enum { S_DZ = 0, S_PQ = 1, S_DD = 2, S_RR = 4, S_ALPHA = 5, S_N = 8 };
 
 namespace {
 
 constexpr int NB = 1024;   // blocks of the reduction kernels
+// regions of the partials buffer: d.z and z.q (k_dot_dz_zq, [0, 2 NB)),
+// |d|^2 of the update, p.q of the separate dot pass
+constexpr int PART_DD = 2 * NB, PART_PQ = 3 * NB;
 
 __global__ __launch_bounds__(256) void k_dot(const double* __restrict__ x,
                                              const double* __restrict__ y, i64 n,
@@ -83,45 +86,31 @@ __global__ __launch_bounds__(256) void k_red1(const double* __restrict__ p, int 
     if (threadIdx.x == 0) S[slot] = s;
 }
 
-// S[DZOLD] = S[DZ]; S[DZ] = sum p[0..n); S[ZQ] = sum p[n..2n)
-__global__ __launch_bounds__(256) void k_red2(const double* __restrict__ p, int n,
-                                              double* __restrict__ S)
+// Sum of n partials in k_red1's order (every block of a launch gets the
+// same value).
+__device__ __forceinline__ double sum_partials(const double* __restrict__ p, int n, double* red)
 {
-    __shared__ double red[256];
-    double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) {
-        a += p[i];
-        b += p[n + i];
-    }
-    a = block_sum<256>(a, red);
-    b = block_sum<256>(b, red);
-    if (threadIdx.x == 0) {
-        S[S_DZOLD] = S[S_DZ];
-        S[S_DZ] = a;
-        S[S_ZQ] = b;
-    }
+    double t = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) t += p[i];
+    return block_sum<256>(t, red);
 }
 
-// alpha = dz / pq ; phi -= alpha p ; d -= alpha q ; d32 = d ; partial |d|^2.
-// With npq > 0, pq is summed by every block from the npq partials at pqp,
-// in k_red1's order (the same value in every block); else it is S[PQ].
+// alpha = dz / pq ; phi -= alpha p ; d -= alpha q ; d32 = d ; partials of
+// |d|^2 into `partial`.  dz (the ndz partials at dzp) and pq (the npq
+// partials at pqp, or S[PQ] when npq = 0) are summed by every block itself;
+// block 0 keeps dz in S[DZ] (the next step's old dz) and alpha in S[ALPHA].
 __global__ __launch_bounds__(256) void k_pcg_update(double* __restrict__ phi, double* __restrict__ d,
                                                     const double* __restrict__ p,
                                                     const double* __restrict__ q,
                                                     float* __restrict__ d32, double* __restrict__ S,
                                                     i64 n, double* __restrict__ partial,
-                                                    const double* __restrict__ pqp, int npq)
+                                                    const double* __restrict__ pqp, int npq,
+                                                    const double* __restrict__ dzp, int ndz)
 {
     __shared__ double red[256];
-    double pq;
-    if (npq > 0) {
-        double t = 0.0;
-        for (int i = threadIdx.x; i < npq; i += 256) t += pqp[i];
-        pq = block_sum<256>(t, red);
-    } else {
-        pq = S[S_PQ];
-    }
-    const double alpha = S[S_DZ] / pq;
+    const double pq = npq > 0 ? sum_partials(pqp, npq, red) : S[S_PQ];
+    const double dz = sum_partials(dzp, ndz, red);
+    const double alpha = dz / pq;
     double s = 0.0;
     for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256) {
         phi[i] = phi[i] - alpha * p[i];
@@ -132,7 +121,10 @@ __global__ __launch_bounds__(256) void k_pcg_update(double* __restrict__ phi, do
     }
     s = block_sum<256>(s, red);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
-    if (blockIdx.x == 0 && threadIdx.x == 0) S[S_ALPHA] = alpha;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S[S_ALPHA] = alpha;
+        S[S_DZ] = dz;
+    }
 }
 
 // partials of d.z and z.q (z from the FP32 preconditioner)
@@ -156,11 +148,16 @@ __global__ __launch_bounds__(256) void k_dot_dz_zq(const double* __restrict__ d,
     }
 }
 
-// p = z + beta p, beta = -alpha (z.q) / (d_old.z_old) (Polak-Ribiere); first: p = z
+// p = z + beta p, beta = -alpha (z.q) / (d_old.z_old) (Polak-Ribiere), z.q
+// summed by every block from the nzq partials at zqp, d_old.z_old = S[DZ]
+// (kept by the previous update); first: p = z
 __global__ __launch_bounds__(256) void k_p_update(double* __restrict__ p, const float* __restrict__ z,
-                                                  const double* __restrict__ S, i64 n, int first)
+                                                  const double* __restrict__ S, i64 n, int first,
+                                                  const double* __restrict__ zqp, int nzq)
 {
-    const double beta = first ? 0.0 : -S[S_ALPHA] * S[S_ZQ] / S[S_DZOLD];
+    __shared__ double red[256];
+    const double zq = first ? 0.0 : sum_partials(zqp, nzq, red);
+    const double beta = first ? 0.0 : -S[S_ALPHA] * zq / S[S_DZ];
     for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256)
         p[i] = first ? (double)z[i] : (double)z[i] + beta * p[i];
 }
@@ -327,7 +324,7 @@ int setup(gpaw_poisson* ps)
     if (!rc) rc = gpaw_check(hipMalloc((void**)&m->q, n0), "mixed: q");
     if (!rc) rc = gpaw_check(hipMalloc((void**)&m->S, sizeof(double) * S_N), "mixed: scalars");
     if (!rc) rc = gpaw_check(hipHostMalloc((void**)&m->h_S, sizeof(double) * S_N), "mixed: pinned");
-    if (!rc) rc = gpaw_check(hipMalloc((void**)&m->partial, sizeof(double) * 2 * NB), "mixed: partials");
+    if (!rc) rc = gpaw_check(hipMalloc((void**)&m->partial, sizeof(double) * 4 * NB), "mixed: partials");
     if (!rc) rc = gpaw_check(hipMemset(m->S, 0, sizeof(double) * S_N), "mixed: scalars");
     // the fused FP64 coarse solve needs its arrays in LDS (FP32 boundary)
     if (!rc && ps->lc > 0 && ps->h_coarse.lds) {
@@ -378,15 +375,17 @@ hipError_t emit_a(gpaw_poisson* ps, MixedState* m, hipStream_t st, bool last = t
         e = launch_stencil(op->st, DT_F64, ST_APPLY, m->p, m->q, nullptr, op->g, ph, 1, 0.0,
                            nullptr, nullptr, true, st);
         if (e != hipSuccess) return e;
-        launch_kernel(k_dot, gridn(n), dim3(256), 0, st, m->p, m->q, n, m->partial);
-        launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x, m->S,
-                      (int)S_PQ);
+        launch_kernel(k_dot, gridn(n), dim3(256), 0, st, m->p, m->q, n, m->partial + PART_PQ);
+        launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial + PART_PQ,
+                      (int)gridn(n).x, m->S, (int)S_PQ);
     }
+    // dz: the d.z partials of the step's k_dot_dz_zq
     launch_kernel(k_pcg_update, gridn(n), dim3(256), 0, st, ps->phiA[0], ps->res[0], m->p, m->q,
-                  m->rho[0], m->S, n, m->partial, pqp, npq);
+                  m->rho[0], m->S, n, m->partial + PART_DD, pqp, npq, (const double*)m->partial,
+                  (int)gridn(n).x);
     if (last)
-        launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x, m->S,
-                      (int)S_DD);
+        launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial + PART_DD, (int)gridn(n).x,
+                      m->S, (int)S_DD);
     return launch_error();
 }
 
@@ -398,8 +397,8 @@ hipError_t emit_b(gpaw_poisson* ps, MixedState* m, hipStream_t st)
     if (e != hipSuccess) return e;
     launch_kernel(k_dot_dz_zq, gridn(n), dim3(256), 0, st, ps->res[0], z_of(m), m->q, n,
                   m->partial);
-    launch_kernel(k_red2, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x, m->S);
-    launch_kernel(k_p_update, gridn(n), dim3(256), 0, st, m->p, z_of(m), m->S, n, 0);
+    launch_kernel(k_p_update, gridn(n), dim3(256), 0, st, m->p, z_of(m), (const double*)m->S, n,
+                  0, (const double*)m->partial + gridn(n).x, (int)gridn(n).x);
     return launch_error();
 }
 
@@ -503,7 +502,7 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
     if (e == hipSuccess) {   // |rho|^2 for the relative criterion
         launch_kernel(k_dot, gridn(n), dim3(256), 0, st, ps->rho[0], ps->rho[0], n, m->partial);
         launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x, m->S,
-                           (int)S_DZOLD);
+                           (int)S_RR);
         e = launch_error();
     }
     if (e != hipSuccess) return gpaw_check(e, "mixed: load");
@@ -512,7 +511,7 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
     rc = residual64(ps, m, st);
     if (!rc) rc = read_scalar(m, S_DD, st, &dd);
     if (rc) return rc;
-    rr = m->h_S[S_DZOLD];
+    rr = m->h_S[S_RR];
     const double target = rel_tol * rel_tol * rr;
     int it = 0;
     if (!rc && mode == 1 && dd > target) {
@@ -521,9 +520,8 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
         if (!rc) {
             launch_kernel(k_dot_dz_zq, gridn(n), dim3(256), 0, st, ps->res[0], z_of(m),
                                ps->res[0], n, m->partial);
-            launch_kernel(k_red2, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x,
-                               m->S);
-            launch_kernel(k_p_update, gridn(n), dim3(256), 0, st, m->p, z_of(m), m->S, n, 1);
+            launch_kernel(k_p_update, gridn(n), dim3(256), 0, st, m->p, z_of(m),
+                          (const double*)m->S, n, 1, (const double*)nullptr, 0);
             rc = gpaw_check(launch_error(), "mixed: start");
         }
         // PCG steps run in chunks of k (b, a) between readbacks of |d|^2; a
