@@ -95,8 +95,9 @@ long autotune_report(char* buf, long len);
 // Work estimate used to size partial-sum buffers.
 int stencil_partials(const Stencil& s, const Geom& g, int nin);
 // Whether an ST_APPLY pass of this stencil and grid can also produce the
-// partial sums of in . out (real data on the tile kernel); launch_stencil
-// refuses ST_APPLY with partials otherwise.
+// partial sums of in . out, and an ST_JACOBI sweep those of |L in - aux|^2
+// (real data on the tile kernel; the same per-block layout as a residual
+// norm pass); launch_stencil refuses these partials otherwise.
 bool stencil_dot_ok(const Stencil& s, int dtype, const Geom& g, int nin);
 
 // S = JF_SWEEPS weighted-Jacobi sweeps of a 7-point star stencil in one

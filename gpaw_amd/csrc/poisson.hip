@@ -66,10 +66,15 @@ hipError_t relax(gpaw_poisson* ps, int l, int k, double*& cur, double*& alt, hip
     }
     if (l == 0 && ps->rot.total) {   // chunked solve: the cycle's input is kept
         for (int it = 0; it < k; it++) {
+            // the cycle's first sweep may also sum the previous cycle's error
+            // (the residual norm of its input) into ps->norm_in
+            double* part = ps->rot.done == 0 ? ps->norm_in : nullptr;
             double* out = ++ps->rot.done == ps->rot.total ? ps->rot.dst
                                                           : ps->rot.t[(ps->rot.done - 1) & 1];
+            int np = 0;
             hipError_t e = launch_stencil(op->st, DT_F64, ST_JACOBI, cur, out, ps->rho[l], op->g,
-                                          ph, 1, ps->w[l], nullptr, nullptr, true, st);
+                                          ph, 1, ps->w[l], part, part ? &np : nullptr, true, st);
+            if (e == hipSuccess && part && np != ps->np0) e = hipErrorInvalidValue;
             if (e != hipSuccess) return e;
             cur = out;
         }
@@ -141,7 +146,7 @@ hipError_t emit_level(gpaw_poisson* ps, int l, double*& cur0, double*& alt0, hip
     }
     e = relax(ps, l, ps->post[l], cur, alt, st);
     if (e != hipSuccess) return e;
-    if (l == 0) {
+    if (l == 0 && !ps->skip_norm) {
         int np = 0;
         double* part = ps->defer_partial ? ps->defer_partial : op->partial;
         e = launch_stencil(op->st, DT_F64, ST_RESID, cur, nullptr, ps->rho[0], op->g, ph, 1, 0.0,
@@ -209,8 +214,17 @@ hipError_t emit_chunk(gpaw_poisson* ps, int k, int state, hipStream_t st)
             ps->rot.done = 0;
             ps->rot.total = n;
             ps->defer_partial = ps->d_chunk_partials + (long long)i * ps->np0;
+            // with fused norms, cycle i's error is summed by cycle i + 1's
+            // first sweep; the chunk's last cycle takes its own norm pass
+            const bool fuse = ps->fuse_norm && stencil_dot_ok(ps->ops[0]->st, DT_F64,
+                                                              ps->ops[0]->g, 1);
+            ps->norm_in = (fuse && i > 0) ? ps->d_chunk_partials + (long long)(i - 1) * ps->np0
+                                          : nullptr;
+            ps->skip_norm = fuse && i < k - 1;
             e = emit_level(ps, 0, cur, alt, st);
             ps->defer_partial = nullptr;
+            ps->norm_in = nullptr;
+            ps->skip_norm = false;
             ps->rot.total = 0;
             if (e == hipSuccess && cur != ps->snap[i]) e = hipErrorInvalidValue;
         }
@@ -343,6 +357,8 @@ extern "C" int gpaw_poisson_new(int levels, gpaw_operator* const* ops,
     ps->fuse_jacobi = !(env && env[0] == '0');
     env = getenv("GPAW_AMD_FOLD_CORRECTION");
     ps->fold_correction = !(env && env[0] == '0');
+    env = getenv("GPAW_AMD_FUSED_NORM");
+    ps->fuse_norm = !(env && env[0] == '0');
     int rc = GPAW_OK;
     for (int l = 0; l <= levels && !rc; l++) {
         const Geom& g = ps->ops[l]->g;

@@ -56,6 +56,11 @@ struct gpaw_poisson {
     // coarse-grid correction applied by the first fused post-smoothing launch
     // (GPAW_AMD_FOLD_CORRECTION=0: a separate interpolate-and-subtract launch)
     bool fold_correction = true;
+    // in a chunk, a V-cycle's error is summed by the next cycle's first
+    // level-0 sweep (GPAW_AMD_FUSED_NORM=0: a norm pass per cycle)
+    bool fuse_norm = true;
+    double* norm_in = nullptr;   // partials row for the next emitted cycle's first sweep
+    bool skip_norm = false;      // the next emitted cycle takes no norm pass
     gpaw::CoarsePlan h_coarse; // host copy of the coarse plan
     MixedState* mixed = nullptr;
 };

@@ -632,8 +632,10 @@ hipError_t launch_stencil(const Stencil& s, int dtype, int mode, const void* in,
             }
         }
     }
-    // in . out partials of an apply come from the tile kernel only
-    if (partial && mode == ST_APPLY && c.kind != K_TILE) return hipErrorInvalidValue;
+    // in . out partials of an apply and input-residual partials of a Jacobi
+    // sweep come from the tile kernel only
+    if (partial && (mode == ST_APPLY || mode == ST_JACOBI) && c.kind != K_TILE)
+        return hipErrorInvalidValue;
     return launch_cfg(c, s, dtype, mode, in, out, aux, g, ph, nin, w, partial, write, st);
 }
 

@@ -245,25 +245,54 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
 #pragma unroll
         for (int j = 0; j < ROWS; j++) {
             const V* row = B + (R + tyi * ROWS + j) * LW + R + tx;
-            V x = vzero<V>();
+            // RIN (Jacobi with NORM): also the residual of the input, summed
+            // as the residual pass sums it (centre term first)
+            constexpr bool RIN = NORM && MODE == ST_JACOBI;
+            V x = vzero<V>(), xr = vzero<V>();
             if (MODE != ST_JACOBI) x = vmac(x, q[j][R], (RT)sc.c[0]);
+            if (RIN) xr = vmac(xr, q[j][R], (RT)sc.c[0]);
 #pragma unroll
-            for (int k = 1; k <= R; k++) x = vmac(x, row[k], (RT)sc.c[k]);
+            for (int k = 1; k <= R; k++) {
+                x = vmac(x, row[k], (RT)sc.c[k]);
+                if (RIN) xr = vmac(xr, row[k], (RT)sc.c[k]);
+            }
 #pragma unroll
-            for (int k = 1; k <= R; k++) x = vmac(x, row[-k], (RT)sc.c[R + k]);
+            for (int k = 1; k <= R; k++) {
+                x = vmac(x, row[-k], (RT)sc.c[R + k]);
+                if (RIN) xr = vmac(xr, row[-k], (RT)sc.c[R + k]);
+            }
 #pragma unroll
-            for (int k = 1; k <= R; k++) x = vmac(x, win[R + j + k], (RT)sc.c[2 * R + k]);
+            for (int k = 1; k <= R; k++) {
+                x = vmac(x, win[R + j + k], (RT)sc.c[2 * R + k]);
+                if (RIN) xr = vmac(xr, win[R + j + k], (RT)sc.c[2 * R + k]);
+            }
 #pragma unroll
-            for (int k = 1; k <= R; k++) x = vmac(x, win[R + j - k], (RT)sc.c[3 * R + k]);
+            for (int k = 1; k <= R; k++) {
+                x = vmac(x, win[R + j - k], (RT)sc.c[3 * R + k]);
+                if (RIN) xr = vmac(xr, win[R + j - k], (RT)sc.c[3 * R + k]);
+            }
 #pragma unroll
-            for (int k = 1; k <= R; k++) x = vmac(x, q[j][R + k], (RT)sc.c[4 * R + k]);
+            for (int k = 1; k <= R; k++) {
+                x = vmac(x, q[j][R + k], (RT)sc.c[4 * R + k]);
+                if (RIN) xr = vmac(xr, q[j][R + k], (RT)sc.c[4 * R + k]);
+            }
 #pragma unroll
-            for (int k = 1; k <= R; k++) x = vmac(x, q[j][R - k], (RT)sc.c[5 * R + k]);
+            for (int k = 1; k <= R; k++) {
+                x = vmac(x, q[j][R - k], (RT)sc.c[5 * R + k]);
+                if (RIN) xr = vmac(xr, q[j][R - k], (RT)sc.c[5 * R + k]);
+            }
             rp[j] = st_epilogue<MODE>(x, q[j][R], MODE == ST_APPLY ? vzero<V>() : s[j], (RT)w,
                                       (RT)sc.c[0]);
-            // NORM: |result|^2, or for ST_APPLY in . result (a CG step's p.Lp)
-            if (NORM && inr[j])
-                acc += MODE == ST_APPLY ? tdot(q[j][R], rp[j]) : tnorm2(rp[j]);
+            // NORM: |result|^2; for ST_APPLY in . result (a CG step's p.Lp);
+            // for ST_JACOBI |L in - rho|^2 (the previous V-cycle's error)
+            if (NORM && inr[j]) {
+                if (MODE == ST_APPLY)
+                    acc += tdot(q[j][R], rp[j]);
+                else if (RIN)
+                    acc += tnorm2(st_epilogue<ST_RESID>(xr, q[j][R], s[j], (RT)w, (RT)sc.c[0]));
+                else
+                    acc += tnorm2(rp[j]);
+            }
         }
 #pragma unroll
         for (int j = 0; j < ROWS; j++)

@@ -58,13 +58,15 @@ hipError_t tile_modes(const Stencil& s, int mode, const void* in, void* out, con
                       const Geom& g, const Phases& ph, double w, const TileShape& sh,
                       double* partial, bool write, hipStream_t st)
 {
-    // partials: residual norms (RESID, the Poisson loop) or in . out (APPLY,
-    // the mixed solver's CG step); other modes never carry them
-    if (partial && mode != ST_RESID && mode != ST_APPLY) return hipErrorInvalidValue;
+    // partials: residual norms (RESID; JACOBI: of the input, the previous
+    // V-cycle's error) or in . out (APPLY, the mixed solver's CG step);
+    // other modes never carry them
+    if (partial && mode != ST_RESID && mode != ST_APPLY && mode != ST_JACOBI)
+        return hipErrorInvalidValue;
     switch (mode) {
     case ST_APPLY: return tile_mode<V, R, TZ, ST_APPLY>(s, in, out, aux, g, ph, w, sh, partial, true, st);
     case ST_RESID: return tile_mode<V, R, TZ, ST_RESID>(s, in, out, aux, g, ph, w, sh, partial, write, st);
-    case ST_JACOBI: return tile_mode<V, R, TZ, ST_JACOBI>(s, in, out, aux, g, ph, w, sh, nullptr, true, st);
+    case ST_JACOBI: return tile_mode<V, R, TZ, ST_JACOBI>(s, in, out, aux, g, ph, w, sh, partial, true, st);
     case ST_PCUPD: return tile_mode<V, R, TZ, ST_PCUPD>(s, in, out, aux, g, ph, w, sh, nullptr, true, st);
     case ST_PCFIN: return tile_mode<V, R, TZ, ST_PCFIN>(s, in, out, aux, g, ph, w, sh, nullptr, true, st);
     case ST_HPSI: return tile_mode<V, R, TZ, ST_HPSI>(s, in, out, aux, g, ph, w, sh, nullptr, true, st);

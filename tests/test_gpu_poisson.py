@@ -91,9 +91,15 @@ def test_every_stopping_cycle():
         phi_ref = start.copy()
         n_ref = ops.solve_neutral(phi_ref, rho.copy(), eps=eps)
         phi = dev(start)
-        n = gps.solve_neutral(phi, dev(rho), eps=eps)
+        rho_d = dev(rho)
+        n = gps.solve_neutral(phi, rho_d, eps=eps)
         assert n == n_ref, eps
         assert np.array_equal(phi.cpu().numpy(), phi_ref), eps
+        # the stopping cycle's error (summed by a norm pass or by the next
+        # cycle's first sweep) is the residual norm of the returned phi
+        norm = _torch().zeros(1, dtype=_torch().float64, device='cuda')
+        gps.operators[0].operator.residual(phi, rho_d, None, norm, og.dv)
+        assert float(norm.cpu()[0]) == gps.error, eps
         seen.add(n)
     assert len(seen) >= 10
 
@@ -253,8 +259,10 @@ def test_fused_jacobi_sweeps_identical(N, pbc, monkeypatch):
     launch, whose first post-smoothing launch also applies the coarse-grid
     correction; the potential and niter must be bit-identical to single
     sweeps (GPAW_AMD_FUSED_JACOBI=0) and to fused sweeps after a separate
-    correction launch (GPAW_AMD_FOLD_CORRECTION=0), both read when the plan
-    is built."""
+    correction launch (GPAW_AMD_FOLD_CORRECTION=0); each V-cycle's error is
+    summed by the next cycle's first sweep or by a norm pass of its own
+    (GPAW_AMD_FUSED_NORM=0), with the same bits, so the same stopping cycle;
+    all read when the plan is built."""
     print('case fused Jacobi', N, pbc, flush=True)
     torch = _torch()
     from gpaw_amd.grid_descriptor import GridDescriptor
@@ -264,16 +272,18 @@ def test_fused_jacobi_sweeps_identical(N, pbc, monkeypatch):
     rho = _density(og, N + 1, 4, 0.5 / BOHR, 1.5 / BOHR, 2.0 / BOHR)
     rho -= og.integrate(rho) / og.dv / og.size_of_global_array().prod()
     out = []
-    for fused, fold in (('1', '1'), ('1', '0'), ('0', '1')):
+    for fused, fold, fnorm in (('1', '1', '1'), ('1', '0', '1'), ('0', '1', '1'), ('1', '1', '0')):
         monkeypatch.setenv('GPAW_AMD_FUSED_JACOBI', fused)
         monkeypatch.setenv('GPAW_AMD_FOLD_CORRECTION', fold)
+        monkeypatch.setenv('GPAW_AMD_FUSED_NORM', fnorm)
         gd = GridDescriptor((N,) * 3, (N * h,) * 3, pbc)
         gps = PoissonSolver(nn=3, relax='J')
         gps.set_grid_descriptor(gd)
         gps.initialize()
         phi = dev(np.zeros(tuple(og.n_c)))
         n = gps.solve_neutral(phi, dev(rho), eps=2e-10)
-        out.append((n, phi))
-    for n, phi in out[1:]:
+        out.append((n, phi, gps.error))
+    for n, phi, err in out[1:]:
         assert n == out[0][0]
         assert torch.equal(phi, out[0][1])
+        assert err == out[0][2]
