@@ -45,7 +45,9 @@ def parse():
     p.add_argument('--no-extra', action='store_true')
     p.add_argument('--cpu-seconds', type=float, default=12.0)
     p.add_argument('--serve-n', type=int, default=2000)
-    p.add_argument('--serve-rate', type=float, default=300.0)
+    p.add_argument('--serve-rate', type=float, default=None,
+                   help='arrival rate of the latency run (default: 80%% of '
+                        'the measured capacity)')
     p.add_argument('--only', choices=sorted(SECTIONS),
                    help='run one measurement section alone and print its '
                         'JSON (for profiling a single workload)')
@@ -222,7 +224,55 @@ def fd_laplace_c3(torch, np):
     res['c1_real_8x48'] = {'avg_apply_us': round(t1 * 1e6, 2),
                            'GBps': round(algo1 / t1 / 1e9, 1),
                            'bytes_per_apply': algo1}
+    del a1, b1
+    res['batch'] = batch_c3(torch, np)
     return res
+
+
+def batch_c3(torch, np):
+    """Several k-points and uneven grids in one batched apply (one
+    persistent launch per stencil range) against the same groups applied
+    one after another: complex128 nn=3 Laplace, 4 seeded k-points x 48
+    bands of 64^3 periodic, plus 32 bands of 48^3 and 24 bands of
+    80 x 64 x 72 (periodic, own spacing and k-point), 16 bands of 56^3 zero
+    boundary (Gamma).  Both bit-identical; algorithmic bytes 32 B per
+    point and band."""
+    from gpaw_amd.batch import BatchApply
+    from gpaw_amd.fd_operators import Laplace
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    rng = np.random.default_rng(41)
+    g = torch.Generator(device='cuda').manual_seed(43)
+    specs = [((64, 64, 64), True, 0.2, 48)] * 4 + [
+        ((48, 48, 48), True, 0.22, 32), ((80, 64, 72), True, 0.18, 24),
+        ((56, 56, 56), False, 0.2, 16)]
+    groups, algo = [], 0.0
+    for N, pbc, h, nb in specs:
+        gd = GridDescriptor(N, np.array(N) * h / BOHR, pbc)
+        op = Laplace(gd, -0.5, 3, complex)
+        a = torch.randn((nb,) + tuple(gd.n_c), dtype=torch.complex128,
+                        device='cuda', generator=g)
+        ph = gd.phase_cd(rng.uniform(-0.5, 0.5, 3)) if pbc else None
+        groups.append((op, a, torch.empty_like(a), ph))
+        algo += 32.0 * nb * np.prod(gd.n_c)
+    batch = BatchApply(max_groups=len(groups))
+    tb = event_time(torch, lambda: batch(groups), 10)
+    outs = [b.clone() for _, _, b, _ in groups]
+
+    def sequential():
+        for op, a, b, ph in groups:
+            op.apply(a, b, ph)
+    ts = event_time(torch, sequential, 10)
+    same = all(torch.equal(o, b) for o, (_, _, b, _) in zip(outs, groups))
+    del groups, outs
+    return {'config': 'complex128 nn=3 Laplace: 4 k-points x 48 bands of 64^3, '
+                      '32 x 48^3, 24 x 80x64x72 (periodic, Bloch phases), '
+                      '16 x 56^3 zero boundary; 7 groups',
+            'batched_ms': round(tb * 1e3, 3), 'batched_GBps': round(algo / tb / 1e9, 1),
+            'batched_pct_peak': round(100 * algo / tb / 1e9 / PEAK_HBM_GBS, 1),
+            'sequential_ms': round(ts * 1e3, 3),
+            'sequential_GBps': round(algo / ts / 1e9, 1),
+            'speedup_batched_over_sequential': round(ts / tb, 3),
+            'bit_identical': bool(same), 'bytes': algo}
 
 
 def preconditioner_c3(torch, np):
@@ -373,9 +423,18 @@ def callers_c3(torch, np):
 
 
 def mixed_vs_fp64(torch, np, N=160, h=0.18, count=32, reps=10):
-    """Mixed-precision vs plain FP64 multigrid on the 160^3 zero-boundary
+    """C4: mixed-precision vs plain FP64 multigrid on the 160^3 zero-boundary
     configuration (159^3 arrays, h = 0.18 A, 32 seeded neutral Gaussians of
-    widths 0.4-1.2 A), both to ||L phi - rho|| / ||rho|| <= 1e-10."""
+    widths 0.4-1.2 A), all to ||L phi - rho|| / ||rho|| <= 1e-10.
+
+    Lines: 'fp64' = the reference's V-cycles (FP64 PoissonSolver);
+    'mixed' = FP32 V-cycle preconditioning FP64 flexible CG (the product's
+    mixed solver); 'pcg_fp64_vcycle' = the same CG with the same V-cycle in
+    FP64 (what the outer iteration alone gains); 'refine_fp32' = FP32
+    V-cycles inside FP64 iterative refinement (two V-cycles per
+    correction).  mixed / pcg_fp64_vcycle is what FP32 itself contributes;
+    the per-iteration time ratio compares one FP32 step with one FP64 step
+    of the same loop."""
     from gpaw_amd.grid_descriptor import GridDescriptor
     from gpaw_amd.poisson import MixedPoissonSolver, PoissonSolver
     from gpaw_amd.utilities import gaussian_density, random_gaussian_charges
@@ -388,8 +447,14 @@ def mixed_vs_fp64(torch, np, N=160, h=0.18, count=32, reps=10):
     eps, _ = eps_for(np, gd, rho)
     res = {}
     phis = {}
-    for name, solver in (('fp64', PoissonSolver(nn=3, eps=eps)),
-                         ('mixed', MixedPoissonSolver(nn=3, rel_tol=1e-10))):
+    solvers = (('fp64', lambda: PoissonSolver(nn=3, eps=eps)),
+               ('mixed', lambda: MixedPoissonSolver(nn=3, rel_tol=1e-10)),
+               ('pcg_fp64_vcycle', lambda: MixedPoissonSolver(
+                   nn=3, rel_tol=1e-10, inner_precision='fp64')),
+               ('refine_fp32', lambda: MixedPoissonSolver(
+                   nn=3, rel_tol=1e-10, outer='refine', inner=2)))
+    for name, make in solvers:
+        solver = make()
         solver.set_grid_descriptor(gd)
         solver.initialize()
         phi = torch.zeros(tuple(gd.n_c), dtype=torch.float64, device='cuda')
@@ -405,7 +470,8 @@ def mixed_vs_fp64(torch, np, N=160, h=0.18, count=32, reps=10):
         dt = (time.perf_counter() - t0) / reps
         res[name] = {'solves_per_s': round(1.0 / dt, 3),
                      'ms_per_solve': round(dt * 1e3, 3),
-                     'iterations': it}
+                     'iterations': it,
+                     'ms_per_iteration': round(dt * 1e3 / max(it, 1), 4)}
         phis[name] = phi
         del solver
     bg = float(rho.sum()) / np.prod(gd.get_size_of_global_array())
@@ -417,47 +483,77 @@ def mixed_vs_fp64(torch, np, N=160, h=0.18, count=32, reps=10):
         lap.operator.residual(phis[name], rho_n, r)
         res[name]['rel_residual'] = float(torch.linalg.vector_norm(r).item() /
                                           torch.linalg.vector_norm(rho_n).item())
-    d = (phis['mixed'] - phis['fp64']).abs().max().item()
-    res['mixed']['rel_maxnorm_err_vs_fp64'] = d / phis['fp64'].abs().max().item()
-    res['speedup_mixed_over_fp64'] = round(
-        res['fp64']['ms_per_solve'] / res['mixed']['ms_per_solve'], 3)
-    res['config'] = ('160^3 zero boundary (159^3 arrays), h=0.18 A, 32 neutral '
-                     'Gaussians; FP64 = reference V-cycles; mixed = FP32 '
-                     'V-cycle preconditioning an FP64 flexible-CG refinement')
+        if name != 'fp64':
+            d = (phis[name] - phis['fp64']).abs().max().item()
+            res[name]['rel_maxnorm_err_vs_fp64'] = d / phis['fp64'].abs().max().item()
+    ms = {k: res[k]['ms_per_solve'] for k in res}
+    res['speedup_mixed_over_fp64'] = round(ms['fp64'] / ms['mixed'], 3)
+    res['speedup_pcg_fp64_vcycle_over_fp64'] = round(ms['fp64'] / ms['pcg_fp64_vcycle'], 3)
+    res['speedup_refine_fp32_over_fp64'] = round(ms['fp64'] / ms['refine_fp32'], 3)
+    res['fp32_contribution'] = round(ms['pcg_fp64_vcycle'] / ms['mixed'], 3)
+    res['fp32_over_fp64_iteration_time'] = round(
+        res['mixed']['ms_per_iteration'] / res['pcg_fp64_vcycle']['ms_per_iteration'], 3)
+    res['config'] = ('C4: 160^3 zero boundary (159^3 arrays), h=0.18 A, 32 neutral '
+                     'Gaussians, 1e-10 relative residual')
     return res
 
 
-def serving(torch, np, n=2000, rate=300.0, slots=4, budget_gb=64):
-    """Seeded Poisson-arrival stream of n requests (80 % mixed-precision
-    Poisson solves to 1e-10, 20 % Laplace applies on 1-64 bands; edges
-    48..160, half periodic), data resident in HBM, served by the
-    continuous-batching server; one warm-up request per key first so that
-    hierarchies are resident."""
+def serving(torch, np, n=2000, rate=None, slots=4, budget_gb=64):
+    """C5: a seeded Poisson-arrival stream of n requests (80 % mixed-
+    precision Poisson solves to 1e-10, 20 % Laplace applies on 1-64 bands;
+    edges 48..160, half periodic), data resident in HBM, served by the
+    continuous-batching server; results are taken by the caller as they
+    complete.  First the capacity: every request released at t = 0 (the
+    slots never idle), requests/s over the span.  Then latency: the same
+    requests arriving as a Poisson process at 80 % of that capacity (or at
+    `rate` when given), p50 / p99 from arrival to result.  One warm-up
+    request per key first, so that hierarchies are resident."""
     from gpaw_amd import ops
     from gpaw_amd.serving import Request, Server, synthetic_stream
     # requests come with arbitrary band counts: no shape tuning on the
     # serving path (the heuristic shapes are used)
     ops.autotune(False)
     t0 = time.perf_counter()
-    reqs = synthetic_stream(n=n, rate=rate, seed=5)
+    base_rate = 300.0
+    reqs = synthetic_stream(n=n, rate=base_rate, seed=5)
     setup = time.perf_counter() - t0
-    srv = Server(slots=slots, hbm_budget=budget_gb << 30)
+
+    def drop(r):
+        r.result = None
+    srv = Server(slots=slots, hbm_budget=budget_gb << 30, on_result=drop)
     seen = {}
     for r in reqs:
         seen.setdefault(r.key, r)
     warm = [Request(-1 - i, r.kind, r.N, r.pbc, r.payload, 0.0)
             for i, r in enumerate(seen.values())]
     srv.run(warm, realtime=False)
-    rep = srv.run(reqs, realtime=True)
-    rep.update({'slots': slots, 'arrival_rate_per_s': rate,
-                'hbm_budget_GB': budget_gb, 'stream_setup_s': round(setup, 2),
-                'config': '%d requests, Poisson arrivals at %g/s, 80%% mixed-'
-                          'precision Poisson (1e-10) / 20%% Laplace nn=3 on '
-                          '1-64 bands, edges 48-160, half periodic' % (n, rate)})
+
+    def fresh(scale):
+        return [Request(r.rid, r.kind, r.N, r.pbc, r.payload, r.arrival * scale)
+                for r in reqs]
+    cap = srv.run(fresh(1.0), realtime=False)
+    capacity = cap['requests_per_s']
+    load = rate if rate else 0.8 * capacity
+    rep = srv.run(fresh(base_rate / load), realtime=True)
+    out = {'capacity': {'requests_per_s': round(capacity, 1),
+                        'poisson_solves_per_s': round(cap['poisson_solves_per_s'], 1),
+                        'span_s': round(cap['span_s'], 3),
+                        'completed': cap['completed'],
+                        'note': 'all requests released at t=0; latency there is '
+                                'queueing time and not reported'},
+           'load_requests_per_s': round(load, 1),
+           'load_fraction_of_capacity': round(load / capacity, 3) if capacity else None}
+    out.update(rep)
+    out.update({'slots': slots, 'hbm_budget_GB': budget_gb,
+                'stream_setup_s': round(setup, 2), 'plans': srv.stats(),
+                'config': 'C5: %d requests, 80%% mixed-precision Poisson (1e-10) / '
+                          '20%% Laplace nn=3 on 1-64 bands, edges 48-160, half '
+                          'periodic; Laplace requests waiting together go out as '
+                          'one batched apply' % n})
     for k in ('p50_ms', 'p99_ms', 'span_s', 'requests_per_s',
               'poisson_solves_per_s'):
-        rep[k] = round(rep[k], 3)
-    return rep
+        out[k] = round(out[k], 3)
+    return out
 
 
 def _cpu_solves(O, kind, gd_shape_args, rho, eps, budget):

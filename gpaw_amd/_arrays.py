@@ -16,6 +16,10 @@ _NP = {np.dtype(np.float64): _lib.GPAW_F64,
        np.dtype(np.complex64): _lib.GPAW_C64}
 
 
+_CODE_NAMES = {_lib.GPAW_F64: 'float64', _lib.GPAW_C128: 'complex128',
+               _lib.GPAW_F32: 'float32', _lib.GPAW_C64: 'complex64'}
+
+
 def torch():
     import torch as _t
     return _t
@@ -52,7 +56,7 @@ class Arg:
     code  GPAW_* dtype code
     """
 
-    def __init__(self, a, write=False, name='array'):
+    def __init__(self, a, write=False, name='array', shape=None, code=None):
         t = torch()
         self.host = None
         self.write = write
@@ -72,6 +76,12 @@ class Arg:
             self.tensor = t.from_numpy(a).to('cuda')
         self.code = dtype_code(self.tensor)
         self.shape = tuple(self.tensor.shape)
+        if code is not None and self.code != code:
+            raise TypeError('%s: dtype %s, expected %s' % (
+                name, self.tensor.dtype, _CODE_NAMES[code]))
+        if shape is not None and self.shape != tuple(shape):
+            raise ValueError('%s: shape %s, expected %s' % (
+                name, self.shape, tuple(shape)))
         self.ptr = self.tensor.data_ptr()
 
     def finish(self):

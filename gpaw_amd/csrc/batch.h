@@ -27,12 +27,14 @@ struct BatchGroup {
     int pad;
 };
 
-template <class V, int R, int TZ>
-__global__ __launch_bounds__(TILE_NT) void k_star_tile_batch(const BatchGroup* __restrict__ groups,
-                                                         int ngroups, int ntiles)
+// NT threads per block: 256, or 1024 for complex data of range <= 3 (the
+// single-grid kernel's wide tiles: taller tiles, fewer halo rows)
+template <class V, int R, int TZ, int NT>
+__global__ __launch_bounds__(NT) void k_star_tile_batch(const BatchGroup* __restrict__ groups,
+                                                        int ngroups, int ntiles)
 {
-    constexpr int ROWS = tile_rows(R, (int)sizeof(V));
-    typedef TileGeom<V, R, TZ, ROWS> TG;
+    constexpr int ROWS = tile_rows(R, (int)sizeof(V), NT);
+    typedef TileGeom<V, R, TZ, ROWS, NT> TG;
     __shared__ V lds[TG::LDS];
     const int nb = gridDim.x;
     const int slot = xcd_tile(blockIdx.x, nb);
@@ -57,15 +59,15 @@ __global__ __launch_bounds__(TILE_NT) void k_star_tile_batch(const BatchGroup* _
         const int band = r / sh.nchunk;
         const i64 boff = (i64)band * gp->ng;
         const int i0s = ch * sh.q;
-        star_tile_sweep<V, R, TZ, ROWS, ST_APPLY, false, true>(
+        star_tile_sweep<V, R, TZ, ROWS, ST_APPLY, false, true, NT>(
             (const V*)gp->in + boff, (V*)gp->out + boff, nullptr, gp->g, gp->ph, gp->sc, 0.0,
             tz * TZ, ty * TG::TY, i0s, min(i0s + sh.q, gp->g.n[0]), lds);
         __syncthreads();   // the next tile rewrites the LDS plane buffers
     }
 }
 
-// Resident-block count of the (V, R, TZ) batch kernel on this device.
-template <class V, int R, int TZ>
+// Resident-block count of the (V, R, TZ, NT) batch kernel on this device.
+template <class V, int R, int TZ, int NT>
 int batch_resident()
 {
     static int n = 0;
@@ -74,7 +76,7 @@ int batch_resident()
         hipGetDevice(&dev);
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, (const void*)k_star_tile_batch<V, R, TZ>, TILE_NT, 0) != hipSuccess ||
+                &per_cu, (const void*)k_star_tile_batch<V, R, TZ, NT>, NT, 0) != hipSuccess ||
             per_cu < 1)
             per_cu = 1;
         n = per_cu * (cus > 0 ? cus : 256);
@@ -82,26 +84,45 @@ int batch_resident()
     return n;
 }
 
-template <class V, int R, int TZ>
-hipError_t batch_go(const BatchGroup* d_groups, int ngroups, int ntiles, hipStream_t st)
+template <class V, int R, int TZ, int NT>
+hipError_t batch_go_nt(const BatchGroup* d_groups, int ngroups, int ntiles, hipStream_t st)
 {
-    int nb = batch_resident<V, R, TZ>();
+    int nb = batch_resident<V, R, TZ, NT>();
     if (nb >= ntiles)
         nb = ntiles;
     else if (nb >= 8)
         nb &= ~7;   // whole rounds of 8 XCDs
-    return launch_kernel(k_star_tile_batch<V, R, TZ>, dim3(nb), dim3(TZ, TILE_NT / TZ), 0, st,
+    return launch_kernel(k_star_tile_batch<V, R, TZ, NT>, dim3(nb), dim3(TZ, NT / TZ), 0, st,
                          d_groups, ngroups, ntiles);
+}
+
+// block size a batch of range R uses for storage type V (groups are cut
+// into tiles of that size)
+template <class V>
+constexpr int batch_nt(int R)
+{
+    return (VT<V>::cplx && R <= 3) ? TILE_NT_WIDE : TILE_NT;
+}
+
+template <class V, int R, int TZ>
+hipError_t batch_go(const BatchGroup* d_groups, int ngroups, int ntiles, int nt, hipStream_t st)
+{
+    if constexpr (VT<V>::cplx && R <= 3) {
+        if (nt == TILE_NT_WIDE)
+            return batch_go_nt<V, R, TZ, TILE_NT_WIDE>(d_groups, ngroups, ntiles, st);
+    }
+    if (nt != TILE_NT) return hipErrorInvalidValue;
+    return batch_go_nt<V, R, TZ, TILE_NT>(d_groups, ngroups, ntiles, st);
 }
 
 template <class V>
 hipError_t batch_dispatch(const BatchGroup* d_groups, int ngroups, int ntiles, int R, int TZ,
-                          hipStream_t st)
+                          int nt, hipStream_t st)
 {
 #define GPAW_BATCH_R(RR)                                                                  \
     case RR:                                                                              \
-        return TZ == 64 ? batch_go<V, RR, 64>(d_groups, ngroups, ntiles, st)             \
-                        : batch_go<V, RR, 32>(d_groups, ngroups, ntiles, st);
+        return TZ == 64 ? batch_go<V, RR, 64>(d_groups, ngroups, ntiles, nt, st)         \
+                        : batch_go<V, RR, 32>(d_groups, ngroups, ntiles, nt, st);
     switch (R) {
         GPAW_BATCH_R(1)
         GPAW_BATCH_R(2)
@@ -113,6 +134,6 @@ hipError_t batch_dispatch(const BatchGroup* d_groups, int ngroups, int ntiles, i
 }
 
 hipError_t batch_launch(int dtype, const BatchGroup* d_groups, int ngroups, int ntiles, int R,
-                        int TZ, hipStream_t st);
+                        int TZ, int nt, hipStream_t st);
 
 }  // namespace gpaw

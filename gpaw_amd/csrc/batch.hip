@@ -7,6 +7,7 @@
 // launches still run.  Groups whose grid cannot be cut into tiles (tiny or
 // very narrow periodic grids) and non-star stencils are applied by the
 // single-operator kernels on the same stream.
+#include <algorithm>
 #include <vector>
 
 #include "batch.h"
@@ -14,21 +15,28 @@
 
 namespace gpaw {
 
-hipError_t batch_launch_f64(const BatchGroup*, int, int, int, int, hipStream_t);
-hipError_t batch_launch_c128(const BatchGroup*, int, int, int, int, hipStream_t);
-hipError_t batch_launch_f32(const BatchGroup*, int, int, int, int, hipStream_t);
-hipError_t batch_launch_c64(const BatchGroup*, int, int, int, int, hipStream_t);
+hipError_t batch_launch_f64(const BatchGroup*, int, int, int, int, int, hipStream_t);
+hipError_t batch_launch_c128(const BatchGroup*, int, int, int, int, int, hipStream_t);
+hipError_t batch_launch_f32(const BatchGroup*, int, int, int, int, int, hipStream_t);
+hipError_t batch_launch_c64(const BatchGroup*, int, int, int, int, int, hipStream_t);
 
 hipError_t batch_launch(int dtype, const BatchGroup* d_groups, int ngroups, int ntiles, int R,
-                        int TZ, hipStream_t st)
+                        int TZ, int nt, hipStream_t st)
 {
     switch (dtype) {
-    case DT_F64: return batch_launch_f64(d_groups, ngroups, ntiles, R, TZ, st);
-    case DT_C128: return batch_launch_c128(d_groups, ngroups, ntiles, R, TZ, st);
-    case DT_F32: return batch_launch_f32(d_groups, ngroups, ntiles, R, TZ, st);
-    case DT_C64: return batch_launch_c64(d_groups, ngroups, ntiles, R, TZ, st);
+    case DT_F64: return batch_launch_f64(d_groups, ngroups, ntiles, R, TZ, nt, st);
+    case DT_C128: return batch_launch_c128(d_groups, ngroups, ntiles, R, TZ, nt, st);
+    case DT_F32: return batch_launch_f32(d_groups, ngroups, ntiles, R, TZ, nt, st);
+    case DT_C64: return batch_launch_c64(d_groups, ngroups, ntiles, R, TZ, nt, st);
     }
     return hipErrorInvalidValue;
+}
+
+// Block size of the batch kernel for range R and storage type dtype.
+static int batch_block(int dtype, int R)
+{
+    const bool cplx = dtype == DT_C128 || dtype == DT_C64;
+    return (cplx && R <= 3) ? TILE_NT_WIDE : TILE_NT;
 }
 
 }  // namespace gpaw
@@ -108,6 +116,35 @@ extern "C" int gpaw_batch_apply(gpaw_batch* b, int ngroups, gpaw_operator* const
         if (nins[i] && ins[i] == outs[i])
             return gpaw_fail(GPAW_EINVAL, "batch: group %d input and output must differ", i);
     }
+    // every group's tiles run concurrently: no output may overlap any input
+    // or another output (inputs may overlap each other)
+    {
+        struct Span {
+            uintptr_t lo, hi;
+            int out, group;
+        };
+        std::vector<Span> sp;
+        sp.reserve(2 * (size_t)ngroups);
+        for (int i = 0; i < ngroups; i++) {
+            if (!nins[i]) continue;
+            const uintptr_t bytes = (uintptr_t)nins[i] * (uintptr_t)ops[i]->g.n[0] *
+                                    (uintptr_t)ops[i]->g.n[1] * (uintptr_t)ops[i]->g.n[2] *
+                                    (uintptr_t)esize;
+            sp.push_back({(uintptr_t)ins[i], (uintptr_t)ins[i] + bytes, 0, i});
+            sp.push_back({(uintptr_t)outs[i], (uintptr_t)outs[i] + bytes, 1, i});
+        }
+        std::sort(sp.begin(), sp.end(), [](const Span& a, const Span& b) { return a.lo < b.lo; });
+        uintptr_t end_all = 0, end_out = 0;
+        for (const Span& x : sp) {
+            if (x.lo < end_out || (x.out && x.lo < end_all))
+                return gpaw_fail(GPAW_EINVAL,
+                                 "batch: group %d's %s overlaps another group's output or its "
+                                 "input (the groups run concurrently)",
+                                 x.group, x.out ? "output" : "input");
+            end_all = std::max(end_all, x.hi);
+            if (x.out) end_out = std::max(end_out, x.hi);
+        }
+    }
     // one slot of descriptors; wait until the launch that last used it ended
     const int s = b->next;
     b->next = (b->next + 1) % SLOTS;
@@ -126,15 +163,27 @@ extern "C" int gpaw_batch_apply(gpaw_batch* b, int ngroups, gpaw_operator* const
     int n = 0;
     int start[GPAW_MAXR + 2] = {0};
     int tiles[GPAW_MAXR + 1] = {0};
+    int nts[GPAW_MAXR + 1] = {0};
     std::vector<char> fallback(ngroups, 0);
     for (int R = 1; R <= 4; R++) {
         start[R] = n;
         const int TZ = tz_vote[R] >= 0 ? 64 : 32;
+        // wide blocks for complex data when every group of the range can be
+        // cut into them (one launch has one block size), else 256
+        int nt = batch_block(dtype, R);
+        for (int i = 0; i < ngroups && nt != TILE_NT; i++) {
+            TileShape sh;
+            if (ops[i]->st.star_r == R && nins[i] &&
+                !tile_geometry(ops[i]->g, R, esize, (int)nins[i], TZ, BATCH_Q, sh, nt) &&
+                tile_geometry(ops[i]->g, R, esize, (int)nins[i], TZ, BATCH_Q, sh, TILE_NT))
+                nt = TILE_NT;
+        }
+        nts[R] = nt;
         for (int i = 0; i < ngroups; i++) {
             gpaw_operator* op = ops[i];
             if (op->st.star_r != R || !nins[i]) continue;
             TileShape sh;
-            if (!tile_geometry(op->g, R, esize, (int)nins[i], TZ, BATCH_Q, sh) ||
+            if (!tile_geometry(op->g, R, esize, (int)nins[i], TZ, BATCH_Q, sh, nt) ||
                 (long)tiles[R] + sh.nblocks > 0x7fffffffL) {
                 fallback[i] = 1;
                 continue;
@@ -159,7 +208,7 @@ extern "C" int gpaw_batch_apply(gpaw_batch* b, int ngroups, gpaw_operator* const
         for (int R = 1; R <= 4 && e == hipSuccess; R++) {
             if (!count[R]) continue;
             const int TZ = tz_vote[R] >= 0 ? 64 : 32;
-            e = batch_launch(dtype, b->d[s] + start[R], count[R], tiles[R], R, TZ, st);
+            e = batch_launch(dtype, b->d[s] + start[R], count[R], tiles[R], R, TZ, nts[R], st);
         }
         // the slot is busy until the copy and the launches reading it are done
         const hipError_t er = hipEventRecord(b->ev[s], st);

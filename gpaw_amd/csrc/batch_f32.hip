@@ -4,9 +4,9 @@
 namespace gpaw {
 
 hipError_t batch_launch_f32(const BatchGroup* d_groups, int ngroups, int ntiles, int R, int TZ,
-                          hipStream_t st)
+                          int nt, hipStream_t st)
 {
-    return batch_dispatch<float>(d_groups, ngroups, ntiles, R, TZ, st);
+    return batch_dispatch<float>(d_groups, ngroups, ntiles, R, TZ, nt, st);
 }
 
 }  // namespace gpaw

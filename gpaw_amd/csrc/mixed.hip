@@ -10,18 +10,24 @@
 //         interpolation of the inner V-cycle on every level above that.
 //
 // Outer loop (FP64), either
-//   mode 0  iterative refinement: d = L phi - rho; e = V32^k(d); phi -= e
-//   mode 1  flexible preconditioned conjugate gradients with one FP32
-//           V-cycle as preconditioner (Polak-Ribiere beta, which tolerates
-//           the preconditioner's FP32 rounding), i.e. iterative refinement
-//           with a Krylov recurrence on the corrections.
+//   iterative refinement  d = L phi - rho; e = V^k(d); phi -= e
+//   flexible PCG          preconditioned conjugate gradients with one inner
+//                         V-cycle as preconditioner (Polak-Ribiere beta,
+//                         which tolerates the preconditioner's FP32
+//                         rounding), i.e. iterative refinement with a Krylov
+//                         recurrence on the corrections; when the recursive
+//                         |d| meets the target the true residual is
+//                         recomputed, and the iteration restarts from it if
+//                         it does not.
 // The inner V-cycle uses the reference hierarchy (gpaw/poisson.py:64-113)
 // with symmetric level-0 smoothing (2 pre + 2 post by default) so that it is
-// a symmetric preconditioner.  Stopping rule: ||d||_2 <= tol * ||rho||_2.
+// a symmetric preconditioner.  It runs in FP32, or -- to measure what the
+// precision itself contributes -- the very same V-cycle in FP64 inside the
+// same outer loop.  Stopping rule: ||d||_2 <= tol * ||rho||_2.
 //
 // Scalars (alpha, beta, dot products) stay on the device; the host reads
-// one 8-byte norm per outer iteration.  The FP32 V-cycle and the per-
-// iteration kernel sequences are captured as hipGraphs.
+// one 8-byte norm per chunk of outer iterations.  The inner V-cycle and the
+// per-iteration kernel sequences are captured as hipGraphs.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -31,24 +37,36 @@
 
 using namespace gpaw;
 
+// One inner V-cycle hierarchy: its arrays and storage type.  The FP32
+// hierarchy owns every level; the FP64 one (used to attribute the speed-up:
+// the same outer loop with the inner V-cycle in FP64) owns level 0 and
+// reuses the plan's own arrays below it, which are scratch during a solve.
+struct VLevels {
+    int dt = DT_F32;
+    std::vector<void*> phiA, phiB, rho, res;
+    std::vector<void*> owned;
+    CoarsePlan* d_coarse = nullptr;     // fused coarse levels (FP64 inside)
+    bool own_coarse = false;
+    int lc = -1;                        // first fused coarse level (-1: none)
+};
+
 struct MixedState {
     int L = 0;
-    int lc = -1;                        // fused FP64 coarse level (-1: none)
     std::vector<int> pre, post;
-    std::vector<float*> phiA, phiB, rho, res;
-    CoarsePlan* d_coarse32 = nullptr;
+    VLevels v[2];                       // [0] FP32, [1] FP64 inner V-cycle
+    bool ready[2] = {false, false};
     double *p = nullptr, *q = nullptr;  // FP64 search direction, L p
     double* S = nullptr;                // device scalars
     double* h_S = nullptr;              // pinned readback
     double* partial = nullptr;
-    int nblk = 0;
-    hipGraphExec_t g_vcycle = nullptr;  // one FP32 V-cycle from zero
-    hipGraphExec_t g_vwarm = nullptr;   // one FP32 V-cycle from the last result
-    hipGraphExec_t g_a = nullptr;       // PCG: q = Lp, alpha, update, |d|^2
-    hipGraphExec_t g_b = nullptr;       // PCG: z = V32(d), beta, p = z + beta p
+    // graphs per inner precision
+    hipGraphExec_t g_vcycle[2] = {nullptr, nullptr};   // one V-cycle from zero
+    hipGraphExec_t g_vwarm[2] = {nullptr, nullptr};    // one V-cycle from the last result
+    hipGraphExec_t g_a[2] = {nullptr, nullptr};        // PCG: q = Lp, alpha, update, |d|^2
+    hipGraphExec_t g_b[2] = {nullptr, nullptr};        // PCG: z = V(d), beta, p = z + beta p
     // k PCG steps (b then a) per launch, k = 1 .. MIXED_CHUNK_MAX
-    std::vector<hipGraphExec_t> g_ba;
-    int tuned = 0;                      // launch shapes tuned (tuner on)
+    std::vector<hipGraphExec_t> g_ba[2];
+    int tuned[2] = {0, 0};              // launch shapes tuned (tuner on)
 };
 
 #define MIXED_CHUNK_MAX 8
@@ -95,14 +113,17 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ p, int
     return block_sum<256>(t, red);
 }
 
-// alpha = dz / pq ; phi -= alpha p ; d -= alpha q ; d32 = d ; partials of
-// |d|^2 into `partial`.  dz (the ndz partials at dzp) and pq (the npq
-// partials at pqp, or S[PQ] when npq = 0) are summed by every block itself;
-// block 0 keeps dz in S[DZ] (the next step's old dz) and alpha in S[ALPHA].
+// alpha = dz / pq ; phi -= alpha p ; d -= alpha q ; dlow = d (the inner
+// V-cycle's right-hand side in its own precision; nullptr when the inner
+// V-cycle reads d itself) ; partials of |d|^2 into `partial`.  dz (the ndz
+// partials at dzp) and pq (the npq partials at pqp, or S[PQ] when npq = 0)
+// are summed by every block itself; block 0 keeps dz in S[DZ] (the next
+// step's old dz) and alpha in S[ALPHA].
+template <class T>
 __global__ __launch_bounds__(256) void k_pcg_update(double* __restrict__ phi, double* __restrict__ d,
                                                     const double* __restrict__ p,
                                                     const double* __restrict__ q,
-                                                    float* __restrict__ d32, double* __restrict__ S,
+                                                    T* __restrict__ dlow, double* __restrict__ S,
                                                     i64 n, double* __restrict__ partial,
                                                     const double* __restrict__ pqp, int npq,
                                                     const double* __restrict__ dzp, int ndz)
@@ -116,7 +137,7 @@ __global__ __launch_bounds__(256) void k_pcg_update(double* __restrict__ phi, do
         phi[i] = phi[i] - alpha * p[i];
         const double di = d[i] - alpha * q[i];
         d[i] = di;
-        d32[i] = (float)di;
+        if (dlow) dlow[i] = (T)di;
         s += di * di;
     }
     s = block_sum<256>(s, red);
@@ -127,9 +148,10 @@ __global__ __launch_bounds__(256) void k_pcg_update(double* __restrict__ phi, do
     }
 }
 
-// partials of d.z and z.q (z from the FP32 preconditioner)
+// partials of d.z and z.q (z from the inner V-cycle)
+template <class T>
 __global__ __launch_bounds__(256) void k_dot_dz_zq(const double* __restrict__ d,
-                                                   const float* __restrict__ z,
+                                                   const T* __restrict__ z,
                                                    const double* __restrict__ q, i64 n,
                                                    double* __restrict__ partial)
 {
@@ -151,7 +173,8 @@ __global__ __launch_bounds__(256) void k_dot_dz_zq(const double* __restrict__ d,
 // p = z + beta p, beta = -alpha (z.q) / (d_old.z_old) (Polak-Ribiere), z.q
 // summed by every block from the nzq partials at zqp, d_old.z_old = S[DZ]
 // (kept by the previous update); first: p = z
-__global__ __launch_bounds__(256) void k_p_update(double* __restrict__ p, const float* __restrict__ z,
+template <class T>
+__global__ __launch_bounds__(256) void k_p_update(double* __restrict__ p, const T* __restrict__ z,
                                                   const double* __restrict__ S, i64 n, int first,
                                                   const double* __restrict__ zqp, int nzq)
 {
@@ -165,17 +188,19 @@ __global__ __launch_bounds__(256) void k_p_update(double* __restrict__ p, const 
 // The first Jacobi sweep from phi = 0: every stencil term is zero, so the
 // sweep is element-wise: the tile kernel's epilogue with a zero sum and a
 // zero centre value (same operations, same bits).
-__global__ __launch_bounds__(256) void k_jacobi_from_zero(float* __restrict__ out,
-                                                          const float* __restrict__ rho, i64 n,
+template <class T>
+__global__ __launch_bounds__(256) void k_jacobi_from_zero(T* __restrict__ out,
+                                                          const T* __restrict__ rho, i64 n,
                                                           double w, double c0)
 {
     for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256)
-        out[i] = st_epilogue<ST_JACOBI>(0.0f, 0.0f, rho[i], (float)w, (float)c0);
+        out[i] = st_epilogue<ST_JACOBI>(T(0), T(0), rho[i], (T)w, (T)c0);
 }
 
-// phi -= e (FP64 accumulation of an FP32 correction)
-__global__ __launch_bounds__(256) void k_sub_f32(double* __restrict__ phi, const float* __restrict__ e,
-                                                 i64 n)
+// phi -= e (FP64 accumulation of the inner V-cycle's correction)
+template <class T>
+__global__ __launch_bounds__(256) void k_sub_corr(double* __restrict__ phi, const T* __restrict__ e,
+                                                  i64 n)
 {
     for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256)
         phi[i] = phi[i] - (double)e[i];
@@ -189,8 +214,8 @@ dim3 gridn(i64 n)
 
 // `corr`: the first fused launch first applies phi -= I corr (the
 // coarse-grid correction, same values as the separate launch)
-hipError_t relax32(gpaw_poisson* ps, MixedState* m, int l, int k, float*& cur, float*& alt,
-                   hipStream_t st, const float* corr = nullptr, bool zero_in = false)
+hipError_t relax_v(gpaw_poisson* ps, VLevels& V, int l, int k, void*& cur, void*& alt,
+                   hipStream_t st, const void* corr = nullptr, bool zero_in = false)
 {
     Phases ph;
     gpaw_phases(nullptr, ph);
@@ -199,92 +224,95 @@ hipError_t relax32(gpaw_poisson* ps, MixedState* m, int l, int k, float*& cur, f
     // follow `cur`, not the sweep parity)
     if (l > 0 && ps->fuse_jacobi && jacobi_fused_ok(op->st, op->g)) {
         for (; k >= JF_SWEEPS; k -= JF_SWEEPS) {
-            hipError_t e = launch_jacobi_fused(op->st, DT_F32, cur, alt, m->rho[l], op->g,
-                                               ps->w[l], st, corr,
-                                               corr ? &ps->itp[l]->t : nullptr, zero_in);
+            hipError_t e = launch_jacobi_fused(op->st, V.dt, cur, alt, V.rho[l], op->g, ps->w[l],
+                                               st, corr, corr ? &ps->itp[l]->t : nullptr,
+                                               zero_in);
             corr = nullptr;
             zero_in = false;
             if (e != hipSuccess) return e;
-            float* t = cur;
-            cur = alt;
-            alt = t;
+            std::swap(cur, alt);
         }
     }
     for (int it = 0; it < k; it++) {
-        hipError_t e = launch_stencil(op->st, DT_F32, ST_JACOBI, cur, alt, m->rho[l], op->g, ph,
-                                      1, ps->w[l], nullptr, nullptr, true, st);
+        hipError_t e = launch_stencil(op->st, V.dt, ST_JACOBI, cur, alt, V.rho[l], op->g, ph, 1,
+                                      ps->w[l], nullptr, nullptr, true, st);
         if (e != hipSuccess) return e;
-        float* t = cur;
-        cur = alt;
-        alt = t;
+        std::swap(cur, alt);
     }
     return hipSuccess;
 }
 
-// one FP32 V-cycle on level l (level 0: rho = d32, result in phiA[0]);
-// `res` returns the buffer that holds level l's result
-hipError_t emit32(gpaw_poisson* ps, MixedState* m, int l, bool zero0, float*& res,
+// one inner V-cycle on level l (level 0: rho = the current residual, result
+// in phiA[0]); `res` returns the buffer that holds level l's result
+hipError_t emit_v(gpaw_poisson* ps, MixedState* m, VLevels& V, int l, bool zero0, void*& res,
                   hipStream_t st)
 {
     Phases ph;
     gpaw_phases(nullptr, ph);
-    if (l == m->lc) {   // the fused coarse kernel leaves it where single sweeps would
+    if (l == V.lc) {   // the fused coarse kernel leaves it where single sweeps would
         const int kc = (l < m->L ? m->pre[l] : 0) + m->post[l];
-        res = (kc % 2) ? m->phiB[l] : m->phiA[l];
-        return launch_coarse_vcycle_f32(m->d_coarse32, ps->coarse_lds, st);
+        res = (kc % 2) ? V.phiB[l] : V.phiA[l];
+        return V.dt == DT_F32 ? launch_coarse_vcycle_f32(V.d_coarse, ps->coarse_lds, st)
+                              : launch_coarse_vcycle(V.d_coarse, ps->coarse_lds, st);
     }
     gpaw_operator* op = ps->ops[l];
-    float* cur = m->phiA[l];
-    float* alt = m->phiB[l];
+    void* cur = V.phiA[l];
+    void* alt = V.phiB[l];
     hipError_t e = hipSuccess;
     int pre = l < m->L ? m->pre[l] : 0;
     if (l == 0 && zero0 && pre > 0 && ps->fuse_jacobi) {
         // the first sweep from zero, element-wise, into alt (as a sweep would)
-        launch_kernel(k_jacobi_from_zero, gridn(ps->npts[0]), dim3(256), 0, st, alt, m->rho[0],
-                      (i64)ps->npts[0], ps->w[0], op->st.coefs[0]);
+        if (V.dt == DT_F32)
+            launch_kernel(k_jacobi_from_zero<float>, gridn(ps->npts[0]), dim3(256), 0, st,
+                          (float*)alt, (const float*)V.rho[0], (i64)ps->npts[0], ps->w[0],
+                          op->st.coefs[0]);
+        else
+            launch_kernel(k_jacobi_from_zero<double>, gridn(ps->npts[0]), dim3(256), 0, st,
+                          (double*)alt, (const double*)V.rho[0], (i64)ps->npts[0], ps->w[0],
+                          op->st.coefs[0]);
         e = launch_error();
         std::swap(cur, alt);
         pre--;
     } else if (l == 0 && zero0) {
-        e = launch_fill(cur, DT_F32, ps->npts[0], st);
+        e = launch_fill(cur, V.dt, ps->npts[0], st);
     }
     if (e != hipSuccess) return e;
     if (l < m->L) {
         // below level 0, phi_l was zeroed by the restriction onto it
-        e = relax32(ps, m, l, pre, cur, alt, st, nullptr, l > 0);
+        e = relax_v(ps, V, l, pre, cur, alt, st, nullptr, l > 0);
         if (e != hipSuccess) return e;
-        e = launch_stencil(op->st, DT_F32, ST_RESID, cur, m->res[l], m->rho[l], op->g, ph, 1, 0.0,
+        e = launch_stencil(op->st, V.dt, ST_RESID, cur, V.res[l], V.rho[l], op->g, ph, 1, 0.0,
                            nullptr, nullptr, true, st);
         if (e != hipSuccess) return e;
-        e = launch_transfer(ps->rst[l]->t, DT_F32, TR_STORE, m->res[l], m->rho[l + 1],
-                            m->phiA[l + 1], ph, 1, st);
+        e = launch_transfer(ps->rst[l]->t, V.dt, TR_STORE, V.res[l], V.rho[l + 1],
+                            V.phiA[l + 1], ph, 1, st);
         if (e != hipSuccess) return e;
-        float* coarse = nullptr;
-        e = emit32(ps, m, l + 1, false, coarse, st);
+        void* coarse = nullptr;
+        e = emit_v(ps, m, V, l + 1, false, coarse, st);
         if (e != hipSuccess) return e;
         if (l > 0 && ps->fold_correction && ps->fuse_jacobi && m->post[l] >= JF_SWEEPS &&
             jacobi_fused_ok(op->st, op->g) && ps->itp[l]->t.K == 2) {
-            e = relax32(ps, m, l, m->post[l], cur, alt, st, coarse);
+            e = relax_v(ps, V, l, m->post[l], cur, alt, st, coarse);
             res = cur;
             return e;
         }
-        e = launch_transfer(ps->itp[l]->t, DT_F32, TR_SUBTRACT, coarse, cur, nullptr, ph, 1, st);
+        e = launch_transfer(ps->itp[l]->t, V.dt, TR_SUBTRACT, coarse, cur, nullptr, ph, 1, st);
         if (e != hipSuccess) return e;
     }
-    e = relax32(ps, m, l, m->post[l], cur, alt, st);
+    e = relax_v(ps, V, l, m->post[l], cur, alt, st);
     res = cur;
     return e;
 }
 
-// the whole FP32 V-cycle from level 0 (its result lands in phiA[0]: level 0
-// does an even number of single sweeps)
-hipError_t emit32_top(gpaw_poisson* ps, MixedState* m, bool zero0, hipStream_t st)
+// the whole inner V-cycle from level 0 (its result lands in phiA[0]: level
+// 0 does an even number of single sweeps)
+hipError_t emit_top(gpaw_poisson* ps, MixedState* m, int pr, bool zero0, hipStream_t st)
 {
-    float* res = nullptr;
-    return emit32(ps, m, 0, zero0, res, st);
+    void* res = nullptr;
+    return emit_v(ps, m, m->v[pr], 0, zero0, res, st);
 }
 
-float* z_of(MixedState* m) { return m->phiA[0]; }   // even level-0 sweep count
+void* z_of(MixedState* m, int pr) { return m->v[pr].phiA[0]; }   // even level-0 sweep count
 
 template <class F>
 int capture_graph(hipGraphExec_t* out, F&& body)
@@ -292,61 +320,119 @@ int capture_graph(hipGraphExec_t* out, F&& body)
     return gpaw_check(build_graph(out, body), "mixed: graph");
 }
 
-int setup(gpaw_poisson* ps)
+int alloc_bytes(void** p, size_t n, std::vector<void*>& owned, const char* what)
 {
-    if (ps->mixed) return GPAW_OK;
-    MixedState* m = new MixedState();
-    ps->mixed = m;
-    m->L = ps->L;
-    m->pre = ps->pre;
-    m->post = ps->post;
-    int s0 = 2, s1 = 2;
-    if (const char* env = getenv("GPAW_AMD_MIXED_SMOOTH")) sscanf(env, "%d,%d", &s0, &s1);
-    if ((s0 + s1) % 2) s1++;             // result must land in phiA[0]
-    m->pre[0] = s0;
-    m->post[0] = s1;
-    if (ps->L == 0) m->post[0] = s0 + s1;
+    int rc = gpaw_check(hipMalloc(p, n > 0 ? n : 1), what);
+    if (!rc) owned.push_back(*p);
+    return rc;
+}
+
+// The FP32 hierarchy: all levels in FP32; the fused coarse levels keep FP64
+// arrays in LDS with an FP32 boundary.
+int setup_v32(gpaw_poisson* ps, MixedState* m)
+{
+    VLevels& V = m->v[0];
+    V.dt = DT_F32;
     int rc = GPAW_OK;
     for (int l = 0; l <= ps->L && !rc; l++) {
-        float *a = nullptr, *b = nullptr, *r = nullptr, *x = nullptr;
-        const size_t n = sizeof(float) * (ps->npts[l] > 0 ? ps->npts[l] : 1);
-        rc = gpaw_check(hipMalloc((void**)&a, n), "mixed: level arrays");
-        if (!rc) rc = gpaw_check(hipMalloc((void**)&b, n), "mixed: level arrays");
-        if (!rc) rc = gpaw_check(hipMalloc((void**)&r, n), "mixed: level arrays");
-        if (!rc) rc = gpaw_check(hipMalloc((void**)&x, n), "mixed: level arrays");
-        m->phiA.push_back(a);
-        m->phiB.push_back(b);
-        m->rho.push_back(r);
-        m->res.push_back(x);
+        void *a = nullptr, *b = nullptr, *r = nullptr, *x = nullptr;
+        const size_t n = sizeof(float) * ps->npts[l];
+        rc = alloc_bytes(&a, n, V.owned, "mixed: level arrays");
+        if (!rc) rc = alloc_bytes(&b, n, V.owned, "mixed: level arrays");
+        if (!rc) rc = alloc_bytes(&r, n, V.owned, "mixed: level arrays");
+        if (!rc) rc = alloc_bytes(&x, n, V.owned, "mixed: level arrays");
+        V.phiA.push_back(a);
+        V.phiB.push_back(b);
+        V.rho.push_back(r);
+        V.res.push_back(x);
     }
-    const size_t n0 = sizeof(double) * ps->npts[0];
-    if (!rc) rc = gpaw_check(hipMalloc((void**)&m->p, n0), "mixed: p");
-    if (!rc) rc = gpaw_check(hipMalloc((void**)&m->q, n0), "mixed: q");
-    if (!rc) rc = gpaw_check(hipMalloc((void**)&m->S, sizeof(double) * S_N), "mixed: scalars");
-    if (!rc) rc = gpaw_check(hipHostMalloc((void**)&m->h_S, sizeof(double) * S_N), "mixed: pinned");
-    if (!rc) rc = gpaw_check(hipMalloc((void**)&m->partial, sizeof(double) * 4 * NB), "mixed: partials");
-    if (!rc) rc = gpaw_check(hipMemset(m->S, 0, sizeof(double) * S_N), "mixed: scalars");
     // the fused FP64 coarse solve needs its arrays in LDS (FP32 boundary)
     if (!rc && ps->lc > 0 && ps->h_coarse.lds) {
         CoarsePlan P = ps->h_coarse;
-        P.lev[0].rho = (double*)m->rho[ps->lc];
-        P.lev[0].phiA = (double*)m->phiA[ps->lc];
-        P.lev[0].phiB = (double*)m->phiB[ps->lc];
-        rc = gpaw_check(hipMalloc((void**)&m->d_coarse32, sizeof P), "mixed: coarse plan");
-        if (!rc) rc = gpaw_check(hipMemcpy(m->d_coarse32, &P, sizeof P, hipMemcpyHostToDevice),
-                                 "mixed: coarse plan");
-        if (!rc) m->lc = ps->lc;
+        P.lev[0].rho = (double*)V.rho[ps->lc];
+        P.lev[0].phiA = (double*)V.phiA[ps->lc];
+        P.lev[0].phiB = (double*)V.phiB[ps->lc];
+        rc = gpaw_check(hipMalloc((void**)&V.d_coarse, sizeof P), "mixed: coarse plan");
+        if (!rc) {
+            V.own_coarse = true;
+            rc = gpaw_check(hipMemcpy(V.d_coarse, &P, sizeof P, hipMemcpyHostToDevice),
+                            "mixed: coarse plan");
+        }
+        if (!rc) V.lc = ps->lc;
     }
     return rc;
 }
 
-void destroy_graphs(MixedState* m)
+// The FP64 hierarchy: level 0 reads the FP64 residual itself and sweeps
+// between two arrays of its own; the levels below reuse the plan's arrays
+// and its fused FP64 coarse kernel.
+int setup_v64(gpaw_poisson* ps, MixedState* m)
 {
-    for (hipGraphExec_t* g : {&m->g_vcycle, &m->g_vwarm, &m->g_a, &m->g_b}) {
+    VLevels& V = m->v[1];
+    V.dt = DT_F64;
+    const size_t n0 = sizeof(double) * ps->npts[0];
+    void *a = nullptr, *b = nullptr, *x = nullptr;
+    int rc = alloc_bytes(&a, n0, V.owned, "mixed: FP64 level-0 arrays");
+    if (!rc) rc = alloc_bytes(&b, n0, V.owned, "mixed: FP64 level-0 arrays");
+    if (!rc) rc = alloc_bytes(&x, n0, V.owned, "mixed: FP64 level-0 arrays");
+    if (rc) return rc;
+    V.phiA.push_back(a);
+    V.phiB.push_back(b);
+    V.rho.push_back(ps->res[0]);
+    V.res.push_back(x);
+    for (int l = 1; l <= ps->L; l++) {
+        V.phiA.push_back(ps->phiA[l]);
+        V.phiB.push_back(ps->phiB[l]);
+        V.rho.push_back(ps->rho[l]);
+        V.res.push_back(ps->res[l]);
+    }
+    if (ps->lc > 0) {
+        V.d_coarse = ps->d_coarse;
+        V.lc = ps->lc;
+    }
+    return GPAW_OK;
+}
+
+int setup(gpaw_poisson* ps, int pr)
+{
+    if (!ps->mixed) {
+        MixedState* m = new MixedState();
+        ps->mixed = m;
+        m->L = ps->L;
+        m->pre = ps->pre;
+        m->post = ps->post;
+        int s0 = 2, s1 = 2;
+        if (const char* env = getenv("GPAW_AMD_MIXED_SMOOTH")) sscanf(env, "%d,%d", &s0, &s1);
+        if ((s0 + s1) % 2) s1++;             // result must land in phiA[0]
+        m->pre[0] = s0;
+        m->post[0] = s1;
+        if (ps->L == 0) m->post[0] = s0 + s1;
+        const size_t n0 = sizeof(double) * ps->npts[0];
+        int rc = gpaw_check(hipMalloc((void**)&m->p, n0), "mixed: p");
+        if (!rc) rc = gpaw_check(hipMalloc((void**)&m->q, n0), "mixed: q");
+        if (!rc) rc = gpaw_check(hipMalloc((void**)&m->S, sizeof(double) * S_N), "mixed: scalars");
+        if (!rc)
+            rc = gpaw_check(hipHostMalloc((void**)&m->h_S, sizeof(double) * S_N), "mixed: pinned");
+        if (!rc)
+            rc = gpaw_check(hipMalloc((void**)&m->partial, sizeof(double) * 4 * NB),
+                            "mixed: partials");
+        if (!rc) rc = gpaw_check(hipMemset(m->S, 0, sizeof(double) * S_N), "mixed: scalars");
+        if (rc) return rc;
+    }
+    MixedState* m = ps->mixed;
+    if (m->ready[pr]) return GPAW_OK;
+    int rc = pr ? setup_v64(ps, m) : setup_v32(ps, m);
+    if (!rc) m->ready[pr] = true;
+    return rc;
+}
+
+void destroy_graphs(MixedState* m, int pr)
+{
+    for (hipGraphExec_t* g : {&m->g_vcycle[pr], &m->g_vwarm[pr], &m->g_a[pr], &m->g_b[pr]}) {
         if (*g) hipGraphExecDestroy(*g);
         *g = nullptr;
     }
-    for (hipGraphExec_t& g : m->g_ba) {
+    for (hipGraphExec_t& g : m->g_ba[pr]) {
         if (g) hipGraphExecDestroy(g);
         g = nullptr;
     }
@@ -355,7 +441,7 @@ void destroy_graphs(MixedState* m)
 // PCG step part a: q = L p, alpha, phi += alpha p, d -= alpha q, |d|^2.
 // |d|^2 (S[DD]) is only read by the host after a launch, so only the last
 // step of a launch sums its partials (`last`).
-hipError_t emit_a(gpaw_poisson* ps, MixedState* m, hipStream_t st, bool last = true)
+hipError_t emit_a(gpaw_poisson* ps, MixedState* m, int pr, hipStream_t st, bool last = true)
 {
     const i64 n = ps->npts[0];
     gpaw_operator* op = ps->ops[0];
@@ -380,54 +466,70 @@ hipError_t emit_a(gpaw_poisson* ps, MixedState* m, hipStream_t st, bool last = t
                       (int)gridn(n).x, m->S, (int)S_PQ);
     }
     // dz: the d.z partials of the step's k_dot_dz_zq
-    launch_kernel(k_pcg_update, gridn(n), dim3(256), 0, st, ps->phiA[0], ps->res[0], m->p, m->q,
-                  m->rho[0], m->S, n, m->partial + PART_DD, pqp, npq, (const double*)m->partial,
-                  (int)gridn(n).x);
+    if (pr == 0)
+        launch_kernel(k_pcg_update<float>, gridn(n), dim3(256), 0, st, ps->phiA[0], ps->res[0],
+                      m->p, m->q, (float*)m->v[0].rho[0], m->S, n, m->partial + PART_DD, pqp,
+                      npq, (const double*)m->partial, (int)gridn(n).x);
+    else
+        launch_kernel(k_pcg_update<double>, gridn(n), dim3(256), 0, st, ps->phiA[0], ps->res[0],
+                      m->p, m->q, (double*)nullptr, m->S, n, m->partial + PART_DD, pqp, npq,
+                      (const double*)m->partial, (int)gridn(n).x);
     if (last)
         launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial + PART_DD, (int)gridn(n).x,
                       m->S, (int)S_DD);
     return launch_error();
 }
 
-// PCG step part b: z = V32(d), beta (flexible), p = z + beta p
-hipError_t emit_b(gpaw_poisson* ps, MixedState* m, hipStream_t st)
+// z, beta (flexible), p = z + beta p after the inner V-cycle; first: p = z
+template <class T>
+void emit_dz_p(gpaw_poisson* ps, MixedState* m, int pr, int first, hipStream_t st)
 {
     const i64 n = ps->npts[0];
-    hipError_t e = emit32_top(ps, m, true, st);
+    launch_kernel(k_dot_dz_zq<T>, gridn(n), dim3(256), 0, st, ps->res[0],
+                  (const T*)z_of(m, pr), m->q, n, m->partial);
+    launch_kernel(k_p_update<T>, gridn(n), dim3(256), 0, st, m->p, (const T*)z_of(m, pr),
+                  (const double*)m->S, n, first, (const double*)m->partial + gridn(n).x,
+                  (int)gridn(n).x);
+}
+
+// PCG step part b: z = V(d), beta (flexible), p = z + beta p
+hipError_t emit_b(gpaw_poisson* ps, MixedState* m, int pr, hipStream_t st)
+{
+    hipError_t e = emit_top(ps, m, pr, true, st);
     if (e != hipSuccess) return e;
-    launch_kernel(k_dot_dz_zq, gridn(n), dim3(256), 0, st, ps->res[0], z_of(m), m->q, n,
-                  m->partial);
-    launch_kernel(k_p_update, gridn(n), dim3(256), 0, st, m->p, z_of(m), (const double*)m->S, n,
-                  0, (const double*)m->partial + gridn(n).x, (int)gridn(n).x);
+    if (pr == 0)
+        emit_dz_p<float>(ps, m, pr, 0, st);
+    else
+        emit_dz_p<double>(ps, m, pr, 0, st);
     return launch_error();
 }
 
-int capture_all(gpaw_poisson* ps)
+int capture_all(gpaw_poisson* ps, int pr)
 {
     MixedState* m = ps->mixed;
-    if (m->g_b) return GPAW_OK;   // the last one built: all four exist
-    int rc = capture_graph(&m->g_vcycle,
-                           [&](hipStream_t st) { return emit32_top(ps, m, true, st); });
+    if (m->g_b[pr]) return GPAW_OK;   // the last one built: all four exist
+    int rc = capture_graph(&m->g_vcycle[pr],
+                           [&](hipStream_t st) { return emit_top(ps, m, pr, true, st); });
     if (!rc)
-        rc = capture_graph(&m->g_vwarm,
-                           [&](hipStream_t st) { return emit32_top(ps, m, false, st); });
-    if (!rc) rc = capture_graph(&m->g_a, [&](hipStream_t st) { return emit_a(ps, m, st); });
-    if (!rc) rc = capture_graph(&m->g_b, [&](hipStream_t st) { return emit_b(ps, m, st); });
-    if (!rc) m->g_ba.assign(MIXED_CHUNK_MAX, nullptr);
-    if (rc) destroy_graphs(m);   // retried from scratch on the next call
+        rc = capture_graph(&m->g_vwarm[pr],
+                           [&](hipStream_t st) { return emit_top(ps, m, pr, false, st); });
+    if (!rc) rc = capture_graph(&m->g_a[pr], [&](hipStream_t st) { return emit_a(ps, m, pr, st); });
+    if (!rc) rc = capture_graph(&m->g_b[pr], [&](hipStream_t st) { return emit_b(ps, m, pr, st); });
+    if (!rc) m->g_ba[pr].assign(MIXED_CHUNK_MAX, nullptr);
+    if (rc) destroy_graphs(m, pr);   // retried from scratch on the next call
     return rc;
 }
 
 // k PCG steps (b, a) in one graph
-int run_ba(gpaw_poisson* ps, MixedState* m, int k, hipStream_t st)
+int run_ba(gpaw_poisson* ps, MixedState* m, int pr, int k, hipStream_t st)
 {
-    hipGraphExec_t& g = m->g_ba[k - 1];
+    hipGraphExec_t& g = m->g_ba[pr][k - 1];
     if (!g) {
         int rc = capture_graph(&g, [&](hipStream_t s) -> hipError_t {
             hipError_t e = hipSuccess;
             for (int i = 0; i < k && e == hipSuccess; i++) {
-                e = emit_b(ps, m, s);
-                if (e == hipSuccess) e = emit_a(ps, m, s, i == k - 1);
+                e = emit_b(ps, m, pr, s);
+                if (e == hipSuccess) e = emit_a(ps, m, pr, s, i == k - 1);
             }
             return e;
         });
@@ -445,8 +547,9 @@ int read_scalar(MixedState* m, int slot, hipStream_t st, double* v)
     return GPAW_OK;
 }
 
-// d = L phi - rho (FP64, stored in res[0]) with |d|^2 into S[DD]; d32 = d
-int residual64(gpaw_poisson* ps, MixedState* m, hipStream_t st)
+// d = L phi - rho (FP64, stored in res[0]) with |d|^2 into S[DD]; the FP32
+// inner V-cycle also gets d32 = d
+int residual64(gpaw_poisson* ps, MixedState* m, int pr, hipStream_t st)
 {
     gpaw_operator* op = ps->ops[0];
     Phases ph;
@@ -455,7 +558,8 @@ int residual64(gpaw_poisson* ps, MixedState* m, hipStream_t st)
     hipError_t e = launch_stencil(op->st, DT_F64, ST_RESID, ps->phiA[0], ps->res[0], ps->rho[0],
                                   op->g, ph, 1, 0.0, op->partial, &np, true, st);
     if (e == hipSuccess) e = launch_reduce_partials(op->partial, np, 1.0, m->S + S_DD, st);
-    if (e == hipSuccess) e = launch_cast(m->rho[0], DT_F32, ps->res[0], DT_F64, ps->npts[0], st);
+    if (e == hipSuccess && pr == 0)
+        e = launch_cast(m->v[0].rho[0], DT_F32, ps->res[0], DT_F64, ps->npts[0], st);
     return gpaw_check(e, "mixed: residual");
 }
 
@@ -464,15 +568,14 @@ int residual64(gpaw_poisson* ps, MixedState* m, hipStream_t st)
 void gpaw_mixed_free(MixedState* m)
 {
     if (!m) return;
-    for (hipGraphExec_t g : {m->g_vcycle, m->g_vwarm, m->g_a, m->g_b})
-        if (g) hipGraphExecDestroy(g);
-    for (auto v : {&m->phiA, &m->phiB, &m->rho, &m->res})
-        for (float* p : *v)
-            if (p) hipFree(p);
+    for (int pr = 0; pr < 2; pr++) {
+        destroy_graphs(m, pr);
+        for (void* p : m->v[pr].owned) hipFree(p);
+        if (m->v[pr].own_coarse && m->v[pr].d_coarse) hipFree(m->v[pr].d_coarse);
+    }
     for (double* p : {m->p, m->q, m->S, m->partial})
         if (p) hipFree(p);
     if (m->h_S) hipHostFree(m->h_S);
-    if (m->d_coarse32) hipFree(m->d_coarse32);
     delete m;
 }
 
@@ -480,61 +583,67 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
                                         double rel_tol, int maxiter, int mode, int inner,
                                         int* niter, double* relres, void* stream)
 {
-    if (!ps || !phi || !rho || maxiter < 1 || rel_tol <= 0.0 || (mode != 0 && mode != 1) ||
+    if (!ps || !phi || !rho || maxiter < 1 || rel_tol <= 0.0 || mode < 0 || mode > 3 ||
         inner < 1)
         return gpaw_fail(GPAW_EINVAL, "mixed: bad argument");
+    if (ps->method != 2) return gpaw_fail(GPAW_EUNSUPPORTED, "mixed: Jacobi smoothing only");
+    const int pcg = mode & 1;   // else iterative refinement
+    const int pr = mode >> 1;   // inner V-cycle precision: 0 FP32, 1 FP64
     hipStream_t st = (hipStream_t)stream;
-    int rc = setup(ps);
-    if (!rc && !ps->mixed->tuned && autotune_set(-1)) {
-        // one eager pass of the graph bodies on the plan's own buffers (all
-        // reloaded below), so that the graphs record tuned launch shapes
-        ps->mixed->tuned = 1;
-        hipError_t e = emit32_top(ps, ps->mixed, true, st);
-        if (e == hipSuccess) e = emit_a(ps, ps->mixed, st);
-        rc = gpaw_check(e, "mixed: tune");
-    }
-    if (!rc) rc = capture_all(ps);
+    int rc = setup(ps, pr);
     if (rc) return rc;
     MixedState* m = ps->mixed;
+    if (!m->tuned[pr] && autotune_set(-1)) {
+        // one eager pass of the graph bodies on the plan's own buffers (all
+        // reloaded below), so that the graphs record tuned launch shapes
+        m->tuned[pr] = 1;
+        hipError_t e = emit_top(ps, m, pr, true, st);
+        if (e == hipSuccess) e = emit_a(ps, m, pr, st);
+        rc = gpaw_check(e, "mixed: tune");
+    }
+    if (!rc) rc = capture_all(ps, pr);
+    if (rc) return rc;
     const i64 n = ps->npts[0];
     hipError_t e = launch_copy(ps->rho[0], rho, DT_F64, n, st);
     if (e == hipSuccess) e = launch_copy(ps->phiA[0], phi, DT_F64, n, st);
     if (e == hipSuccess) {   // |rho|^2 for the relative criterion
         launch_kernel(k_dot, gridn(n), dim3(256), 0, st, ps->rho[0], ps->rho[0], n, m->partial);
         launch_kernel(k_red1, dim3(1), dim3(256), 0, st, m->partial, (int)gridn(n).x, m->S,
-                           (int)S_RR);
+                      (int)S_RR);
         e = launch_error();
     }
     if (e != hipSuccess) return gpaw_check(e, "mixed: load");
     double rr = 0.0, dd = 0.0;
     // |rho|^2 and the first |d|^2 come back in one readback
-    rc = residual64(ps, m, st);
+    rc = residual64(ps, m, pr, st);
     if (!rc) rc = read_scalar(m, S_DD, st, &dd);
     if (rc) return rc;
     rr = m->h_S[S_RR];
     const double target = rel_tol * rel_tol * rr;
+    // loops continue while !(dd <= target): a NaN norm ends them and is
+    // reported below instead of passing for convergence
     int it = 0;
-    if (!rc && mode == 1 && dd > target) {
-        // z = V32(d); dz = d.z; p = z
-        rc = gpaw_check(hipGraphLaunch(m->g_vcycle, st), "mixed: V-cycle");
+    while (pcg && !rc && !(dd <= target) && std::isfinite(dd) && it < maxiter) {
+        // (re)start from the true residual d: z = V(d); dz = d.z; p = z
+        rc = gpaw_check(hipGraphLaunch(m->g_vcycle[pr], st), "mixed: V-cycle");
         if (!rc) {
-            launch_kernel(k_dot_dz_zq, gridn(n), dim3(256), 0, st, ps->res[0], z_of(m),
-                               ps->res[0], n, m->partial);
-            launch_kernel(k_p_update, gridn(n), dim3(256), 0, st, m->p, z_of(m),
-                          (const double*)m->S, n, 1, (const double*)nullptr, 0);
+            if (pr == 0)
+                emit_dz_p<float>(ps, m, pr, 1, st);
+            else
+                emit_dz_p<double>(ps, m, pr, 1, st);
             rc = gpaw_check(launch_error(), "mixed: start");
         }
         // PCG steps run in chunks of k (b, a) between readbacks of |d|^2; a
         // chunk may run a few steps past the tolerance, which only makes the
         // result more accurate.  k follows the observed convergence rate.
-        if (!rc) rc = gpaw_check(hipGraphLaunch(m->g_a, st), "mixed: iteration");
+        if (!rc) rc = gpaw_check(hipGraphLaunch(m->g_a[pr], st), "mixed: iteration");
         if (!rc) rc = read_scalar(m, S_DD, st, &dd);
-        it = 1;
+        it++;
         double prev = dd;
         int k = 2;
-        while (!rc && dd > target && it < maxiter) {
+        while (!rc && !(dd <= target) && std::isfinite(dd) && it < maxiter) {
             if (k > maxiter - it) k = maxiter - it;
-            rc = run_ba(ps, m, k, st);
+            rc = run_ba(ps, m, pr, k, st);
             if (!rc) rc = read_scalar(m, S_DD, st, &dd);
             it += k;
             // steps still needed at the rate of this chunk (|d|^2 per step)
@@ -549,27 +658,42 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
             }
             prev = dd;
         }
-    } else if (!rc && mode == 0) {
-        while (!rc && dd > target && it < maxiter) {
-            rc = gpaw_check(hipGraphLaunch(m->g_vcycle, st), "mixed: V-cycle");
-            for (int k = 1; k < inner && !rc; k++)
-                rc = gpaw_check(hipGraphLaunch(m->g_vwarm, st), "mixed: V-cycle");
-            if (!rc) {
-                launch_kernel(k_sub_f32, gridn(n), dim3(256), 0, st, ps->phiA[0], z_of(m), n);
-                rc = gpaw_check(launch_error(), "mixed: correction");
-            }
-            if (!rc) rc = residual64(ps, m, st);
+        // the recurrence's |d| can drift from the true residual: recompute it
+        // (L phi - rho in FP64); if it misses the target, restart from it
+        if (!rc && std::isfinite(dd)) {
+            rc = residual64(ps, m, pr, st);
             if (!rc) rc = read_scalar(m, S_DD, st, &dd);
-            it++;
         }
     }
+    while (!pcg && !rc && !(dd <= target) && std::isfinite(dd) && it < maxiter) {
+        rc = gpaw_check(hipGraphLaunch(m->g_vcycle[pr], st), "mixed: V-cycle");
+        for (int k = 1; k < inner && !rc; k++)
+            rc = gpaw_check(hipGraphLaunch(m->g_vwarm[pr], st), "mixed: V-cycle");
+        if (!rc) {
+            if (pr == 0)
+                launch_kernel(k_sub_corr<float>, gridn(n), dim3(256), 0, st, ps->phiA[0],
+                              (const float*)z_of(m, pr), n);
+            else
+                launch_kernel(k_sub_corr<double>, gridn(n), dim3(256), 0, st, ps->phiA[0],
+                              (const double*)z_of(m, pr), n);
+            rc = gpaw_check(launch_error(), "mixed: correction");
+        }
+        if (!rc) rc = residual64(ps, m, pr, st);
+        if (!rc) rc = read_scalar(m, S_DD, st, &dd);
+        it++;
+    }
     if (rc) return rc;
-    rc = gpaw_check(launch_copy(phi, ps->phiA[0], DT_F64, n, st), "mixed: store");
     if (niter) *niter = it;
     if (relres) *relres = rr > 0 ? sqrt(dd / rr) : 0.0;
+    if (!std::isfinite(dd))
+        return gpaw_fail(GPAW_EDEVICE, "mixed-precision Poisson solver broke down (residual "
+                                       "norm %g after %d iterations); phi not updated",
+                         dd, it);
+    rc = gpaw_check(launch_copy(phi, ps->phiA[0], DT_F64, n, st), "mixed: store");
     if (rc) return rc;
-    if (dd > target)
+    if (!(dd <= target))
         return gpaw_fail(GPAW_ENOCONV, "mixed-precision Poisson solver did not reach %g in %d "
-                                       "iterations", rel_tol, maxiter);
+                                       "iterations",
+                         rel_tol, maxiter);
     return GPAW_OK;
 }

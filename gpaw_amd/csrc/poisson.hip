@@ -196,8 +196,15 @@ int capture(gpaw_poisson* ps)
 // snap[i] before cycle i.
 bool chunk_rotates(const gpaw_poisson* ps)
 {
-    // Jacobi with level 0 swept by single launches (not the fused coarse kernel)
-    return ps->method == 2 && ps->lc != 0;
+    // Jacobi with level 0 swept by single launches (not the fused coarse
+    // kernel), and a first level-0 sweep that reads the cycle's input before
+    // anything else touches it: with levels below 0 that is a pre-smoothing
+    // sweep (without one, the coarse-grid correction would be subtracted in
+    // place from the previous cycle's kept result, and a fused norm would be
+    // summed from the corrected field); on a single level, a post-smoothing
+    // sweep.  Other schedules copy phi before each cycle instead.
+    if (ps->method != 2 || ps->lc == 0) return false;
+    return ps->L > 0 ? ps->pre[0] > 0 : ps->post[0] > 0;
 }
 
 hipError_t emit_chunk(gpaw_poisson* ps, int k, int state, hipStream_t st)

@@ -110,11 +110,13 @@ class OperatorHandle:
     def residual(self, phi, rho, r=None, norm_out=None, scale=1.0):
         """r = L phi - rho (and scale * sum r^2 into the 1-element tensor
         norm_out), fused."""
-        p = Arg(phi, name='phi')
-        q = Arg(rho, name='rho')
-        rr = Arg(r, write=True, name='r') if r is not None else None
-        nn = Arg(norm_out, write=True, name='norm') if norm_out is not None \
-            else None
+        F64 = _lib.GPAW_F64
+        p = Arg(phi, name='phi', shape=self.shape, code=F64)
+        q = Arg(rho, name='rho', shape=self.shape, code=F64)
+        rr = Arg(r, write=True, name='r', shape=self.shape, code=F64) \
+            if r is not None else None
+        nn = Arg(norm_out, write=True, name='norm', shape=(1,), code=F64) \
+            if norm_out is not None else None
         _lib.check(_lib.lib().gpaw_operator_residual(
             self.h, p.ptr, q.ptr, rr.ptr if rr else None,
             nn.ptr if nn else None, float(scale), _arrays.stream()))
@@ -223,8 +225,8 @@ def dots(a_g, b_xg, scale=1.0):
 
 def sub_scalar(y, x, b):
     """y = x - b on float64 device tensors."""
-    yy = Arg(y, write=True)
-    xx = Arg(x)
+    yy = Arg(y, write=True, name='y', code=_lib.GPAW_F64)
+    xx = Arg(x, name='x', shape=yy.shape, code=_lib.GPAW_F64)
     _lib.check(_lib.lib().gpaw_sub_scalar(yy.ptr, xx.ptr, float(b),
                                           int(np.prod(yy.shape)),
                                           _arrays.stream()))
@@ -233,8 +235,8 @@ def sub_scalar(y, x, b):
 
 def axpby(y, x, a, b):
     """y = a*y + b*x on float64 device tensors."""
-    yy = Arg(y, write=True)
-    xx = Arg(x)
+    yy = Arg(y, write=True, name='y', code=_lib.GPAW_F64)
+    xx = Arg(x, name='x', shape=yy.shape, code=_lib.GPAW_F64)
     _lib.check(_lib.lib().gpaw_axpby(yy.ptr, xx.ptr, float(a), float(b),
                                      int(np.prod(yy.shape)), _arrays.stream()))
     yy.finish()

@@ -146,8 +146,7 @@ class PoissonSolver:
               zero_initial_phi=False):
         if eps is None:
             eps = self.eps
-        P = Arg(phi, write=True, name='phi')
-        R = Arg(rho, name='rho')
+        P, R = self._args(phi, rho)
         phi_d, rho_d = P.tensor, R.tensor
         actual_charge = self.gd.integrate(rho_d)
         background = (actual_charge / self.gd.dv /
@@ -186,9 +185,16 @@ class PoissonSolver:
         P.finish()
         return niter
 
+    def _args(self, phi, rho):
+        """phi (written) and rho as device arguments; both must be float64
+        arrays of this grid's shape (the plan reads and writes exactly that
+        many points)."""
+        n = tuple(int(x) for x in self.gd.n_c)
+        return (Arg(phi, write=True, name='phi', shape=n, code=_lib.GPAW_F64),
+                Arg(rho, name='rho', shape=n, code=_lib.GPAW_F64))
+
     def solve_neutral(self, phi, rho, eps=2e-10):
-        P = Arg(phi, write=True, name='phi')
-        R = Arg(rho, name='rho')
+        P, R = self._args(phi, rho)
         if self.B is not None:
             # Mehrstellen: the right-hand side is B rho (gpaw/poisson.py:196-199)
             brho = _arrays.torch().empty_like(R.tensor)
@@ -218,8 +224,7 @@ class PoissonSolver:
         """One V-cycle on (phis[0], rhos[0]); returns dv * sum r^2."""
         assert level == 0
         errs = (ctypes.c_double * 1)()
-        P = Arg(self.phis[0], write=True)
-        R = Arg(self.rhos[0])
+        P, R = self._args(self.phis[0], self.rhos[0])
         _lib.check(_lib.lib().gpaw_poisson_vcycles(
             self._plan, P.ptr, R.ptr, 1, errs, _arrays.stream()))
         P.finish()
@@ -229,8 +234,7 @@ class PoissonSolver:
         """Exactly `count` V-cycles on an already neutral rho; returns the
         error after each cycle (fixed-cycle comparisons)."""
         errs = (ctypes.c_double * max(count, 1))()
-        P = Arg(phi, write=True)
-        R = Arg(rho)
+        P, R = self._args(phi, rho)
         _lib.check(_lib.lib().gpaw_poisson_vcycles(
             self._plan, P.ptr, R.ptr, int(count), errs, _arrays.stream()))
         P.finish()
@@ -257,36 +261,42 @@ class MixedPoissonSolver(PoissonSolver):
     Same hierarchy and charge handling as PoissonSolver (the charge
     integral, background and monopole / Madelung terms stay FP64).  The
     neutral problem is solved by an FP64 outer loop -- residual L phi - rho,
-    correction accumulation, dot products -- around FP32 V-cycles (FP32
+    correction accumulation, dot products -- around inner V-cycles (FP32
     smoothing, residuals, restriction, interpolation; the smallest levels in
     FP64).  outer='pcg': flexible preconditioned CG, one V-cycle per
     iteration; outer='refine': iterative refinement with `inner` V-cycles per
-    correction.  Converged when ||L phi - rho||_2 <= rel_tol * ||rho||_2 for
-    the neutral density; `eps` is accepted for signature compatibility but
-    the relative criterion is the one applied.
+    correction.  inner_precision='fp64' runs the same inner V-cycle in FP64
+    (same outer loop), which separates what FP32 contributes from what the
+    outer iteration does.  Converged when ||L phi - rho||_2 <= rel_tol *
+    ||rho||_2 for the neutral density (checked on the true residual); `eps`
+    is accepted for signature compatibility but the relative criterion is
+    the one applied.
     """
 
     def __init__(self, nn=3, relax='J', eps=2e-10, rel_tol=1e-10, outer='pcg',
-                 inner=1, maxiter=200):
+                 inner=1, maxiter=200, inner_precision='fp32'):
         PoissonSolver.__init__(self, nn, relax, eps)
         if outer not in ('pcg', 'refine'):
             raise ValueError('outer must be "pcg" or "refine"')
+        if inner_precision not in ('fp32', 'fp64'):
+            raise ValueError('inner_precision must be "fp32" or "fp64"')
         self.rel_tol = rel_tol
         self.outer = outer
         self.inner = inner
+        self.inner_precision = inner_precision
         self.maxiter = maxiter
 
     def solve_neutral(self, phi, rho, eps=None):
         if self.relax_method != 2 or self.B is not None:
             raise NotImplementedError('mixed precision needs Jacobi and a '
                                       'Laplace operator')
-        P = Arg(phi, write=True, name='phi')
-        R = Arg(rho, name='rho')
+        P, R = self._args(phi, rho)
         niter = ctypes.c_int(0)
         rel = ctypes.c_double(0.0)
         rc = _lib.lib().gpaw_poisson_solve_mixed(
             self._plan, P.ptr, R.ptr, float(self.rel_tol), int(self.maxiter),
-            1 if self.outer == 'pcg' else 0, int(self.inner),
+            (1 if self.outer == 'pcg' else 0) |
+            (2 if self.inner_precision == 'fp64' else 0), int(self.inner),
             ctypes.byref(niter), ctypes.byref(rel), _arrays.stream())
         self.relres = rel.value
         if rc == _lib.GPAW_ENOCONV:
@@ -303,5 +313,6 @@ class MixedPoissonSolver(PoissonSolver):
         return niter.value
 
     def __repr__(self):
-        return ('MixedPoissonSolver(nn=%r, outer=%r, rel_tol=%e)' %
-                (self.nn, self.outer, self.rel_tol))
+        return ('MixedPoissonSolver(nn=%r, outer=%r, rel_tol=%e, '
+                'inner_precision=%r)' % (self.nn, self.outer, self.rel_tol,
+                                         self.inner_precision))

@@ -6,27 +6,34 @@ with continuous batching on one GPU:
 * slots      S worker threads, each with its own HIP stream.  A request runs
              on a free slot; requests on different slots overlap on the
              device.  Library calls release the GIL (ctypes), so the host
-             side of one solve (one 8-byte readback per V-cycle) does not
-             stall the others.
-* residency  multigrid hierarchies (PoissonSolver plans) and Laplace
-             operators stay on the device between requests, keyed by
-             (kind, grid, boundary); a plan serves one request at a time and
-             more than one plan per key is built when requests of the same
-             key overlap.
+             side of one solve (one 8-byte readback per chunk of V-cycles)
+             does not stall the others.
+* residency  multigrid hierarchies (PoissonSolver plans) stay on the device
+             between requests, keyed by (grid edge, boundary); a plan serves
+             one request at a time and more than one plan per key is built
+             when requests of the same key overlap.  Laplace operators hold
+             no device memory and are kept per key.
 * admission  a request is admitted when a slot is free AND its device bytes
              (resident plan if not yet built + its arrays) fit the HBM budget
-             together with everything already resident or in flight; idle
-             plans are evicted least-recently-used to make room.  Requests
-             are admitted in arrival order (no starvation), so latency
-             includes queueing.
-* batching   Laplace requests of the same key that are waiting together are
-             fused into one launch over all their bands.
+             together with everything resident, in flight or held as
+             results; idle plans are evicted least-recently-used to make
+             room (the plan a request is about to reuse is never the one
+             evicted for it).  Requests are admitted in arrival order (no
+             starvation), so latency includes queueing.
+* results    a finished request's result stays charged to the budget until
+             the caller takes it: with `on_result` the server hands each
+             result to that callback as it completes and then releases it;
+             without, results stay charged until `take(request)`.
+* batching   Laplace requests waiting together -- of any grid, boundary and
+             band count, one dtype -- go to the GPU as one batched apply
+             (one persistent launch per stencil range, each request with its
+             own operator and output; results bit-identical to single
+             applies).
 
 Latency is measured from a request's arrival time to the moment its result
-is ready on the host; the server reports p50/p99 and throughput.
+is ready; the server reports p50/p99 and throughput.
 """
 import collections
-import heapq
 import threading
 import time
 
@@ -37,11 +44,12 @@ BOHR = 0.52917720859
 
 class Request:
     """kind: 'poisson' or 'laplace'; N: cubic grid edge; pbc: bool;
-    payload: density (poisson, host array of the grid's shape) or bands
-    (laplace, host array [nbands, *grid]); arrival: seconds on the
-    server's clock."""
+    payload: density (poisson, array of the grid's shape) or bands
+    (laplace, array [nbands, *grid]), host numpy or GPU tensor; arrival:
+    seconds on the server's clock."""
     __slots__ = ('rid', 'kind', 'N', 'pbc', 'payload', 'arrival', 'h',
-                 'result', 'done', 'niter', 'error', 'start', 'finish')
+                 'result', 'done', 'niter', 'error', 'start', 'finish',
+                 'held')
 
     def __init__(self, rid, kind, N, pbc, payload, arrival=0.0, h=0.2 / BOHR):
         self.rid = rid
@@ -56,6 +64,7 @@ class Request:
         self.error = None
         self.start = None
         self.finish = None
+        self.held = 0
 
     @property
     def key(self):
@@ -93,13 +102,25 @@ def _is_complex(a):
         return bool(np.iscomplexobj(a))
 
 
+def _bands(a):
+    return int(a.shape[0]) if len(a.shape) == 4 else 1
+
+
 def request_bytes(r):
     """Device bytes of a request's own arrays while it runs."""
     n = r.n_points()
     if r.kind == 'poisson':
         return 3 * 8 * n                      # rho, rho - background, phi
-    nb = r.payload.shape[0]
-    return 2 * (16 if _is_complex(r.payload) else 8) * n * nb
+    return 2 * (16 if _is_complex(r.payload) else 8) * n * _bands(r.payload)
+
+
+def result_bytes(r):
+    """Device bytes of a finished request's result (phi, or the output
+    bands)."""
+    n = r.n_points()
+    if r.kind == 'poisson':
+        return 8 * n
+    return (16 if _is_complex(r.payload) else 8) * n * _bands(r.payload)
 
 
 class Admission:
@@ -109,10 +130,13 @@ class Admission:
         self.budget = budget_bytes
         self.resident = collections.OrderedDict()   # plan id -> (key, bytes, busy)
         self.inflight = 0
+        self.results = 0                            # finished, not yet taken
         self.evicted = []                           # ids evicted, not yet dropped
+        self.n_evicted = 0
 
     def used(self):
-        return self.inflight + sum(b for _, b, _ in self.resident.values())
+        return (self.inflight + self.results +
+                sum(b for _, b, _ in self.resident.values()))
 
     def try_admit(self, need_plan, plan_key, plan_nbytes, req_nbytes):
         """Return True (and reserve) if the request fits, evicting idle
@@ -131,26 +155,37 @@ class Admission:
         self.resident[pid] = (key, nbytes, True)
 
     def use_plan(self, pid):
+        """Mark an idle plan busy (and most recently used)."""
         key, b, _ = self.resident.pop(pid)
-        self.resident[pid] = (key, b, True)     # most recently used
+        self.resident[pid] = (key, b, True)
 
-    def release(self, pid, req_nbytes):
+    def idle_plan(self, pid):
         key, b, _ = self.resident[pid]
         self.resident[pid] = (key, b, False)
+
+    def release(self, pid, req_nbytes):
+        if pid is not None:
+            self.idle_plan(pid)
         self.inflight -= req_nbytes
 
     def evict(self, pid):
         self.resident.pop(pid)
         self.evicted.append(pid)
+        self.n_evicted += 1
 
 
 class Server:
-    """Continuous-batching server.  `executor(plan, requests, stream)` runs
-    one batch on a slot and fills request.result; `make_plan(key)` builds a
-    resident plan.  Both default to the GPU implementation."""
+    """Continuous-batching server.
+
+    executor(plans, requests, slot) runs one batch on a slot and fills
+    request.result (plans[i] serves requests[i]); make_plan(key) builds a
+    resident plan.  Both default to the GPU implementation.  on_result(r),
+    if given, receives every finished request (result set) and the server
+    releases the result's budget charge once it returns."""
 
     def __init__(self, slots=4, hbm_budget=64 << 30, executor=None,
-                 make_plan=None, rel_tol=1e-10, mixed=True, clock=None):
+                 make_plan=None, rel_tol=1e-10, mixed=True, clock=None,
+                 max_fuse_bands=256, on_result=None):
         self.slots = slots
         self.adm = Admission(hbm_budget)
         self.rel_tol = rel_tol
@@ -158,9 +193,12 @@ class Server:
         self.executor = executor or self._gpu_execute
         self.make_plan = make_plan or self._gpu_plan
         self.clock = clock or time.perf_counter
+        self.max_fuse_bands = max_fuse_bands
+        self.on_result = on_result
         self.plans = {}              # pid -> plan object
         self.plan_key = {}           # pid -> key
         self.free_plans = collections.defaultdict(list)
+        self.laplace_plans = {}      # key -> plan (no device memory)
         self.next_pid = 0
         self.lock = threading.Condition()
         self.waiting = collections.deque()
@@ -168,6 +206,7 @@ class Server:
         self.busy_slots = 0
         self.t0 = None
         self._streams = None
+        self._batches = None
 
     # ------------------------------------------------------------ GPU parts
     def _gpu_plan(self, key):
@@ -187,7 +226,7 @@ class Server:
         return {'gd': gd, 'real': Laplace(gd, -0.5, 3, float),
                 'complex': Laplace(gd, -0.5, 3, complex)}
 
-    def _gpu_execute(self, plan, reqs, slot):
+    def _gpu_execute(self, plans, reqs, slot):
         """Payloads may be GPU tensors (served data already in HBM: no
         copies, the result stays on the device) or numpy arrays (copied in,
         result copied back)."""
@@ -196,7 +235,7 @@ class Server:
         with torch.cuda.stream(stream):
             on_dev = isinstance(reqs[0].payload, torch.Tensor)
             if reqs[0].kind == 'poisson':
-                r = reqs[0]
+                r, plan = reqs[0], plans[0]
                 rho = r.payload if on_dev else torch.from_numpy(r.payload).to('cuda')
                 phi = torch.zeros_like(rho)
                 if self.mixed:
@@ -206,28 +245,20 @@ class Server:
                     r.niter = plan.solve(phi, rho, eps=eps)
                 r.result = phi if on_dev else phi.cpu().numpy()
             else:
-                if len(reqs) == 1:
-                    a = reqs[0].payload if on_dev else torch.from_numpy(reqs[0].payload).to('cuda')
-                else:
-                    a = torch.cat([q.payload if on_dev else torch.from_numpy(q.payload)
-                                   for q in reqs]).to('cuda')
-                b = torch.empty_like(a)
-                cplx = a.is_complex()
-                op = plan['complex' if cplx else 'real']
-                op.apply(a, b, np.ones((3, 2), complex) if cplx else None)
-                k = 0
-                for q in reqs:
-                    nb = q.payload.shape[0]
-                    q.result = b[k:k + nb] if on_dev else b[k:k + nb].cpu().numpy()
-                    k += nb
+                groups = []
+                for q, plan in zip(reqs, plans):
+                    a = q.payload if on_dev else torch.from_numpy(q.payload).to('cuda')
+                    cplx = a.is_complex()
+                    groups.append((plan['complex' if cplx else 'real'], a,
+                                   torch.empty_like(a),
+                                   np.ones((3, 2), complex) if cplx else None))
+                self._batches[slot](groups)
+                for q, (_, _, b, _) in zip(reqs, groups):
+                    q.result = b if on_dev else b.cpu().numpy()
             stream.synchronize()
 
     # ------------------------------------------------------------- serving
-    def _get_plan(self, key):
-        if self.free_plans[key]:
-            pid = self.free_plans[key].pop()
-            self.adm.use_plan(pid)
-            return pid
+    def _new_plan(self, key):
         pid = self.next_pid
         self.next_pid += 1
         self.plans[pid] = None        # built by the worker, outside the lock
@@ -235,68 +266,132 @@ class Server:
         self.adm.add_plan(pid, key, plan_bytes(*key))
         return pid
 
+    def _fail_never_fits(self, r, nbytes):
+        r.error = ('needs %d device bytes, HBM budget is %d' %
+                   (nbytes, self.adm.budget))
+        r.finish = self.clock() - self.t0
+        self.waiting.popleft()
+        self.done.append(r)
+
     def _admit_locked(self):
         """Admit requests in arrival order while slots and memory allow."""
         batches = []
         while self.waiting and self.busy_slots + len(batches) < self.slots:
             r = self.waiting[0]
             key = r.key
-            need_plan = not self.free_plans[key]
-            group = [r]
-            if r.kind == 'laplace':     # fuse waiting applies of the same key
-                for q in list(self.waiting)[1:]:
-                    if q.key == key and _is_complex(q.payload) == \
-                            _is_complex(r.payload) and \
-                            sum(g.payload.shape[0] for g in group) + \
-                            q.payload.shape[0] <= 256:
-                        group.append(q)
-            nbytes = sum(request_bytes(q) for q in group)
-            if not self.adm.try_admit(need_plan, key, plan_bytes(*key), nbytes):
-                if self.busy_slots == 0 and not batches:
-                    # nothing runs and nothing can be evicted: it never fits
-                    r.error = ('needs %d device bytes, HBM budget is %d' %
-                               (nbytes + plan_bytes(*key), self.adm.budget))
-                    r.finish = self.clock() - self.t0
-                    self.waiting.popleft()
-                    self.done.append(r)
+            idle_now = self.busy_slots == 0 and not batches
+            if r.kind == 'poisson':
+                # reserve an idle plan of this key first, so that eviction
+                # for this very request cannot drop it
+                pid = self.free_plans[key].pop() if self.free_plans[key] else None
+                if pid is not None:
+                    self.adm.use_plan(pid)
+                nbytes = request_bytes(r)
+                if not self.adm.try_admit(pid is None, key, plan_bytes(*key), nbytes):
+                    if pid is not None:
+                        self.adm.idle_plan(pid)
+                        self.free_plans[key].append(pid)
+                    if idle_now and not self.adm.results:
+                        # nothing runs, nothing can be evicted: it never fits
+                        self._fail_never_fits(r, nbytes + plan_bytes(*key))
+                        continue
+                    break
+                self._drop_evicted()
+                self.waiting.popleft()
+                if pid is None:
+                    pid = self._new_plan(key)
+                batches.append(([pid], [r], nbytes))
+                continue
+            # Laplace: this request and the Laplace requests waiting behind
+            # it (any grid, one dtype) in one batched apply
+            cplx = _is_complex(r.payload)
+            group, nb = [r], _bands(r.payload)
+            for q in list(self.waiting)[1:]:
+                if q.kind == 'laplace' and _is_complex(q.payload) == cplx and \
+                        nb + _bands(q.payload) <= self.max_fuse_bands:
+                    group.append(q)
+                    nb += _bands(q.payload)
+            while group:
+                nbytes = sum(request_bytes(q) for q in group)
+                if self.adm.try_admit(False, key, 0, nbytes):
+                    break
+                group.pop()           # fewer fused requests may still fit
+            if not group:
+                if idle_now and not self.adm.results:
+                    self._fail_never_fits(r, request_bytes(r))
                     continue
                 break
-            for pid in self.adm.evicted:
-                self._drop_plan(pid)
-            self.adm.evicted.clear()
+            self._drop_evicted()
             for q in group:
                 self.waiting.remove(q)
-            pid = self._get_plan(key)
-            batches.append((pid, group, nbytes))
+            batches.append(([None] * len(group), group, nbytes))
         return batches
 
-    def _drop_plan(self, pid):
-        key = self.plan_key.pop(pid)
-        if pid in self.free_plans[key]:
-            self.free_plans[key].remove(pid)
-        self.plans.pop(pid, None)
+    def _drop_evicted(self):
+        for pid in self.adm.evicted:
+            key = self.plan_key.pop(pid)
+            if pid in self.free_plans[key]:
+                self.free_plans[key].remove(pid)
+            self.plans.pop(pid, None)
+        self.adm.evicted.clear()
 
-    def _worker(self, slot, pid, group, nbytes):
+    def _laplace_plan(self, key):
+        with self.lock:
+            plan = self.laplace_plans.get(key)
+        if plan is None:
+            plan = self.make_plan(key)
+            with self.lock:
+                plan = self.laplace_plans.setdefault(key, plan)
+        return plan
+
+    def _worker(self, slot, pids, group, nbytes):
         try:
-            if self.plans.get(pid) is None:
-                self.plans[pid] = self.make_plan(self.plan_key[pid])
+            plans = []
+            for pid, q in zip(pids, group):
+                if pid is None:
+                    plans.append(self._laplace_plan(q.key))
+                else:
+                    if self.plans.get(pid) is None:
+                        self.plans[pid] = self.make_plan(self.plan_key[pid])
+                    plans.append(self.plans[pid])
             now = self.clock() - self.t0
             for q in group:
                 q.start = now
-            self.executor(self.plans[pid], group, slot)
+            self.executor(plans, group, slot)
         except Exception as e:          # the request fails, the server runs on
             for q in group:
                 q.error = repr(e)
+                q.result = None
         fin = self.clock() - self.t0
         with self.lock:
             for q in group:
                 q.finish = fin
+                q.held = result_bytes(q) if q.error is None else 0
+                self.adm.results += q.held
                 self.done.append(q)
-            self.adm.release(pid, nbytes)
-            self.free_plans[self.plan_key[pid]].append(pid)
+            for pid in pids:
+                if pid is not None:
+                    self.adm.idle_plan(pid)
+                    self.free_plans[self.plan_key[pid]].append(pid)
+            self.adm.inflight -= nbytes
             self.busy_slots -= 1
             self.free_slot_ids.append(slot)
             self.lock.notify_all()
+        if self.on_result is not None:
+            for q in group:
+                try:
+                    self.on_result(q)
+                finally:
+                    self.take(q)
+
+    def take(self, r):
+        """Hand a finished request's result to the caller and release its
+        budget charge; returns the result."""
+        with self.lock:
+            self.adm.results -= r.held
+            r.held = 0
+            self.lock.notify_all()
+        return r.result
 
     def run(self, requests, realtime=True):
         """Serve `requests` (arrival times in seconds from start).  With
@@ -304,7 +399,10 @@ class Server:
         latencies then include queueing from t=0)."""
         import torch
         if self._streams is None and self.executor == self._gpu_execute:
+            from .batch import BatchApply
             self._streams = [torch.cuda.Stream() for _ in range(self.slots)]
+            self._batches = [BatchApply(max_groups=max(64, self.max_fuse_bands))
+                             for _ in range(self.slots)]
         pending = sorted(requests, key=lambda r: r.arrival)
         self.free_slot_ids = list(range(self.slots))
         self.t0 = self.clock()
@@ -317,13 +415,22 @@ class Server:
                                             pending[i].arrival <= now):
                     self.waiting.append(pending[i])
                     i += 1
-                for pid, group, nb in self._admit_locked():
+                for pids, group, nb in self._admit_locked():
                     slot = self.free_slot_ids.pop()
                     self.busy_slots += 1
                     t = threading.Thread(target=self._worker,
-                                         args=(slot, pid, group, nb))
+                                         args=(slot, pids, group, nb))
                     t.start()
                     threads.append(t)
+                if self.waiting and not self.busy_slots and self.adm.results \
+                        and self.on_result is None:
+                    # everything left is blocked by results nobody took
+                    for r in list(self.waiting):
+                        r.error = 'blocked: results held by the caller fill the HBM budget'
+                        r.finish = self.clock() - self.t0
+                        self.done.append(r)
+                    self.waiting.clear()
+                    continue
                 if i < len(pending) and realtime:
                     wait = pending[i].arrival - (self.clock() - self.t0)
                     self.lock.wait(timeout=max(0.0, min(wait, 0.05)))
@@ -346,6 +453,10 @@ class Server:
                 'span_s': float(span),
                 'requests_per_s': len(ok) / span if span > 0 else 0.0,
                 'poisson_solves_per_s': npois / span if span > 0 else 0.0}
+
+    def stats(self):
+        """Plans built and evicted, largest number resident so far."""
+        return {'plans_built': self.next_pid, 'plans_evicted': self.adm.n_evicted}
 
 
 def synthetic_stream(n=2000, rate=200.0, seed=0, edges=(48, 64, 80, 96, 128, 160),

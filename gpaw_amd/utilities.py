@@ -2,7 +2,9 @@
 
   coordinates        gpaw/utilities/tools.py:92-124
   Gaussian (L = 0)   gpaw/utilities/gauss.py:47-83 (gauss_L[0], gausspot_L[0])
-  madelung / Ewald   gpaw/utilities/ewald.py:12-150
+  madelung           the Madelung shift of gpaw/poisson.py:160-174 (the
+                     reference evaluates it with gpaw/utilities/ewald.py),
+                     summed here by Ewald's method to a stated tolerance
 These run once per solver on the host (numpy) and their results are moved
 to the device.
 """
@@ -54,73 +56,50 @@ class Gaussian:
         return 2.0 * 1.7724538509055159 * np.vectorize(erf)(np.sqrt(a) * r) / r
 
 
-class Ewald:
-    """Ewald sums for a lattice of point charges (gpaw/utilities/ewald.py)."""
-
-    def __init__(self, cell, G=5, Ng=(9, 9, 9), Nl=(3, 3, 3)):
-        self.cell = np.asarray(cell, float)
-        self.Vcell = abs(np.linalg.det(self.cell))
-        self.recip_cell = np.linalg.inv(self.cell).T
-        self.Ng = Ng
-        self.Nl = Nl
-        self.G = G / np.sqrt((self.cell ** 2).sum(1).max())
-
-    def get_sum_recip_ij(self, r_v, eps=1e-10):
-        N = self.Ng
-        b1, b2, b3 = self.recip_cell
-        E_g = 0.
-        for i in range(-N[0], N[0] + 1):
-            for j in range(-N[1], N[1] + 1):
-                for k in range(-N[2], N[2] + 1):
-                    g_v = 2 * pi * (i * b1 + j * b2 + k * b3)
-                    g2 = np.dot(g_v, g_v)
-                    if g2 > eps:
-                        x = .25 * g2 / self.G ** 2
-                        E_g += np.exp(1.j * np.dot(r_v, g_v) - x) / x
-        return pi / (self.Vcell * self.G ** 2) * E_g.real
-
-    def get_sum_real_ij(self, r_v, eps=1e-5):
-        N = self.Nl
-        a1, a2, a3 = self.cell
-        E_r = 0.
-        for i in range(-N[0], N[0] + 1):
-            for j in range(-N[1], N[1] + 1):
-                for k in range(-N[2], N[2] + 1):
-                    l_v = i * a1 + j * a2 + k * a3
-                    if np.linalg.norm(l_v) > eps:
-                        lr = np.linalg.norm(l_v - r_v)
-                        E_r += erfc(self.G * lr) / lr
-        return E_r
-
-    def get_electrostatic_potential(self, r, r_B, q_B, excludefroml0=None):
-        E0 = 0.
-        if excludefroml0 is None:
-            excludefroml0 = np.zeros(len(q_B), dtype=int)
-        if excludefroml0 in range(len(q_B)):
-            i = excludefroml0
-            excludefroml0 = np.zeros(len(q_B), dtype=int)
-            excludefroml0[i] = 1
-        assert sum(excludefroml0) <= 1
-        for i, q in enumerate(q_B):
-            rprime = r - r_B[i]
-            absr = np.linalg.norm(rprime)
-            E0 += q * self.get_sum_real_ij(rprime)
-            E0 += q * self.get_sum_recip_ij(rprime)
-            if excludefroml0[i]:
-                if absr < 1e-14:
-                    E0 -= q * 2. * self.G / np.sqrt(pi)
-                else:
-                    E0 -= q * erf(absr * self.G) / absr
-            else:
-                E0 += q * erfc(absr * self.G) / absr
-        q_hom = -sum(q_B)
-        E0 += q_hom * pi / (self.G ** 2 * self.Vcell)
-        return E0
+def _lattice_points(basis, radius):
+    """All integer combinations n . basis with |n . basis| <= radius (rows of
+    `basis` are the generating vectors), as an array of vectors."""
+    dual = np.linalg.inv(basis).T
+    nmax = [int(np.ceil(radius * np.linalg.norm(d))) for d in dual]
+    rng = [np.arange(-m, m + 1) for m in nmax]
+    n = np.stack(np.meshgrid(*rng, indexing='ij'), -1).reshape(-1, 3)
+    v = n @ basis
+    return v[(v ** 2).sum(1) <= radius * radius]
 
 
-def madelung(cell):
-    return Ewald(cell).get_electrostatic_potential(np.zeros(3), np.zeros(3),
-                                                   [-1], 0)
+def madelung(cell, alpha=None, tol=1e-16):
+    """Electrostatic potential at a lattice site of a periodic array of unit
+    NEGATIVE point charges in a uniform neutralising background, without the
+    site's own charge: the potential shift of a charged periodic system
+    (gpaw/poisson.py:160-174 multiplies it by the net charge).
+
+    Ewald's splitting of 1/r into erfc(alpha r)/r + erf(alpha r)/r: with
+    psi = sum_{L != 0} erfc(alpha|L|)/|L|
+          + (4 pi / V) sum_{g != 0} exp(-g^2 / 4 alpha^2) / g^2
+          - 2 alpha / sqrt(pi) - pi / (alpha^2 V)
+    the potential of unit positive charges at a site, the result is -psi.
+    Both sums are taken over every lattice / reciprocal vector whose term
+    exceeds `tol` relative to the leading one (erfc(x) < tol beyond
+    alpha|L| = x_cut, the Gaussian factor below tol beyond g = 2 alpha
+    x_cut), so the value does not depend on alpha.  alpha defaults to
+    sqrt(pi) / V^(1/3), which balances the two sums."""
+    cell = np.asarray(cell, float)
+    vol = abs(np.linalg.det(cell))
+    if alpha is None:
+        alpha = sqrt(pi) / vol ** (1.0 / 3.0)
+    x_cut = 1.0
+    while erfc(x_cut) > tol:
+        x_cut += 0.25
+    L = _lattice_points(cell, x_cut / alpha)
+    d = np.sqrt((L ** 2).sum(1))
+    d = d[d > 0]
+    real = sum(erfc(alpha * x) / x for x in np.sort(d)[::-1])
+    G = _lattice_points(2 * pi * np.linalg.inv(cell).T, 2 * alpha * x_cut)
+    g2 = (G ** 2).sum(1)
+    g2 = np.sort(g2[g2 > 0])[::-1]
+    recip = 4 * pi / vol * np.sum(np.exp(-g2 / (4 * alpha * alpha)) / g2)
+    psi = real + recip - 2 * alpha / sqrt(pi) - pi / (alpha * alpha * vol)
+    return -psi
 
 
 def gaussian_density(gd, centers, widths, charges, grid_normalized=True):

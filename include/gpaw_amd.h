@@ -140,12 +140,16 @@ int gpaw_poisson_vcycles(gpaw_poisson* ps, double* phi, const double* rho,
 /* Mixed-precision solve of the same problem (no reference counterpart: the
  * FP32 / FP64 split of the configuration "mixed-precision multigrid
  * Poisson").  FP64 outer loop (residual L phi - rho, correction
- * accumulation, dot products, convergence) around FP32 V-cycles
- * (smoothing, residuals, restriction, interpolation in FP32; the fused
- * coarsest levels in FP64).  mode 0: iterative refinement with `inner`
- * V-cycles per step; mode 1: flexible preconditioned CG with one V-cycle
- * per iteration.  Stops when ||L phi - rho||_2 <= rel_tol * ||rho||_2
- * (rho as given, already neutral); *relres receives the final ratio. */
+ * accumulation, dot products, convergence) around inner V-cycles
+ * (smoothing, residuals, restriction, interpolation; the fused coarsest
+ * levels in FP64).  mode bit 0: 0 = iterative refinement with `inner`
+ * V-cycles per step, 1 = flexible preconditioned CG with one V-cycle per
+ * iteration; mode bit 1: 0 = inner V-cycle in FP32, 1 = the same inner
+ * V-cycle in FP64 (for attributing the speed-up).  Stops when
+ * ||L phi - rho||_2 <= rel_tol * ||rho||_2 (rho as given, already
+ * neutral), checked on the true residual; *relres receives the final
+ * ratio.  A non-finite residual norm (breakdown) returns GPAW_EDEVICE and
+ * leaves phi unchanged. */
 int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi,
                              const double* rho, double rel_tol, int maxiter,
                              int mode, int inner, int* niter, double* relres,
@@ -200,7 +204,9 @@ void gpaw_preconditioner_free(gpaw_preconditioner* pc);
  * stencil range.  Group i applies ops[i] to nins[i] bands at ins[i] into
  * outs[i] with the phase_cd of phases[12*i .. 12*i+11] (host memory; NULL
  * = all phases 1).  Results are bit-identical to gpaw_operator_apply of
- * each group.  A batch object holds the descriptor buffers for up to
+ * each group.  All groups run concurrently, so no output may overlap any
+ * group's input or another output (GPAW_EINVAL otherwise); inputs may be
+ * shared.  A batch object holds the descriptor buffers for up to
  * max_groups groups; it may be reused, one call at a time. */
 typedef struct gpaw_batch gpaw_batch;
 

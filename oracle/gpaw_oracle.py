@@ -269,6 +269,89 @@ def gradient_stencil(grid, v, scale=1.0, n=1):
     return np.array(coefs), np.array(offsets, int)
 
 
+def _first_derivative_weights(n):
+    """Central first-derivative weights w_1..w_n of order 2n as exact
+    rationals: f'(0) ~ sum_k w_k (f(k) - f(-k)).  Solves the odd-moment
+    conditions sum_k 2 w_k k^(2j-1) = [j == 1] for j = 1..n."""
+    from fractions import Fraction
+    A = [[Fraction(2 * k ** (2 * j - 1)) for k in range(1, n + 1)]
+         for j in range(1, n + 1)]
+    b = [Fraction(int(j == 1)) for j in range(1, n + 1)]
+    # Gauss-Jordan elimination over the rationals
+    for c in range(n):
+        p = next(r for r in range(c, n) if A[r][c] != 0)
+        A[c], A[p] = A[p], A[c]
+        b[c], b[p] = b[p], b[c]
+        for r in range(n):
+            if r != c and A[r][c] != 0:
+                f = A[r][c] / A[c][c]
+                A[r] = [x - f * y for x, y in zip(A[r], A[c])]
+                b[r] -= f * b[c]
+    return [b[k] / A[k][k] for k in range(n)]
+
+
+def guc_stencil(grid, scale=1.0, n=1):
+    """GUCLaplace coefficients and offsets (gpaw/fd_operators.py:317-367).
+
+    In a cell with unit axis vectors u_c and dual vectors d_c (d_c . u_c' =
+    delta), nabla^2 = sum_cc' (d_c . d_c') d_s_c d_s_c', s_c the length along
+    axis c.  Pure terms use the order-2n second-difference table; each mixed
+    pair (c, c+1 mod 3) counts twice and uses the product of two order-2n
+    first differences, whose weights (1,1),(1,2),..,(n,n) are derived here
+    rather than tabulated.  Terms and offsets follow the reference's order,
+    which fixes the summation order of the kernel.
+    """
+    n = int(n)
+    w = _first_derivative_weights(n)
+    mixed = [float(w[i] * w[j]) for i in range(n) for j in range(i, n)]
+    hlen = np.sqrt((grid.h_cv ** 2).sum(1))
+    dual = grid.xxxiucell_cv
+    g_diag = (dual ** 2).sum(1)
+    offsets = [(0, 0, 0)]
+    coefs = [scale * np.sum(g_diag * np.divide(LAPLACE_TABLE[n][0], hlen ** 2))]
+    for k in range(1, n + 1):
+        ck = scale * g_diag * np.divide(LAPLACE_TABLE[n][k], hlen ** 2)
+        for ax in range(3):
+            for sgn in (-1, 1):
+                o = [0, 0, 0]
+                o[ax] = sgn * k
+                offsets.append(tuple(o))
+                coefs.append(ck[ax])
+    idx = 0
+    for i in range(1, n + 1):
+        for j in range(i, n + 1):
+            for ax in range(3):
+                nx = (ax + 1) % 3
+                cval = (scale * 2.0 * mixed[idx] * np.dot(dual[ax], dual[nx]) /
+                        (hlen[ax] * hlen[nx]))
+                if abs(cval) <= 1e-11:
+                    continue
+                # the four corners (+i,+j), (-i,+j), (+i,-j), (-i,-j) of the
+                # pair (ax, nx); the z-x pair stores its first step on x
+                for (si, sj), sg in (((1, 1), 1), ((-1, 1), -1), ((1, -1), -1),
+                                     ((-1, -1), 1)):
+                    o = [0, 0, 0]
+                    if ax == 2:
+                        o[2], o[0] = si * i, sj * j
+                    else:
+                        o[ax], o[nx] = si * i, sj * j
+                    offsets.append(tuple(o))
+                    coefs.append(sg * cval)
+                if j > i:
+                    # the mirror pair (j, i) on the same two axes
+                    for (si, sj), sg in (((1, 1), 1), ((-1, 1), -1),
+                                         ((1, -1), -1), ((-1, -1), 1)):
+                        o = [0, 0, 0]
+                        if ax == 2:
+                            o[0], o[2] = si * i, sj * j
+                        else:
+                            o[nx], o[ax] = si * i, sj * j
+                        offsets.append(tuple(o))
+                        coefs.append(sg * cval)
+            idx += 1
+    return np.array(coefs), np.array(offsets, int)
+
+
 def mehrstellen_a(grid, scale):
     """LaplaceA (gpaw/fd_operators.py:192-217)."""
     c = np.divide(-1 / 12, grid.h_cv.diagonal() ** 2) * scale

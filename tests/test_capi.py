@@ -111,12 +111,29 @@ def test_coefficients_match_oracle():
     assert np.array_equal(a.coef_p, c)
 
 
-def test_madelung_matches_oracle_and_constant():
+def test_madelung_matches_oracle_and_constants():
+    """The product's Ewald sum (its own summation to a tolerance) against the
+    oracle's fixed-truncation restatement on orthorhombic, hexagonal and
+    fcc cells; against the published Madelung constants of the Wigner
+    lattices (alpha_M / r_s, r_s the Wigner-Seitz radius: sc 1.7601189,
+    bcc 1.7918585, fcc 1.7917472); and independent of the Ewald
+    splitting parameter."""
     import oracle as O
     from gpaw_amd.utilities import madelung
-    for cell in (np.eye(3) * 5.0, np.diag([4.0, 5.0, 6.0])):
+    for cell in (np.eye(3) * 5.0, np.diag([4.0, 5.0, 6.0]),
+                 [[3.0, 0, 0], [1.5, 2.6, 0], [0, 0, 3.3]],
+                 [[0, 2.0, 2.0], [2.0, 0, 2.0], [2.0, 2.0, 0]]):
         assert abs(madelung(cell) - O.madelung(cell)) < 1e-12
     assert abs(madelung(np.eye(3) * 2.0) * 2.0 - 2.837297479480) < 1e-9
+    lattices = [(np.eye(3), 1.7601189),
+                (0.5 * np.array([[-1, 1, 1], [1, -1, 1], [1, 1, -1.0]]), 1.7918585),
+                (0.5 * np.array([[0, 1, 1], [1, 0, 1], [1, 1, 0.0]]), 1.7917472)]
+    for cell, aM in lattices:
+        for a in (1.0, 4.5):
+            c = a * cell
+            rs = (3 * abs(np.linalg.det(c)) / (4 * np.pi)) ** (1 / 3)
+            assert abs(madelung(c) * rs - aM) < 1e-7
+            assert abs(madelung(c, alpha=0.7 / a) - madelung(c, alpha=3.0 / a)) < 1e-12 / a
 
 
 def test_poisson_hierarchy_shape():

@@ -109,3 +109,50 @@ def test_autotuned_shapes_give_identical_results():
         print(rep, flush=True)
     finally:
         ops.autotune(prev)
+
+
+@pytest.mark.parametrize('dtype', [float, complex])
+def test_batch_equals_oracle(dtype):
+    """A batch of uneven grids, k-points, boundaries, spacings and stencil
+    ranges against the oracle's restatement of the reference's apply (its
+    own coefficients and phases per group): bit for bit."""
+    print('case batch vs oracle', dtype.__name__, flush=True)
+    torch = _torch()
+    import oracle as O
+    from gpaw_amd.batch import BatchApply
+    rng = np.random.default_rng(29)
+    groups = _groups(torch, dtype, rng)
+    batch = BatchApply(max_groups=16)
+    for _, _, b, _ in groups:
+        b.fill_(float('nan'))
+    batch(groups)
+    torch.cuda.synchronize()
+    for i, (op, a, b, ph) in enumerate(groups):
+        gd = op.gd
+        og = O.Grid(gd.N_c, gd.cell_cv, gd.pbc_c)
+        assert np.array_equal(np.asarray(op.coef_p), O.laplace_stencil(og, -0.5, op.n)[0])
+        ha = a.cpu().numpy()
+        ref = np.zeros_like(ha)
+        if ha.shape[0]:
+            O.Operator(op.coef_p, op.offset_pc, og, dtype).apply(ha, ref, ph)
+        assert np.array_equal(b.cpu().numpy(), ref), i
+
+
+def test_batch_refuses_overlapping_outputs():
+    """All groups run concurrently: an output overlapping another group's
+    input or output is refused (inputs may be shared)."""
+    torch = _torch()
+    from gpaw_amd import GpawAmdError
+    from gpaw_amd.batch import BatchApply
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.fd_operators import Laplace
+    gd = GridDescriptor((32, 32, 32), (6.0, 6.0, 6.0))
+    op = Laplace(gd, 1.0, 3)
+    a = torch.randn((2,) + tuple(gd.n_c), dtype=torch.float64, device='cuda')
+    b = torch.empty_like(a)
+    c = torch.empty_like(a)
+    BatchApply(4)([(op, a, b, None), (op, a, c, None)])     # shared input: fine
+    with pytest.raises(GpawAmdError):
+        BatchApply(4)([(op, a, b, None), (op, b, c, None)])  # b written and read
+    with pytest.raises(GpawAmdError):
+        BatchApply(4)([(op, a[0], b[0], None), (op, a[1], b[0], None)])

@@ -162,11 +162,15 @@ def test_generic_stencils(kind, dtype):
     elif kind == 'grad':
         c, o = O.gradient_stencil(og, 1, 1.0, 3)
     else:
+        # the device GUCLaplace (newgucstencil=False) against the oracle's
+        # own GUC construction: same coefficients and offsets, same bits
         from gpaw_amd.fd_operators import GUCLaplace
         g2 = _gd(N, [(3.0, 0, 0), (0.4, 3.0, 0), (0, 0, 3.0)], True)
         op = GUCLaplace(g2, 1.0, 2, dtype)
         og = O.Grid(N, g2.cell_cv, True)
-        c, o = op.coef_p, op.offset_pc
+        c, o = O.guc_stencil(og, 1.0, 2)
+        assert np.array_equal(np.asarray(op.coef_p), c)
+        assert np.array_equal(np.asarray(op.offset_pc), o)
         gd = g2
     a = rng.normal(size=(2,) + tuple(og.n_c))
     ph = None

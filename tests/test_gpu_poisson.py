@@ -287,3 +287,61 @@ def test_fused_jacobi_sweeps_identical(N, pbc, monkeypatch):
         assert n == out[0][0]
         assert torch.equal(phi, out[0][1])
         assert err == out[0][2]
+
+
+@pytest.mark.parametrize('pre0,post0', [(0, 1), (0, 2), (1, 0), (3, 2)])
+def test_other_level0_sweep_counts(pre0, post0):
+    """Level-0 sweep counts other than the reference's 2/1 (the plan takes
+    them as arguments): no pre-smoothing means the coarse-grid correction
+    meets the cycle's input, which must not be the kept result of the
+    previous cycle; the device must match the oracle V-cycle for V-cycle
+    and stop at the same cycle."""
+    print('case sweeps', pre0, post0, flush=True)
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.poisson import PoissonSolver
+    N, h = 32, 0.2 / BOHR
+    cell = (N * h,) * 3
+    og = O.Grid((N,) * 3, cell, False)
+    gd = GridDescriptor((N,) * 3, cell, False)
+    ops = O.PoissonSolver(nn=3, relax='J')
+    ops.set_grid_descriptor(og)
+    gps = PoissonSolver(nn=3, relax='J')
+    gps.set_grid_descriptor(gd)
+    for s in (ops, gps):
+        s.presmooths[0], s.postsmooths[0] = pre0, post0
+        s.initialize()
+    rho = _density(og, 3, 4, 0.5 / BOHR, 1.5 / BOHR, 2.0 / BOHR)
+    rho -= og.integrate(rho) / og.dv / og.size_of_global_array().prod()
+    phi_ref = og.zeros()
+    errs_ref = ops.run_vcycles(phi_ref, rho.copy(), 3)
+    phi = dev(np.zeros(tuple(og.n_c)))
+    errs = gps.vcycles(phi, dev(rho), 3)
+    assert np.array_equal(phi.cpu().numpy(), phi_ref)
+    assert np.allclose(errs, errs_ref, rtol=1e-12, atol=0)
+    phi_ref = og.zeros()
+    n_ref = ops.solve_neutral(phi_ref, rho.copy(), eps=1e-12)
+    phi = dev(np.zeros(tuple(og.n_c)))
+    n = gps.solve_neutral(phi, dev(rho), eps=1e-12)
+    assert n == n_ref
+    assert np.array_equal(phi.cpu().numpy(), phi_ref)
+
+
+def test_argument_shape_and_dtype_checked():
+    """Arrays of the wrong shape or dtype are refused before any device
+    call (the plan reads and writes exactly the grid's points)."""
+    print('case arg checks', flush=True)
+    t = _torch()
+    og, gd, ops, gps = _pair(16, 0.3, False)
+    n = tuple(og.n_c)
+    rho = t.zeros(n, dtype=t.float64, device='cuda')
+    with pytest.raises(ValueError):
+        gps.solve_neutral(t.zeros((16, 16, 16), dtype=t.float64, device='cuda'), rho)
+    with pytest.raises(TypeError):
+        gps.solve_neutral(t.zeros(n, dtype=t.float32, device='cuda'), rho)
+    with pytest.raises(ValueError):
+        gps.solve(t.zeros(n, dtype=t.float64, device='cuda'), rho[:-1].contiguous())
+    from gpaw_amd import ops as gops
+    with pytest.raises(ValueError):
+        gops.sub_scalar(rho, t.zeros(5, dtype=t.float64, device='cuda'), 1.0)
+    with pytest.raises(TypeError):
+        gops.axpby(rho, rho.float(), 1.0, 1.0)

@@ -42,7 +42,7 @@ CELLS = [
 
 @pytest.mark.parametrize('name,D,cell', CELLS, ids=[c[0] for c in CELLS])
 def test_laplace_gradient_polynomial_exactness(name, D, cell):
-    for n in range(1, 4):
+    for n in range(1, 6):
         N = 2 * n + 2
         g = O.Grid((N, N, N), cell)
         b = g.zeros()
@@ -184,3 +184,137 @@ def test_poisson_residual_decreases():
     ps.initialize()
     errs = ps.run_vcycles(g.zeros(), rho, 6)
     assert all(b < 0.5 * a for a, b in zip(errs, errs[1:]))
+
+
+def _guc_poly_error(c, o, g, n):
+    """Largest deviation from the exact Laplacian at the central point over
+    every monomial of degree <= 2n (gpaw/test/laplace.py's check)."""
+    lap = Operator(c, o, g)
+    r_gv = g.grid_point_coordinates().transpose((1, 2, 3, 0)) - g.cell_cv.sum(0) / 2
+    b = g.zeros()
+    worst = 0.0
+    for m in range(0, 2 * n + 1):
+        for ix in range(m + 1):
+            for iy in range(m - ix + 1):
+                iz = m - ix - iy
+                a = np.ascontiguousarray((r_gv ** (ix, iy, iz)).prod(3))
+                r = 2.0 if (m == 2 and max(ix, iy, iz) == 2) else 0.0
+                lap.apply(a, b)
+                worst = max(worst, abs(b[n + 1, n + 1, n + 1] - r))
+    return worst
+
+
+@pytest.mark.parametrize('name,D,cell', CELLS, ids=[c[0] for c in CELLS])
+def test_guc_laplace_polynomial_exactness(name, D, cell):
+    """GUCLaplace (the newgucstencil=False Laplacian, gpaw/fd_operators.py:
+    317-367) is exact on every monomial of degree <= 2n, n = 1..5, on the
+    eleven cells of gpaw/test/laplace.py; its mixed-derivative weights are
+    derived in the oracle from first-difference weights, not tabulated."""
+    for n in range(1, 6):
+        g = O.Grid((2 * n + 2,) * 3, cell)
+        c, o = O.guc_stencil(g, 1.0, n)
+        assert _guc_poly_error(c, o, g, n) < 2e-12, (name, n)
+
+
+def test_guc_laplace_planted_mistakes_are_caught():
+    """A dropped mirror pair, a mixed term without its factor 2, and a
+    flipped corner sign each break the exactness check."""
+    g = O.Grid((8, 8, 8), CELLS[0][2])
+    n = 3
+    c, o = O.guc_stencil(g, 1.0, n)
+    assert _guc_poly_error(c, o, g, n) < 2e-12
+    mixed = np.flatnonzero((o != 0).sum(1) == 2)
+    assert len(mixed) > 0
+    # mixed term halved
+    c2 = c.copy()
+    c2[mixed] *= 0.5
+    assert _guc_poly_error(c2, o, g, n) > 1e-3
+    # one corner's sign flipped
+    c3 = c.copy()
+    c3[mixed[0]] *= -1
+    assert _guc_poly_error(c3, o, g, n) > 1e-3
+    # the (j, i) mirror of an (i, j) pair dropped: offsets whose two nonzero
+    # components differ in magnitude, taken where the larger one is first
+    drop = [p for p in mixed
+            if abs(o[p][o[p] != 0][0]) > abs(o[p][o[p] != 0][1])]
+    assert drop
+    keep = np.setdiff1d(np.arange(len(c)), drop)
+    assert _guc_poly_error(c[keep], o[keep], g, n) > 1e-3
+
+
+def test_first_derivative_weights_closed_forms():
+    from fractions import Fraction as F
+    w = O.gpaw_oracle._first_derivative_weights
+    assert w(1) == [F(1, 2)]
+    assert w(2) == [F(2, 3), F(-1, 12)]
+    assert w(3) == [F(3, 4), F(-3, 20), F(1, 60)]
+
+
+@pytest.mark.parametrize('cell', [[3.0, 3.0, 3.0], [3.0, 3.3, 2.7]])
+def test_mehrstellen_identity(cell):
+    """LaplaceA / LaplaceB (gpaw/fd_operators.py:192-229) form the compact
+    fourth-order (Collatz "Mehrstellen") pair: A p = -scale * B (nabla^2 p)
+    holds exactly for every polynomial of degree <= 5 (the truncation errors
+    of both sides agree through h^2 and differ first in sixth derivatives),
+    and fails at degree 6, which shows the check has teeth."""
+    g = O.Grid((10, 11, 9), cell, False)
+    r = g.grid_point_coordinates() - np.array(cell)[:, None, None, None] / 2
+    x, y, z = r
+    inner = (slice(1, -1),) * 3
+    for scale in (1.0, 0.25 / np.pi):
+        A = Operator(*O.mehrstellen_a(g, scale), g)
+        B = Operator(*O.mehrstellen_b(), g)
+        for deg in range(0, 7):
+            worst = 0.0
+            for ix in range(deg + 1):
+                for iy in range(deg - ix + 1):
+                    iz = deg - ix - iy
+                    p = np.ascontiguousarray(x ** ix * y ** iy * z ** iz)
+                    lap = np.zeros_like(p)
+                    if ix >= 2:
+                        lap += ix * (ix - 1) * x ** (ix - 2) * y ** iy * z ** iz
+                    if iy >= 2:
+                        lap += iy * (iy - 1) * x ** ix * y ** (iy - 2) * z ** iz
+                    if iz >= 2:
+                        lap += iz * (iz - 1) * x ** ix * y ** iy * z ** (iz - 2)
+                    Ap = g.zeros()
+                    A.apply(p, Ap)
+                    Bl = g.zeros()
+                    B.apply(np.ascontiguousarray(lap), Bl)
+                    ref = scale * Bl[inner]
+                    worst = max(worst, np.abs(Ap[inner] + ref).max() /
+                                max(1.0, np.abs(ref).max()))
+            if deg <= 5:
+                assert worst < 1e-13, (scale, deg, worst)
+            else:
+                assert worst > 1e-4, (scale, deg, worst)
+
+
+def test_mehrstellen_planted_mistakes_are_caught():
+    """A wrong face weight or edge weight in A, or a wrong centre weight in
+    B, breaks the degree-4 identity."""
+    cell = [3.0, 3.3, 2.7]
+    g = O.Grid((10, 11, 9), cell, False)
+    r = g.grid_point_coordinates() - np.array(cell)[:, None, None, None] / 2
+    x, y, z = r
+    inner = (slice(1, -1),) * 3
+    p = np.ascontiguousarray(x ** 2 * y ** 2 + 0.3 * z ** 4 + x * y * z)
+    lap = np.ascontiguousarray(2 * y ** 2 + 2 * x ** 2 + 3.6 * z ** 2)
+    ca, oa = O.mehrstellen_a(g, 1.0)
+    cb, ob = O.mehrstellen_b()
+
+    def miss(ca, cb):
+        Ap = g.zeros()
+        Operator(ca, oa, g).apply(p, Ap)
+        Bl = g.zeros()
+        Operator(cb, ob, g).apply(lap, Bl)
+        return np.abs(Ap[inner] + Bl[inner]).max()
+
+    assert miss(ca, cb) < 1e-12
+    for k in (1, 7):     # a face weight, an edge weight
+        c2 = ca.copy()
+        c2[k] *= 1.01
+        assert miss(c2, cb) > 1e-4
+    c3 = cb.copy()
+    c3[0] = 0.6
+    assert miss(ca, c3) > 1e-4

@@ -106,7 +106,7 @@ def _torch():
 @pytest.mark.gpu
 def test_laplace_gradient_polynomial_exactness_device():
     """gpaw/test/laplace.py through the device Laplace / Gradient: on the
-    eleven cells, every monomial of degree <= 2n has the exact Laplacian
+    eleven cells and n = 1..5, every monomial of degree <= 2n has the exact Laplacian
     (2 for x^2-type terms, else 0) and gradient at the central point, to
     2e-12."""
     print('case laplace.py', flush=True)
@@ -115,7 +115,7 @@ def test_laplace_gradient_polynomial_exactness_device():
     from gpaw_amd.fd_operators import Gradient, Laplace
     from gpaw_amd.grid_descriptor import GridDescriptor
     for name, D, cell in POLY_CELLS:
-        for n in range(1, 4):
+        for n in range(1, 6):
             Nn = 2 * n + 2
             g = O.Grid((Nn,) * 3, cell)
             gd = GridDescriptor((Nn,) * 3, cell)
