@@ -551,8 +551,20 @@ TuneVal tune(const TuneKey& key, const LaunchCfg& heuristic, const Stencil& s, i
         hipEventDestroy(e0);
         return best;
     }
+    // GPAW_AMD_TUNE_LOG=1: every candidate's time on stderr
+    static const bool log = [] {
+        const char* e = getenv("GPAW_AMD_TUNE_LOG");
+        return e && atoi(e) > 0;
+    }();
     for (const LaunchCfg& c : cand) {
         const float us = time_cfg(c, s, dtype, mode, in, out, aux, g, ph, nin, w, write, st, e0, e1);
+        if (log)
+            fprintf(stderr,
+                    "tune dtype=%d mode=%d R=%d n=%dx%dx%d nin=%d: %s TZ=%d q=%d threads=%d "
+                    "%.2f us\n",
+                    dtype, mode, s.star_r, g.n[0], g.n[1], g.n[2], nin,
+                    c.kind == K_TILE ? "tile" : "queue", c.TZ, c.kind == K_TILE ? c.sh.q : 0,
+                    c.kind == K_TILE ? c.sh.nt : 256, us);
         if (us < 0) continue;
         best.candidates++;
         if (best.us < 0 || us < best.us) {

@@ -341,7 +341,13 @@ __global__ __launch_bounds__(NT) void k_star_tile(const V* __restrict__ in, V* _
 // occupancy gained outweighs the extra L2 reads of the taller halo: 1 row
 // measured 4.1 / 3.6 TB/s on 64^3 float64 / complex128 bands against
 // 2.7-3.3 / 3.0-3.2 TB/s for 2-4 rows.
-constexpr int tile_rows(int, int, int = TILE_NT) { return 1; }
+#ifndef GPAW_TILE_ROWS_F32
+#define GPAW_TILE_ROWS_F32 1
+#endif
+constexpr int tile_rows(int, int esize, int nt = TILE_NT)
+{
+    return (esize == 4 && nt == TILE_NT) ? GPAW_TILE_ROWS_F32 : 1;
+}
 
 // host-side shape selection and launch (stencil_tile_*.hip)
 int tile_tz(int n2);

@@ -9,7 +9,9 @@ points.  This module restates the Python layers that drive them:
                           gpaw/domain.py:15-150
 * stencil coefficients    gpaw/fd_operators.py:20-66 (FDOperator),
                           :123-144 (Gradient), :192-229 (LaplaceA/B),
-                          :231-292 (NewGUCLaplace and its table)
+                          :231-292 (NewGUCLaplace and its table),
+                          :317-367 (GUCLaplace; its mixed-derivative
+                          weights derived here, not tabulated)
 * transformer paddings    gpaw/transformers.py:19-62
 * multigrid Poisson       gpaw/poisson.py:21-276
 * charged-system helpers  gpaw/utilities/gauss.py:47-83,
