@@ -56,6 +56,7 @@ PROTOTYPES = {
     'gpaw_batch_free': (None, [_vp]),
     'gpaw_autotune': (_i, [_i]),
     'gpaw_autotune_report': (_l, [ctypes.c_char_p, _l]),
+    'gpaw_stencil_kernel': (_i, [_i]),
     'gpaw_sum': (_i, [_vp, _l, _d, _vp, _vp]),
     'gpaw_dots': (_i, [_vp, _vp, _l, _l, _d, _vp, _vp]),
     'gpaw_axpby': (_i, [_vp, _vp, _d, _d, _l, _vp]),

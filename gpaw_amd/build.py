@@ -18,7 +18,8 @@ LIB = os.path.join(HERE, 'libgpaw_amd.so')
 ARCH = os.environ.get('GPAW_AMD_ARCH', 'gfx950')
 
 SOURCES = ['stencil.hip', 'stencil_tile_f64.hip', 'stencil_tile_c128.hip',
-           'stencil_tile_f32.hip', 'stencil_tile_c64.hip', 'transfer.hip',
+           'stencil_tile_f32.hip', 'stencil_tile_c64.hip', 'stencil_box_f64.hip',
+           'stencil_box_c128.hip', 'stencil_box_f32.hip', 'stencil_box_c64.hip', 'transfer.hip',
            'transfer_f64.hip', 'transfer_c128.hip', 'transfer_f32.hip',
            'transfer_c64.hip',
            'misc.hip', 'capi.hip', 'poisson.hip', 'coarse.hip', 'mixed.hip', 'gs.hip',

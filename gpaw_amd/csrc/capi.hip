@@ -493,3 +493,13 @@ extern "C" int gpaw_scale(double* y, double a, long n, void* stream)
 extern "C" int gpaw_autotune(int enable) { return autotune_set(enable); }
 
 extern "C" long gpaw_autotune_report(char* buf, long len) { return autotune_report(buf, len); }
+
+extern "C" int gpaw_stencil_kernel(int kind)
+{
+    // header numbering: 0 default, 1 tile, 2 box, 3 queue, 4 generic
+    static const int to_internal[5] = {-1, 2, 3, 1, 0};
+    static const int to_header[5] = {4, 3, 1, 2, 0};   // internal 0..3 -> header
+    if (kind < -1 || kind > 4) return -1;
+    const int prev = stencil_kernel_set(kind < 0 ? -2 : to_internal[kind]);
+    return prev < 0 ? 0 : to_header[prev];
+}

@@ -90,6 +90,9 @@ struct Stencil {
 // query (-1); returns the previous state.  autotune_report writes a text
 // table of the choices made.
 int autotune_set(int on);
+// Kernel family of star-stencil launches without partial sums: -1 default,
+// 0 generic, 1 register queue, 2 tile, 3 box; returns the previous value.
+int stencil_kernel_set(int kind);
 long autotune_report(char* buf, long len);
 
 // Work estimate used to size partial-sum buffers.

@@ -28,9 +28,19 @@
 #include <vector>
 
 #include "kernels.h"
+#include "stencil_box.h"
 #include "stencil_tile.h"
 
 namespace gpaw {
+
+hipError_t box_launch_f64(const Stencil&, int, const void*, void*, const void*, const Geom&,
+                          const Phases&, double, const TileShape&, double*, bool, hipStream_t);
+hipError_t box_launch_c128(const Stencil&, int, const void*, void*, const void*, const Geom&,
+                           const Phases&, double, const TileShape&, double*, bool, hipStream_t);
+hipError_t box_launch_f32(const Stencil&, int, const void*, void*, const void*, const Geom&,
+                          const Phases&, double, const TileShape&, double*, bool, hipStream_t);
+hipError_t box_launch_c64(const Stencil&, int, const void*, void*, const void*, const Geom&,
+                          const Phases&, double, const TileShape&, double*, bool, hipStream_t);
 
 hipError_t tile_launch_f64(const Stencil&, int, const void*, void*, const void*, const Geom&,
                           const Phases&, double, int, const TileShape&, double*, bool, hipStream_t);
@@ -324,19 +334,117 @@ hipError_t dispatch_mode(bool star, const Stencil& s, int mode, const void* in, 
 
 }  // namespace
 
-// Kernel choice.  GPAW_AMD_STENCIL=tile|queue|generic forces one (timing
-// experiments); the default is the LDS tile kernel on large grids, the
-// register-queue kernel on other star stencils, the generic one otherwise.
-enum { K_GENERIC = 0, K_QUEUE = 1, K_TILE = 2 };
+// Kernel choice.  GPAW_AMD_STENCIL=tile|box|queue|generic forces one
+// (timing experiments); the default is the one-shot box kernel for real
+// data on grids that sit in the caches, the LDS tile kernel on other large
+// grids, the register-queue kernel on other star stencils, the generic one
+// otherwise.
+enum { K_GENERIC = 0, K_QUEUE = 1, K_TILE = 2, K_BOX = 3 };
 
 static int forced_kind()
 {
     static int f = -2;
     if (f == -2) {
         const char* e = getenv("GPAW_AMD_STENCIL");
-        f = !e ? -1 : e[0] == 't' ? K_TILE : e[0] == 'q' ? K_QUEUE : e[0] == 'g' ? K_GENERIC : -1;
+        f = !e ? -1
+               : e[0] == 't' ? K_TILE
+               : e[0] == 'b' ? K_BOX
+               : e[0] == 'q' ? K_QUEUE
+               : e[0] == 'g' ? K_GENERIC
+                             : -1;
     }
     return f;
+}
+
+// Kernel family for launches without partial sums, set at run time
+// (gpaw_stencil_kernel: tests and timing experiments); -1 = the default
+// choice.  Launches with partial sums keep the default, whose block count
+// sizes the callers' partial buffers.
+static std::atomic<int> g_kind_override{-1};
+
+int stencil_kernel_set(int kind)
+{
+    const int prev = g_kind_override.load();
+    if (kind >= -1 && kind <= K_BOX) g_kind_override.store(kind);
+    return prev;
+}
+
+static const char* kind_name(int k)
+{
+    return k == K_TILE ? "tile" : k == K_BOX ? "box" : k == K_QUEUE ? "queue" : "generic";
+}
+
+// Box kernel shapes (stencil_box.h): 32 x 8 columns, Q planes per block.
+// Compiled for the (type, mode) pairs the tile kernel has.
+bool box_mode_ok(int dtype, int mode)
+{
+    const bool cplx = dtype_complex(dtype), f64 = dtype == DT_F64 || dtype == DT_C128;
+    switch (mode) {
+    case ST_APPLY:
+    case ST_RESID: return true;
+    case ST_JACOBI: return !cplx;
+    case ST_PCUPD:
+    case ST_PCFIN:
+    case ST_HPSI: return f64;
+    }
+    return false;
+}
+
+bool box_geometry(const Geom& g, int R, int nin, int Q, TileShape& sh)
+{
+    if (R < 1 || R > 4 || (Q != 2 && Q != 4 && Q != 8) || nin < 1) return false;
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    sh.nt = BOX_NT;
+    sh.ntz = (n2 + BOX_TZ - 1) / BOX_TZ;
+    sh.nty = (n1 + BOX_NTY - 1) / BOX_NTY;
+    // every coordinate a block touches lies within one period on either side
+    // of the grid (the last chunk's column reaches R planes past its Q)
+    if (sh.ntz * BOX_TZ + R > 2 * n2 || sh.nty * BOX_NTY + R > 2 * n1 || n0 < Q + R ||
+        n1 < R || n2 < R)
+        return false;
+    if ((long)n1 * n2 >= PLANE_REF_MAX) return false;
+    sh.q = Q;
+    sh.nchunk = (n0 + Q - 1) / Q;
+    const long nb = (long)sh.ntz * sh.nty * nin * sh.nchunk;
+    if (nb > 0x7fffffffL) return false;
+    sh.nblocks = (int)nb;
+    return true;
+}
+
+// Grids of up to this many points (all bands) take the box kernel by
+// default: there the tile kernel's per-plane round trips dominate.
+constexpr long BOX_MAX_POINTS = 1L << 23;
+
+int box_default_q(const Geom& g, int R, int nin)
+{
+    (void)g;
+    (void)R;
+    (void)nin;
+    return 4;
+}
+
+static bool box_default(const Stencil& s, const Geom& g, int dtype, int mode, int nin,
+                        TileShape& sh)
+{
+    // real data: the multigrid levels and their residual / norm passes
+    if (dtype_complex(dtype) || !box_mode_ok(dtype, mode) || s.star_r < 1 || s.star_r > 4)
+        return false;
+    if ((long)g.n[0] * g.n[1] * g.n[2] * nin > BOX_MAX_POINTS) return false;
+    return box_geometry(g, s.star_r, nin, box_default_q(g, s.star_r, nin), sh);
+}
+
+hipError_t box_launch(const Stencil& s, int dtype, int mode, const void* in, void* out,
+                      const void* aux, const Geom& g, const Phases& ph, double w,
+                      const TileShape& sh, double* partial, bool write, hipStream_t st)
+{
+    if (!box_mode_ok(dtype, mode)) return hipErrorInvalidValue;
+    switch (dtype) {
+    case DT_F64: return box_launch_f64(s, mode, in, out, aux, g, ph, w, sh, partial, write, st);
+    case DT_C128: return box_launch_c128(s, mode, in, out, aux, g, ph, w, sh, partial, write, st);
+    case DT_F32: return box_launch_f32(s, mode, in, out, aux, g, ph, w, sh, partial, write, st);
+    case DT_C64: return box_launch_c64(s, mode, in, out, aux, g, ph, w, sh, partial, write, st);
+    }
+    return hipErrorInvalidValue;
 }
 
 int tile_tz(int)
@@ -393,13 +501,26 @@ bool tile_shape(const Geom& g, int R, int esize, int nin, int& TZ, TileShape& sh
     return (long)sh.nblocks * nt >= 256L * TILE_NT;
 }
 
-static int stencil_kind(const Stencil& s, const Geom& g, int esize, int nin, int& TZ,
+// `dtype` and `mode` decide whether the box kernel applies; partial-sum
+// layouts are asked for with DT_F64 / ST_RESID (norms are taken of real
+// data only, and float and double grids get the same box and tile shapes)
+static int stencil_kind(const Stencil& s, const Geom& g, int dtype, int mode, int nin, int& TZ,
                         TileShape& sh, int nt = TILE_NT)
 {
     const int f = forced_kind();
-    const bool tile_ok = s.star_r > 0 && tile_shape(g, s.star_r, esize, nin, TZ, sh, nt);
+    const int esize = dtype_size(dtype);
     if (f == K_GENERIC && s.d_coefs) return K_GENERIC;
     if (f == K_QUEUE && s.star_r > 0) return K_QUEUE;
+    if (f != K_QUEUE && f != K_TILE && s.star_r > 0 && box_mode_ok(dtype, mode)) {
+        TileShape bs;
+        if (f == K_BOX ? box_geometry(g, s.star_r, nin, box_default_q(g, s.star_r, nin), bs)
+                       : box_default(s, g, dtype, mode, nin, bs)) {
+            TZ = BOX_TZ;
+            sh = bs;
+            return K_BOX;
+        }
+    }
+    const bool tile_ok = s.star_r > 0 && tile_shape(g, s.star_r, esize, nin, TZ, sh, nt);
     if (tile_ok && f != K_QUEUE) return K_TILE;
     return s.star_r > 0 ? K_QUEUE : K_GENERIC;
 }
@@ -410,8 +531,8 @@ int stencil_partials(const Stencil& s, const Geom& g, int nin)
     // shape (4- and 8-byte elements)
     int TZ;
     TileShape sh;
-    const int kind = stencil_kind(s, g, 8, nin, TZ, sh);
-    if (kind == K_TILE) return sh.nblocks;
+    const int kind = stencil_kind(s, g, DT_F64, ST_RESID, nin, TZ, sh);
+    if (kind == K_TILE || kind == K_BOX) return sh.nblocks;
     if (kind == K_QUEUE) {
         StarShape q = star_shape(g, nin);
         return q.grid.x * q.grid.y * q.grid.z;
@@ -432,6 +553,8 @@ static hipError_t launch_cfg(const LaunchCfg& c, const Stencil& s, int dtype, in
                              const Phases& ph, int nin, double w, double* partial, bool write,
                              hipStream_t st)
 {
+    if (c.kind == K_BOX)
+        return box_launch(s, dtype, mode, in, out, aux, g, ph, w, c.sh, partial, write, st);
     if (c.kind == K_TILE) {
         switch (dtype) {
         case DT_F64: return tile_launch_f64(s, mode, in, out, aux, g, ph, w, c.TZ, c.sh, partial, write, st);
@@ -496,32 +619,48 @@ bool tune_enabled()
     return v > 0;
 }
 
-// mean time of one candidate's launch (after one untimed launch)
+// Mean time of one candidate's launch, replayed back to back in a hipGraph
+// as the multigrid plans run them (eager launches add host submission time,
+// which varies by more than the differences being measured).  One untimed
+// eager launch first checks the candidate and gives a rough duration, from
+// which the graph gets enough launches for ~300 us.
 float time_cfg(const LaunchCfg& c, const Stencil& s, int dtype, int mode, const void* in,
                void* out, const void* aux, const Geom& g, const Phases& ph, int nin, double w,
                bool write, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
+    hipEventRecord(e0, st);
     if (launch_cfg(c, s, dtype, mode, in, out, aux, g, ph, nin, w, nullptr, write, st) != hipSuccess)
         return -1.0f;
-    // 3 launches, then (for short kernels) enough launches for ~300 us so
-    // that launches of a few microseconds are told apart
-    float us = 0.0f;
-    for (int reps = 3, pass = 0; pass < 2; pass++) {
+    hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess || hipGetLastError() != hipSuccess) return -1.0f;
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, e0, e1);
+    int reps = ms > 0.0f ? (int)(0.3f / ms) : 32;
+    reps = reps < 2 ? 2 : reps > 32 ? 32 : reps;
+    hipGraphExec_t ge = nullptr;
+    if (build_graph(&ge, [&](hipStream_t) {
+            hipError_t e = hipSuccess;
+            for (int r = 0; r < reps && e == hipSuccess; r++)
+                e = launch_cfg(c, s, dtype, mode, in, out, aux, g, ph, nin, w, nullptr, write,
+                               nullptr);
+            return e;
+        }) != hipSuccess)
+        return -1.0f;
+    float best = -1.0f;
+    for (int pass = 0; pass < 4; pass++) {   // first replay untimed
         hipEventRecord(e0, st);
-        for (int r = 0; r < reps; r++)
-            launch_cfg(c, s, dtype, mode, in, out, aux, g, ph, nin, w, nullptr, write, st);
+        hipError_t e = hipGraphLaunch(ge, st);
         hipEventRecord(e1, st);
-        if (hipEventSynchronize(e1) != hipSuccess || hipGetLastError() != hipSuccess)
-            return -1.0f;
-        float ms = 0.0f;
+        if (e != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            best = -1.0f;
+            break;
+        }
         hipEventElapsedTime(&ms, e0, e1);
-        us = 1e3f * ms / reps;
-        if (us * reps >= 100.0f) break;
-        reps = us > 0.0f ? (int)(300.0f / us) : 64;
-        if (reps > 64) reps = 64;
-        if (reps < 3) break;
+        const float us = 1e3f * ms / reps;
+        if (pass > 0 && (best < 0 || us < best)) best = us;
     }
-    return us;
+    hipGraphExecDestroy(ge);
+    return best;
 }
 
 TuneVal tune(const TuneKey& key, const LaunchCfg& heuristic, const Stencil& s, int dtype,
@@ -544,6 +683,11 @@ TuneVal tune(const TuneKey& key, const LaunchCfg& heuristic, const Stencil& s, i
                 if (!dup) cand.push_back(c);
             }
     }
+    if (box_mode_ok(dtype, mode))
+        for (int q : {2, 4, 8}) {
+            LaunchCfg c{K_BOX, BOX_TZ, TileShape{}};
+            if (box_geometry(g, s.star_r, nin, q, c.sh)) cand.push_back(c);
+        }
     TuneVal best{heuristic, -1.0f, 0};
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return best;
@@ -562,9 +706,8 @@ TuneVal tune(const TuneKey& key, const LaunchCfg& heuristic, const Stencil& s, i
             fprintf(stderr,
                     "tune dtype=%d mode=%d R=%d n=%dx%dx%d nin=%d: %s TZ=%d q=%d threads=%d "
                     "%.2f us\n",
-                    dtype, mode, s.star_r, g.n[0], g.n[1], g.n[2], nin,
-                    c.kind == K_TILE ? "tile" : "queue", c.TZ, c.kind == K_TILE ? c.sh.q : 0,
-                    c.kind == K_TILE ? c.sh.nt : 256, us);
+                    dtype, mode, s.star_r, g.n[0], g.n[1], g.n[2], nin, kind_name(c.kind),
+                    c.TZ, c.kind >= K_TILE ? c.sh.q : 0, c.kind >= K_TILE ? c.sh.nt : 256, us);
         if (us < 0) continue;
         best.candidates++;
         if (best.us < 0 || us < best.us) {
@@ -598,10 +741,9 @@ long autotune_report(char* buf, long len)
         snprintf(line, sizeof line,
                  "dtype=%d mode=%d R=%d n=%dx%dx%d periodic=%d nin=%d -> %s TZ=%d q=%d "
                  "threads=%d %.2f us (%d candidates)\n",
-                 k.dtype, k.mode, k.R, k.n0, k.n1, k.n2, k.per, k.nin,
-                 v.cfg.kind == K_TILE ? "tile" : "queue", v.cfg.kind == K_TILE ? v.cfg.TZ : 0,
-                 v.cfg.kind == K_TILE ? v.cfg.sh.q : 0, v.cfg.kind == K_TILE ? v.cfg.sh.nt : 256,
-                 v.us, v.candidates);
+                 k.dtype, k.mode, k.R, k.n0, k.n1, k.n2, k.per, k.nin, kind_name(v.cfg.kind),
+                 v.cfg.kind >= K_TILE ? v.cfg.TZ : 0, v.cfg.kind >= K_TILE ? v.cfg.sh.q : 0,
+                 v.cfg.kind >= K_TILE ? v.cfg.sh.nt : 256, v.us, v.candidates);
         r += line;
     }
     if (buf && len > 0) {
@@ -621,8 +763,26 @@ hipError_t launch_stencil(const Stencil& s, int dtype, int mode, const void* in,
     const int nt = (dtype_complex(dtype) && (mode == ST_APPLY || mode == ST_HPSI) && s.star_r <= 3)
                        ? TILE_NT_WIDE
                        : TILE_NT;
-    c.kind = stencil_kind(s, g, dtype_size(dtype), nin, c.TZ, c.sh, nt);
+    c.kind = stencil_kind(s, g, dtype, mode, nin, c.TZ, c.sh, nt);
     if (npartial) *npartial = stencil_partials(s, g, nin);
+    const int ov = g_kind_override.load();
+    if (!partial && ov >= 0 && ov != c.kind) {
+        // the requested family when it can run this launch, else the default
+        LaunchCfg o{ov, 64, TileShape{}};
+        bool ok = false;
+        if (ov == K_BOX) {
+            ok = box_mode_ok(dtype, mode) && s.star_r >= 1 && s.star_r <= 4 &&
+                 box_geometry(g, s.star_r, nin, box_default_q(g, s.star_r, nin), o.sh);
+            o.TZ = BOX_TZ;
+        } else if (ov == K_TILE) {
+            ok = s.star_r >= 1 && tile_shape(g, s.star_r, dtype_size(dtype), nin, o.TZ, o.sh, nt);
+        } else if (ov == K_QUEUE) {
+            ok = s.star_r > 0;
+        } else {
+            ok = s.d_coefs != nullptr;
+        }
+        if (ok) return launch_cfg(o, s, dtype, mode, in, out, aux, g, ph, nin, w, nullptr, write, st);
+    }
     if (!partial && s.star_r >= 1 && s.star_r <= 4 && forced_kind() < 0 && tune_enabled()) {
         const TuneKey key{dtype, mode, s.star_r, g.n[0], g.n[1], g.n[2],
                           g.per[0] | g.per[1] << 1 | g.per[2] << 2, nin};
@@ -645,8 +805,8 @@ hipError_t launch_stencil(const Stencil& s, int dtype, int mode, const void* in,
         }
     }
     // in . out partials of an apply and input-residual partials of a Jacobi
-    // sweep come from the tile kernel only
-    if (partial && (mode == ST_APPLY || mode == ST_JACOBI) && c.kind != K_TILE)
+    // sweep come from the tile and box kernels only
+    if (partial && (mode == ST_APPLY || mode == ST_JACOBI) && c.kind != K_TILE && c.kind != K_BOX)
         return hipErrorInvalidValue;
     return launch_cfg(c, s, dtype, mode, in, out, aux, g, ph, nin, w, partial, write, st);
 }
@@ -655,8 +815,8 @@ bool stencil_dot_ok(const Stencil& s, int dtype, const Geom& g, int nin)
 {
     int TZ;
     TileShape sh;
-    return (dtype == DT_F64 || dtype == DT_F32) &&
-           stencil_kind(s, g, dtype_size(dtype), nin, TZ, sh) == K_TILE;
+    const int k = stencil_kind(s, g, dtype, ST_RESID, nin, TZ, sh);
+    return (dtype == DT_F64 || dtype == DT_F32) && (k == K_TILE || k == K_BOX);
 }
 
 }  // namespace gpaw

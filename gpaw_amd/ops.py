@@ -297,6 +297,17 @@ def autotune(enable=True):
     return bool(_lib.lib().gpaw_autotune(int(bool(enable))))
 
 
+STENCIL_KERNELS = {'default': 0, 'tile': 1, 'box': 2, 'queue': 3, 'generic': 4}
+
+
+def stencil_kernel(kind='default'):
+    """Kernel family for star-stencil launches without partial sums
+    ('default', 'tile', 'box', 'queue' or 'generic'; every family gives the
+    same bits); returns the previous family."""
+    prev = _lib.lib().gpaw_stencil_kernel(STENCIL_KERNELS[kind])
+    return {v: k for k, v in STENCIL_KERNELS.items()}[prev]
+
+
 def autotune_report():
     """The tuner's choices so far, one line per (dtype, mode, range, grid,
     bands) combination."""

@@ -234,6 +234,16 @@ int gpaw_autotune(int enable);
  * length (buf may be NULL to query it). */
 long gpaw_autotune_report(char* buf, long len);
 
+/* No reference counterpart: the kernel family of star-stencil launches that
+ * carry no partial sums, for parity tests of every family and timing
+ * experiments.  kind: 0 the default choice (heuristic or tuner), 1 the
+ * plane-sweeping tile kernel, 2 the one-shot box kernel, 3 the register-
+ * queue column kernel, 4 the generic kernel; a family that cannot run a
+ * launch (grid too small, stencil not a star) leaves it to the default;
+ * -1 only queries.  Returns the previous setting.  Every family gives
+ * bit-identical results. */
+int gpaw_stencil_kernel(int kind);
+
 /* -------------------------------------------------------------- helpers */
 /* Deterministic FP64 sum of n doubles times scale into *result (device):
  * gd.integrate / comm.sum of the reference (gpaw/grid_descriptor.py:197-208). */
