@@ -59,6 +59,7 @@ PROTOTYPES = {
     'gpaw_stencil_kernel': (_i, [_i]),
     'gpaw_sum': (_i, [_vp, _l, _d, _vp, _vp]),
     'gpaw_dots': (_i, [_vp, _vp, _l, _l, _d, _vp, _vp]),
+    'gpaw_dots_noalloc': (_i, [_vp, _vp, _l, _l, _d, _vp, _vp]),
     'gpaw_axpby': (_i, [_vp, _vp, _d, _d, _l, _vp]),
     'gpaw_sub_scalar': (_i, [_vp, _vp, _d, _l, _vp]),
     'gpaw_cast': (_i, [_vp, _i, _vp, _i, _l, _vp]),

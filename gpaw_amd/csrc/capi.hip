@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "capi_internal.h"
@@ -443,6 +444,63 @@ extern "C" int gpaw_dots(const double* a, const double* b, long nb, long n, doub
     hipError_t e2 = hipFreeAsync(partial, (hipStream_t)stream);
     if (e == hipSuccess) e = e2;
     return gpaw_check(e, "dots");
+}
+
+// Partials of gpaw_dots_noalloc: one file-scope device array (each GPU the
+// library runs on holds its own copy until the process ends).  In host code
+// its name only stands in for the symbol: the device address of this GPU's
+// copy comes from hipGetSymbolAddress, once per device, and that address is
+// what the kernels get.  Every call on a GPU shares that GPU's copy, so calls
+// that overlap on one GPU race; each call writes every partial it reads.
+#define GPAW_DOTS_SCRATCH (1 << 19)   // doubles (4 MiB)
+static __device__ __align__(256) double scratch[GPAW_DOTS_SCRATCH];
+
+static double* dots_scratch(int* rc)
+{
+    static std::mutex mu;
+    static double* ptr[64] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        *rc = gpaw_check(e, "dots_noalloc: current device");
+        return nullptr;
+    }
+    if (dev < 0 || dev >= 64) {
+        *rc = gpaw_fail(GPAW_EUNSUPPORTED, "dots_noalloc: device %d beyond 63", dev);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            *rc = gpaw_check(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                             "dots_noalloc: scratch address");
+            return nullptr;
+        }
+        ptr[dev] = (double*)p;
+    }
+    *rc = GPAW_OK;
+    return ptr[dev];
+}
+
+extern "C" int gpaw_dots_noalloc(const double* a, const double* b, long nb, long n, double scale,
+                                 double* result, void* stream)
+{
+    if (!a || !result || (!b && nb > 0) || n < 0 || nb < 0 || nb > 65535)
+        return gpaw_fail(GPAW_EINVAL, "dots_noalloc: bad argument (0 <= nb <= 65535)");
+    if (nb == 0) return GPAW_OK;
+    const long need = nb * (long)dots_chunks(n);
+    if (need > GPAW_DOTS_SCRATCH)
+        return gpaw_fail(GPAW_EINVAL,
+                         "dots_noalloc: %ld rows x %d chunks need %ld partials, the static "
+                         "scratch holds %d (use gpaw_dots)",
+                         nb, dots_chunks(n), need, GPAW_DOTS_SCRATCH);
+    int rc = GPAW_OK;
+    double* partial = dots_scratch(&rc);
+    if (!partial) return rc;
+    return gpaw_check(launch_dots(a, b, (int)nb, n, scale, partial, result, (hipStream_t)stream),
+                      "dots_noalloc");
 }
 
 extern "C" int gpaw_axpby(double* y, const double* x, double a, double b, long n, void* stream)

@@ -257,6 +257,18 @@ int gpaw_sum(const double* a, long n, double scale, double* result,
 int gpaw_dots(const double* a, const double* b, long nb, long n, double scale,
               double* result, void* stream);
 
+/* gpaw_dots without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The partial sums live in a static device array of 2^19
+ * doubles (4 MiB per GPU, held until the process ends), so nb times the
+ * number of chunks of n (ceil(n / 1024), at most 256) must not exceed 2^19;
+ * larger calls return GPAW_EINVAL.  Same kernels, same results as
+ * gpaw_dots.  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams, other threads) race. */
+int gpaw_dots_noalloc(const double* a, const double* b, long nb, long n,
+                      double scale, double* result, void* stream);
+
 /* y = a*y + b*x over n doubles (numpy's in-place updates around the
  * solver, e.g. phi += q*phi_gauss, gpaw/poisson.py:180,186). */
 int gpaw_axpby(double* y, const double* x, double a, double b, long n,
