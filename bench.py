@@ -110,6 +110,10 @@ def jacobi_kernel_name():
                 q = line.split(' q=')[1].split()[0]
                 return ('k_star_tile<double,3,%s,1,JACOBI> level-0 sweep '
                         '(tiles of %s planes)' % (tz, q))
+            if '-> box' in line:
+                q = line.split(' q=')[1].split()[0]
+                return ('k_star_box<double,3,%s,JACOBI> level-0 sweep '
+                        '(32 x 8 columns x %s planes per block)' % (q, q))
             return 'k_star<double,3,JACOBI> level-0 sweep'
     return 'k_star<double,3,JACOBI> level-0 sweep'
 
@@ -152,8 +156,9 @@ def jacobi_roofline(torch, np, ps, rho_d):
 
 def copy_bandwidth(torch):
     """Device-to-device copy of 2 GiB (read + write 4 GiB) timed with events
-    with the library's 16-byte copy kernel (k_copy16): the attainable HBM
-    bandwidth on this box, reported beside the peak."""
+    with the library's copy (k_copy16_flat: one 16-byte element per thread
+    over the whole range, the fastest copy measured here): the attainable
+    HBM bandwidth on this box, reported beside the peak."""
     from gpaw_amd.ops import copy
     n = 1 << 28
     a = torch.empty(n, dtype=torch.float64, device='cuda').fill_(1.0)
@@ -264,15 +269,43 @@ def batch_c3(torch, np):
     ts = event_time(torch, sequential, 10)
     same = all(torch.equal(o, b) for o, (_, _, b, _) in zip(outs, groups))
     del groups, outs
-    return {'config': 'complex128 nn=3 Laplace: 4 k-points x 48 bands of 64^3, '
-                      '32 x 48^3, 24 x 80x64x72 (periodic, Bloch phases), '
-                      '16 x 56^3 zero boundary; 7 groups',
-            'batched_ms': round(tb * 1e3, 3), 'batched_GBps': round(algo / tb / 1e9, 1),
-            'batched_pct_peak': round(100 * algo / tb / 1e9 / PEAK_HBM_GBS, 1),
-            'sequential_ms': round(ts * 1e3, 3),
-            'sequential_GBps': round(algo / ts / 1e9, 1),
-            'speedup_batched_over_sequential': round(ts / tb, 3),
-            'bit_identical': bool(same), 'bytes': algo}
+    out = {'config': 'complex128 nn=3 Laplace: 4 k-points x 48 bands of 64^3, '
+                     '32 x 48^3, 24 x 80x64x72 (periodic, Bloch phases), '
+                     '16 x 56^3 zero boundary; 7 groups',
+           'batched_ms': round(tb * 1e3, 3), 'batched_GBps': round(algo / tb / 1e9, 1),
+           'batched_pct_peak': round(100 * algo / tb / 1e9 / PEAK_HBM_GBS, 1),
+           'sequential_ms': round(ts * 1e3, 3),
+           'sequential_GBps': round(algo / ts / 1e9, 1),
+           'speedup_batched_over_sequential': round(ts / tb, 3),
+           'bit_identical': bool(same), 'bytes': algo}
+    # many small uneven groups (k-points of small cells, serving-sized
+    # requests): there a launch per group costs more than its data
+    groups, algo = [], 0.0
+    for i in range(96):
+        e = int(rng.choice([24, 32, 40, 48]))
+        N = (e, e + 8 * int(rng.integers(0, 2)), e)
+        pbc = bool(i % 4)
+        gd = GridDescriptor(N, np.array(N) * 0.2 / BOHR, pbc)
+        nb = int(rng.integers(1, 5))
+        a = torch.randn((nb,) + tuple(gd.n_c), dtype=torch.complex128,
+                        device='cuda', generator=g)
+        ph = gd.phase_cd(rng.uniform(-0.5, 0.5, 3)) if pbc else None
+        groups.append((Laplace(gd, -0.5, 3, complex), a, torch.empty_like(a), ph))
+        algo += 32.0 * nb * np.prod(gd.n_c)
+    batch = BatchApply(max_groups=len(groups))
+    tb = event_time(torch, lambda: batch(groups), 10)
+    outs = [b.clone() for _, _, b, _ in groups]
+    ts = event_time(torch, sequential, 10)
+    same = all(torch.equal(o, b) for o, (_, _, b, _) in zip(outs, groups))
+    del groups, outs
+    out['many_small'] = {
+        'config': 'complex128 nn=3 Laplace: 96 groups of 1-4 bands on grids of edges '
+                  '24-56, 3 of 4 periodic with their own k-point',
+        'batched_ms': round(tb * 1e3, 3), 'batched_GBps': round(algo / tb / 1e9, 1),
+        'sequential_ms': round(ts * 1e3, 3),
+        'speedup_batched_over_sequential': round(ts / tb, 3),
+        'bit_identical': bool(same), 'bytes': algo}
+    return out
 
 
 def preconditioner_c3(torch, np):

@@ -112,6 +112,13 @@ __global__ __launch_bounds__(256) void k_scale(double* __restrict__ y, double a,
 
 // 16-byte copy, four loads in flight per thread, grid-stride over blocks of
 // 4 x 256 vectors (the attainable HBM read + write rate)
+__global__ __launch_bounds__(256) void k_copy16_flat(uint4* __restrict__ y,
+                                                     const uint4* __restrict__ x, i64 n)
+{
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = x[i];
+}
+
 __global__ __launch_bounds__(256) void k_copy16(uint4* __restrict__ y, const uint4* __restrict__ x,
                                                 i64 n)
 {
@@ -260,6 +267,14 @@ hipError_t launch_scale(double* y, double a, long long n, hipStream_t st)
 hipError_t launch_copy_bytes(void* y, const void* x, long long nbytes, hipStream_t st)
 {
     const i64 n = nbytes / 16;
+    const i64 flat = (n + 255) / 256;
+    if (flat >= 1 && flat < 0x7fffffffLL) {
+        // one 16-byte element per thread over the whole range: the fastest
+        // copy measured on MI355X (every wave's loads in flight at once)
+        launch_kernel(k_copy16_flat, dim3((unsigned)flat), dim3(256), 0, st, (uint4*)y,
+                      (const uint4*)x, n);
+        return launch_error();
+    }
     i64 blocks = (n + 1023) / 1024;
     if (blocks > 8192) blocks = 8192;   // 32 blocks per CU, then grid-stride
     if (blocks < 1) blocks = 1;
