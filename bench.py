@@ -526,6 +526,20 @@ def mixed_vs_fp64(torch, np, N=160, h=0.18, count=32, reps=10):
     res['fp32_contribution'] = round(ms['pcg_fp64_vcycle'] / ms['mixed'], 3)
     res['fp32_over_fp64_iteration_time'] = round(
         res['mixed']['ms_per_iteration'] / res['pcg_fp64_vcycle']['ms_per_iteration'], 3)
+    # one inner V-cycle alone (gpaw_poisson_precondition: the same schedule
+    # in FP32 and in FP64; the FP32 call also converts r in and z out)
+    pc = MixedPoissonSolver(nn=3)
+    pc.set_grid_descriptor(gd)
+    pc.initialize()
+    z = torch.empty_like(rho_n)
+    tv = {p: event_time(torch, lambda p=p: pc.precondition(rho_n, z, p), 20)
+          for p in ('fp32', 'fp64')}
+    res['inner_vcycle'] = {
+        'fp32_ms': round(tv['fp32'] * 1e3, 4), 'fp64_ms': round(tv['fp64'] * 1e3, 4),
+        'fp32_over_fp64': round(tv['fp32'] / tv['fp64'], 3),
+        'note': 'one preconditioning V-cycle from zero (2 + 2 level-0 sweeps, reference '
+                'hierarchy below), FP32 including the FP64 <-> FP32 conversion of r and z'}
+    del pc
     res['config'] = ('C4: 160^3 zero boundary (159^3 arrays), h=0.18 A, 32 neutral '
                      'Gaussians, 1e-10 relative residual')
     return res

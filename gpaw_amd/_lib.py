@@ -42,6 +42,7 @@ PROTOTYPES = {
     'gpaw_poisson_vcycles': (_i, [_vp, _vp, _vp, _i, _dp, _vp]),
     'gpaw_poisson_solve_mixed': (_i, [_vp, _vp, _vp, _d, _i, _i, _i, _ip, _dp,
                                       _vp]),
+    'gpaw_poisson_precondition': (_i, [_vp, _vp, _vp, _i, _vp]),
     'gpaw_poisson_free': (None, [_vp]),
     'gpaw_preconditioner_new': (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                      ctypes.POINTER(_vp)]),

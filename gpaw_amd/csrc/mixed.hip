@@ -579,6 +579,38 @@ void gpaw_mixed_free(MixedState* m)
     delete m;
 }
 
+extern "C" int gpaw_poisson_precondition(gpaw_poisson* ps, const double* r, double* z,
+                                         int inner_precision, void* stream)
+{
+    if (!ps || !r || !z || inner_precision < 0 || inner_precision > 1)
+        return gpaw_fail(GPAW_EINVAL, "precondition: bad argument");
+    if (ps->method != 2) return gpaw_fail(GPAW_EUNSUPPORTED, "precondition: Jacobi smoothing only");
+    if (r == z) return gpaw_fail(GPAW_EINVAL, "precondition: r and z must differ");
+    const int pr = inner_precision;
+    hipStream_t st = (hipStream_t)stream;
+    int rc = setup(ps, pr);
+    if (rc) return rc;
+    MixedState* m = ps->mixed;
+    const i64 n = ps->npts[0];
+    // the right-hand side where the inner V-cycle reads it: the FP32
+    // hierarchy's level-0 rho, or (FP64) the plan's level-0 residual array
+    hipError_t e = pr == 0 ? launch_cast(m->v[0].rho[0], DT_F32, r, DT_F64, n, st)
+                           : launch_copy(ps->res[0], r, DT_F64, n, st);
+    if (e != hipSuccess) return gpaw_check(e, "precondition: load");
+    if (!m->tuned[pr] && autotune_set(-1)) {
+        m->tuned[pr] = 1;
+        e = emit_top(ps, m, pr, true, st);
+        if (e != hipSuccess) return gpaw_check(e, "precondition: tune");
+    }
+    rc = capture_all(ps, pr);
+    if (rc) return rc;
+    rc = gpaw_check(hipGraphLaunch(m->g_vcycle[pr], st), "precondition: V-cycle");
+    if (rc) return rc;
+    e = pr == 0 ? launch_cast(z, DT_F64, z_of(m, pr), DT_F32, n, st)
+                : launch_copy(z, z_of(m, pr), DT_F64, n, st);
+    return gpaw_check(e, "precondition: store");
+}
+
 extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const double* rho,
                                         double rel_tol, int maxiter, int mode, int inner,
                                         int* niter, double* relres, void* stream)

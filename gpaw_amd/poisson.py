@@ -312,6 +312,24 @@ class MixedPoissonSolver(PoissonSolver):
         P.finish()
         return niter.value
 
+    def precondition(self, r, z, precision=None):
+        """z = one inner V-cycle applied to r from z = 0 (the multigrid
+        preconditioner of the outer CG): float64 grid functions of the
+        solver's grid, on the device or as numpy arrays.  precision 'fp32'
+        or 'fp64' (default: the solver's inner_precision)."""
+        precision = precision or self.inner_precision
+        if precision not in ('fp32', 'fp64'):
+            raise ValueError('precision must be "fp32" or "fp64"')
+        if self.relax_method != 2:
+            raise NotImplementedError('the mixed V-cycle uses Jacobi smoothing')
+        shape = tuple(self.gd.n_c)
+        Rr = Arg(r, name='r', shape=shape, code=_lib.GPAW_F64)
+        Z = Arg(z, write=True, name='z', shape=shape, code=_lib.GPAW_F64)
+        _lib.check(_lib.lib().gpaw_poisson_precondition(
+            self._plan, Rr.ptr, Z.ptr, 1 if precision == 'fp64' else 0,
+            _arrays.stream()))
+        Z.finish()
+
     def __repr__(self):
         return ('MixedPoissonSolver(nn=%r, outer=%r, rel_tol=%e, '
                 'inner_precision=%r)' % (self.nn, self.outer, self.rel_tol,

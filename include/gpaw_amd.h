@@ -137,6 +137,18 @@ int gpaw_poisson_solve_neutral(gpaw_poisson* ps, double* phi,
 int gpaw_poisson_vcycles(gpaw_poisson* ps, double* phi, const double* rho,
                          int ncycles, double* errors, void* stream);
 
+/* One inner V-cycle of the mixed solver applied to r, from z = 0: the
+ * multigrid preconditioner z ~ L^-1 r of its flexible CG, for callers that
+ * run their own Krylov loop.  No reference counterpart (gpaw/poisson.py
+ * iterates V-cycles on phi only).  The reference hierarchy and weights
+ * (gpaw/poisson.py:64-113) with the mixed solver's symmetric level-0
+ * smoothing (2 pre + 2 post sweeps); inner_precision 0: every level's
+ * sweeps, residuals and transfers in FP32 (the smallest levels in FP64),
+ * 1: all in FP64.  r, z: n0*n1*n2 doubles on the device, distinct; the
+ * plan must use Jacobi smoothing.  Asynchronous on `stream`. */
+int gpaw_poisson_precondition(gpaw_poisson* ps, const double* r, double* z,
+                              int inner_precision, void* stream);
+
 /* Mixed-precision solve of the same problem (no reference counterpart: the
  * FP32 / FP64 split of the configuration "mixed-precision multigrid
  * Poisson").  FP64 outer loop (residual L phi - rho, correction
