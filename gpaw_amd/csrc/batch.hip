@@ -45,7 +45,18 @@ using namespace gpaw;
 
 namespace {
 constexpr int SLOTS = 4;
-constexpr int BATCH_Q = 16;   // planes per tile: similar tile costs across groups
+// Planes per tile of a group: the whole axis up to 64 planes (each tile
+// re-reads 2R queue planes, and a full periodic period reads none twice
+// but the wrap), longer axes in equal runs of at most 64.  Tiles of
+// different groups then cost within a small factor of each other, and the
+// persistent grid takes the next tile as it frees up.
+int batch_q(const Geom& g)
+{
+    const int n0 = g.n[0];
+    if (n0 <= 64) return n0;
+    const int runs = (n0 + 63) / 64;
+    return (n0 + runs - 1) / runs;
+}
 }
 
 struct gpaw_batch {
@@ -174,8 +185,10 @@ extern "C" int gpaw_batch_apply(gpaw_batch* b, int ngroups, gpaw_operator* const
         for (int i = 0; i < ngroups && nt != TILE_NT; i++) {
             TileShape sh;
             if (ops[i]->st.star_r == R && nins[i] &&
-                !tile_geometry(ops[i]->g, R, esize, (int)nins[i], TZ, BATCH_Q, sh, nt) &&
-                tile_geometry(ops[i]->g, R, esize, (int)nins[i], TZ, BATCH_Q, sh, TILE_NT))
+                !tile_geometry(ops[i]->g, R, esize, (int)nins[i], TZ, batch_q(ops[i]->g), sh,
+                               nt) &&
+                tile_geometry(ops[i]->g, R, esize, (int)nins[i], TZ, batch_q(ops[i]->g), sh,
+                              TILE_NT))
                 nt = TILE_NT;
         }
         nts[R] = nt;
@@ -183,7 +196,7 @@ extern "C" int gpaw_batch_apply(gpaw_batch* b, int ngroups, gpaw_operator* const
             gpaw_operator* op = ops[i];
             if (op->st.star_r != R || !nins[i]) continue;
             TileShape sh;
-            if (!tile_geometry(op->g, R, esize, (int)nins[i], TZ, BATCH_Q, sh, nt) ||
+            if (!tile_geometry(op->g, R, esize, (int)nins[i], TZ, batch_q(op->g), sh, nt) ||
                 (long)tiles[R] + sh.nblocks > 0x7fffffffL) {
                 fallback[i] = 1;
                 continue;
