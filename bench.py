@@ -136,13 +136,19 @@ def measured_traffic(label):
 
 def jacobi_roofline(torch, np, ps, rho_d):
     """Average duration of one level-0 Jacobi sweep (the dominant kernel of
-    the V-cycle) over a timed run, HIP events on torch's current stream,
-    which is the stream the library launches on."""
+    the V-cycle): 100 relax calls of 2 sweeps captured once in a graph (as
+    the V-cycles run them, back to back without host launch gaps) and the
+    graph replayed 4 times, HIP events on the capture's stream."""
     op = ps.operators[0].operator
     n = int(np.prod(ps.gd.n_c))
     f = torch.zeros(tuple(ps.gd.n_c), dtype=torch.float64, device='cuda')
-    reps = 200
-    t = event_time(torch, lambda: op.relax(2, f, rho_d, 2, 2.0 / 3.0), reps) / 2
+    op.relax(2, f, rho_d, 2, 2.0 / 3.0)        # launch shape and scratch, before capture
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(100):
+            op.relax(2, f, rho_d, 2, 2.0 / 3.0)
+    t = event_time(torch, g.replay, 4) / 200
     algo = 24.0 * n          # read phi, read rho, write phi (8 B each)
     return {'bound': 'hbm', 'achieved': round(algo / t / 1e9, 1),
             'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
