@@ -45,6 +45,20 @@ struct TileShape {
     int nt = TILE_NT;          // threads per block
 };
 
+// Output store of the tile sweep.  NT: non-temporal (streaming) store,
+// for outputs that are not read again soon (large complex applies).
+template <bool NT, class V>
+__device__ __forceinline__ void tile_store(V* p, V v)
+{
+    if constexpr (NT && VT<V>::cplx) {
+        typedef typename VT<V>::R RT;
+        typedef RT RT2 __attribute__((ext_vector_type(2)));
+        __builtin_nontemporal_store(__builtin_bit_cast(RT2, v), reinterpret_cast<RT2*>(p));
+    } else {
+        *p = v;
+    }
+}
+
 template <class V>
 __device__ __forceinline__ double tnorm2(V v);
 // in . result for real data (norm passes of ST_APPLY)
@@ -90,6 +104,9 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
     typedef typename VT<V>::R RT;
     typedef TileGeom<V, R, TZ, ROWS, NT> TG;
     constexpr int TY = TG::TY, LW = TG::LW, LH = TG::LH, NHALO = TG::NHALO, NH = TG::NH;
+    // streaming stores for the wide complex applies (many bands: the output
+    // is not re-read before it leaves the caches)
+    constexpr bool NTS = VT<V>::cplx && MODE == ST_APPLY && NT == TILE_NT_WIDE;
 
     const int tx = threadIdx.x, tyi = threadIdx.y;
     const int tid = tx + TZ * tyi;
@@ -211,7 +228,7 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
         if (WRITE && i0 > i0s) {
 #pragma unroll
             for (int j = 0; j < ROWS; j++)
-                if (inr[j]) o[(i64)(i0 - 1) * plane + pref_off(qr[j])] = rp[j];
+                if (inr[j]) tile_store<NTS>(o + (i64)(i0 - 1) * plane + pref_off(qr[j]), rp[j]);
         }
         if (i0 + 1 < i0e) {
             i64 base;
@@ -302,7 +319,7 @@ __device__ __forceinline__ double star_tile_sweep(const V* __restrict__ a, V* __
     if (WRITE && i0e > i0s) {
 #pragma unroll
         for (int j = 0; j < ROWS; j++)
-            if (inr[j]) o[(i64)(i0e - 1) * plane + pref_off(qr[j])] = rp[j];
+            if (inr[j]) tile_store<NTS>(o + (i64)(i0e - 1) * plane + pref_off(qr[j]), rp[j]);
     }
     return acc;
 }
