@@ -86,8 +86,11 @@ hipError_t tr_launch(const Transfer& tr, int mode, const void* in, void* out, vo
                                (V*)out, t, nc[0], nc[1], nc[2], ph);
         return launch_error();
     }
-    // wide transfers on large planes: LDS-tiled plane sweep (8 x 32 tiles)
-    if (K >= 4 && t.nout[1] >= 4 && t.nout[2] >= 16 &&
+    // wide transfers on large planes, and K = 2 restrictions of many points
+    // (band sets: the point-wise kernel reads each input up to 8 times through
+    // the caches): LDS-tiled plane sweep (8 x 32 tiles)
+    const bool big2 = K == 2 && !tr.interpolate && t.ngout * nin >= (1 << 20);
+    if ((K >= 4 || big2) && t.nout[1] >= 4 && t.nout[2] >= 16 &&
         (long)t.gin.n[1] * t.gin.n[2] < PLANE_REF_MAX && !getenv("GPAW_AMD_POINTWISE_TRANSFER")) {
         TileTr tt;
         tt.nt1 = (t.nout[1] + 7) / 8;
