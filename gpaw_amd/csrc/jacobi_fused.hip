@@ -38,7 +38,13 @@ TrArgs make_trargs(const Transfer& tr);
 
 namespace {
 
-constexpr int JF_B = 8;   // output box edge
+// A launch runs S = 4 or 2 sweeps on a 16^3 LDS box whose output is the
+// inner B^3 = (16 - 2S)^3 cells: 8^3 outputs for four sweeps, 12^3 for two.
+// Two-sweep launches recompute fewer halo cells per output (4472 cell
+// updates per 1728 outputs against 5984 per 512) at one more launch per
+// four sweeps: the better trade on levels with enough 12^3 boxes to fill
+// the chip.
+constexpr int jf_box(int S) { return 16 - 2 * S; }
 
 __device__ __forceinline__ int jf_wrap(int j, int n, int per, bool& out)
 {
@@ -71,14 +77,15 @@ __global__ __launch_bounds__(JF_NT) void k_jacobi_fused(const V* __restrict__ in
                                                         const V* __restrict__ corr, TrArgs ct,
                                                         int zero_in)
 {
-    constexpr int E = JF_B + 2 * S;   // box edge with halo
+    constexpr int B = jf_box(S);      // output box edge
+    constexpr int E = B + 2 * S;      // box edge with halo
     static_assert(E * E == 256, "one thread per (y, z) column of the box and plane run");
     constexpr int PX = E * E * E / JF_NT;   // planes per thread
     __shared__ V P[2][E * E * E];
     __shared__ V C[JF_CB * JF_CB * JF_CB];
     const int b = blockIdx.x;
     const int bz = b % nb2, by = (b / nb2) % nb1, bx = b / (nb2 * nb1);
-    const int o0 = bx * JF_B - S, o1 = by * JF_B - S, o2 = bz * JF_B - S;
+    const int o0 = bx * B - S, o1 = by * B - S, o2 = bz * B - S;
     const int tz = threadIdx.x % E, ty = (threadIdx.x / E) % E;
     const int x0 = (threadIdx.x / (E * E)) * PX;   // first plane of this thread
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
@@ -157,12 +164,12 @@ __global__ __launch_bounds__(JF_NT) void k_jacobi_fused(const V* __restrict__ in
     }
     // the box: cells S .. S + B - 1 along each axis, inside the grid
     const int i1 = o1 + ty, i2 = o2 + tz;
-    if (ty >= S && ty < S + JF_B && tz >= S && tz < S + JF_B && i1 < n1 && i2 < n2) {
+    if (ty >= S && ty < S + B && tz >= S && tz < S + B && i1 < n1 && i2 < n2) {
 #pragma unroll
         for (int p = 0; p < PX; p++) {
             const int x = x0 + p;
             const int i0 = o0 + x;
-            if (x >= S && x < S + JF_B && i0 < n0)
+            if (x >= S && x < S + B && i0 < n0)
                 out[(long)i0 * plane + (long)i1 * n2 + i2] = P[S & 1][(x * E + ty) * E + tz];
         }
     }
@@ -175,11 +182,18 @@ bool jacobi_fused_ok(const Stencil& s, const Geom& g)
     return s.star_r == 1 && g.n[0] >= 1 && g.n[1] >= 1 && g.n[2] >= 1;
 }
 
+int jacobi_fused_sweeps(const Geom& g)
+{
+    // two sweeps per launch once the 12^3 boxes fill the chip (256 CUs)
+    const long nb = (long)((g.n[0] + 11) / 12) * ((g.n[1] + 11) / 12) * ((g.n[2] + 11) / 12);
+    return nb >= 256 ? 2 : JF_SWEEPS;
+}
+
 hipError_t launch_jacobi_fused(const Stencil& s, int dtype, const void* in, void* out,
                                const void* rho, const Geom& g, double w, hipStream_t st,
-                               const void* corr, const Transfer* ctr, bool zero_in)
+                               const void* corr, const Transfer* ctr, bool zero_in, int sweeps)
 {
-    if (!jacobi_fused_ok(s, g)) return hipErrorInvalidValue;
+    if (!jacobi_fused_ok(s, g) || (sweeps != 2 && sweeps != 4)) return hipErrorInvalidValue;
     TrArgs ct{};
     if (corr) {
         // the correction is a K = 2 interpolation onto exactly this grid
@@ -190,19 +204,26 @@ hipError_t launch_jacobi_fused(const Stencil& s, int dtype, const void* in, void
     }
     StarCoefs7 c;
     for (int k = 0; k < 7; k++) c.c[k] = s.coefs[k];
-    const int nb0 = (g.n[0] + JF_B - 1) / JF_B, nb1 = (g.n[1] + JF_B - 1) / JF_B,
-              nb2 = (g.n[2] + JF_B - 1) / JF_B;
+    const int B = jf_box(sweeps);
+    const int nb0 = (g.n[0] + B - 1) / B, nb1 = (g.n[1] + B - 1) / B, nb2 = (g.n[2] + B - 1) / B;
     const dim3 grid((unsigned)(nb0 * nb1 * nb2));
-    if (dtype == DT_F64)
-        launch_kernel(k_jacobi_fused<double, JF_SWEEPS>, grid, dim3(JF_NT), 0, st,
-                      (const double*)in, (double*)out, (const double*)rho, g, c, w, nb1, nb2,
-                      (const double*)corr, ct, (int)zero_in);
-    else if (dtype == DT_F32)
-        launch_kernel(k_jacobi_fused<float, JF_SWEEPS>, grid, dim3(JF_NT), 0, st,
-                      (const float*)in, (float*)out, (const float*)rho, g, c, w, nb1, nb2,
-                      (const float*)corr, ct, (int)zero_in);
-    else
+#define GPAW_JF(T, S)                                                                        \
+    launch_kernel(k_jacobi_fused<T, S>, grid, dim3(JF_NT), 0, st, (const T*)in, (T*)out,      \
+                  (const T*)rho, g, c, w, nb1, nb2, (const T*)corr, ct, (int)zero_in)
+    if (dtype == DT_F64) {
+        if (sweeps == 4)
+            GPAW_JF(double, 4);
+        else
+            GPAW_JF(double, 2);
+    } else if (dtype == DT_F32) {
+        if (sweeps == 4)
+            GPAW_JF(float, 4);
+        else
+            GPAW_JF(float, 2);
+    } else {
         return hipErrorInvalidValue;
+    }
+#undef GPAW_JF
     return launch_error();
 }
 

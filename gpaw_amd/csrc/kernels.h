@@ -117,11 +117,15 @@ struct StarCoefs7 {
 // `zero_in`: the caller guarantees in == 0 everywhere (a level's first
 // sweeps after the restriction zeroed it), so it is not read.
 struct Transfer;
+// jacobi_fused_sweeps: sweeps per launch on this grid (JF_SWEEPS, or 2 on
+// grids large enough for 12^3 output boxes to fill the chip); `sweeps` of
+// launch_jacobi_fused is 2 or 4.
 bool jacobi_fused_ok(const Stencil& s, const Geom& g);
+int jacobi_fused_sweeps(const Geom& g);
 hipError_t launch_jacobi_fused(const Stencil& s, int dtype, const void* in, void* out,
                                const void* rho, const Geom& g, double w, hipStream_t st,
                                const void* corr = nullptr, const Transfer* ctr = nullptr,
-                               bool zero_in = false);
+                               bool zero_in = false, int sweeps = JF_SWEEPS);
 
 // Launch one stencil pass.  `partial` (optional) receives one FP64 partial
 // sum of |result|^2 per thread block; `write` = false skips the store.

@@ -220,13 +220,14 @@ hipError_t relax_v(gpaw_poisson* ps, VLevels& V, int l, int k, void*& cur, void*
     Phases ph;
     gpaw_phases(nullptr, ph);
     gpaw_operator* op = ps->ops[l];
-    // below level 0, runs of JF_SWEEPS sweeps are one fused launch (callers
+    // below level 0, runs of 4 (2 on large levels) sweeps are one fused launch (callers
     // follow `cur`, not the sweep parity)
     if (l > 0 && ps->fuse_jacobi && jacobi_fused_ok(op->st, op->g)) {
-        for (; k >= JF_SWEEPS; k -= JF_SWEEPS) {
+        const int S = jacobi_fused_sweeps(op->g);
+        for (; k >= S; k -= S) {
             hipError_t e = launch_jacobi_fused(op->st, V.dt, cur, alt, V.rho[l], op->g, ps->w[l],
                                                st, corr, corr ? &ps->itp[l]->t : nullptr,
-                                               zero_in);
+                                               zero_in, S);
             corr = nullptr;
             zero_in = false;
             if (e != hipSuccess) return e;
@@ -290,8 +291,8 @@ hipError_t emit_v(gpaw_poisson* ps, MixedState* m, VLevels& V, int l, bool zero0
         void* coarse = nullptr;
         e = emit_v(ps, m, V, l + 1, false, coarse, st);
         if (e != hipSuccess) return e;
-        if (l > 0 && ps->fold_correction && ps->fuse_jacobi && m->post[l] >= JF_SWEEPS &&
-            jacobi_fused_ok(op->st, op->g) && ps->itp[l]->t.K == 2) {
+        if (l > 0 && ps->fold_correction && ps->fuse_jacobi && jacobi_fused_ok(op->st, op->g) &&
+            m->post[l] >= jacobi_fused_sweeps(op->g) && ps->itp[l]->t.K == 2) {
             e = relax_v(ps, V, l, m->post[l], cur, alt, st, coarse);
             res = cur;
             return e;

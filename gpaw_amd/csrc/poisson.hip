@@ -41,13 +41,14 @@ int alloc(double** p, long long n)
 bool folds_correction(const gpaw_poisson* ps, int l)
 {
     return ps->fold_correction && ps->method == 2 && l > 0 && l < ps->L && ps->fuse_jacobi &&
-           ps->post[l] >= JF_SWEEPS && jacobi_fused_ok(ps->ops[l]->st, ps->ops[l]->g) &&
+           jacobi_fused_ok(ps->ops[l]->st, ps->ops[l]->g) &&
+           ps->post[l] >= jacobi_fused_sweeps(ps->ops[l]->g) &&
            ps->itp[l]->t.K == 2;
 }
 
 // relax k sweeps on level l, ping-ponging between phi[l] and alt; returns via
 // `cur` which buffer holds the result.  On the levels below 0, runs of
-// JF_SWEEPS Jacobi sweeps are one fused launch (the result then sits in the
+// 4 (2 on large levels) Jacobi sweeps are one fused launch (the result then sits in the
 // other buffer than after as many single sweeps; callers follow `cur`).
 // `corr` (with folds_correction): the first fused launch first applies
 // phi -= I corr.  `zero_in`: phi is zero on entry (the first fused launch
@@ -81,10 +82,11 @@ hipError_t relax(gpaw_poisson* ps, int l, int k, double*& cur, double*& alt, hip
         return hipSuccess;
     }
     if (l > 0 && ps->fuse_jacobi && jacobi_fused_ok(op->st, op->g)) {
-        for (; k >= JF_SWEEPS; k -= JF_SWEEPS) {
+        const int S = jacobi_fused_sweeps(op->g);
+        for (; k >= S; k -= S) {
             hipError_t e = launch_jacobi_fused(op->st, DT_F64, cur, alt, ps->rho[l], op->g,
                                                ps->w[l], st, corr, corr ? &ps->itp[l]->t : nullptr,
-                                               zero_in);
+                                               zero_in, S);
             corr = nullptr;
             zero_in = false;
             if (e != hipSuccess) return e;

@@ -253,10 +253,12 @@ def test_reference_symmetry_case(relax):
     assert rel_maxnorm(b, ref) < 1e-13
 
 
-@pytest.mark.parametrize('N,pbc', [(40, True), (48, False), (64, True), (96, False)])
+@pytest.mark.parametrize('N,pbc', [(40, True), (48, False), (64, True), (96, False),
+                                   (160, True), (160, False)])
 def test_fused_jacobi_sweeps_identical(N, pbc, monkeypatch):
     """Levels below 0 run their Jacobi sweeps four at a time in one fused
-    launch, whose first post-smoothing launch also applies the coarse-grid
+    launch (two at a time on levels of 80^3 and 79^3, whose 12^3 output
+    boxes fill the chip), whose first post-smoothing launch also applies the coarse-grid
     correction; the potential and niter must be bit-identical to single
     sweeps (GPAW_AMD_FUSED_JACOBI=0) and to fused sweeps after a separate
     correction launch (GPAW_AMD_FOLD_CORRECTION=0); each V-cycle's error is
