@@ -127,6 +127,14 @@ hipError_t launch_jacobi_fused(const Stencil& s, int dtype, const void* in, void
                                const void* corr = nullptr, const Transfer* ctr = nullptr,
                                bool zero_in = false, int sweeps = JF_SWEEPS);
 
+// Residual of a 7-point star stencil and its K = 2 restriction in one
+// launch: rho_c = restrict(L phi - rho), and phi_c = 0 when given; the
+// same values as an ST_RESID pass followed by a TR_STORE restriction.
+bool resid_restrict_ok(const Stencil& s, const Geom& g, const Transfer& tr);
+hipError_t launch_resid_restrict(const Stencil& s, int dtype, const void* phi, const void* rho,
+                                 void* rho_c, void* phi_c, const Geom& g, const Transfer& tr,
+                                 hipStream_t st);
+
 // Launch one stencil pass.  `partial` (optional) receives one FP64 partial
 // sum of |result|^2 per thread block; `write` = false skips the store.
 hipError_t launch_stencil(const Stencil& s, int dtype, int mode, const void* in, void* out,

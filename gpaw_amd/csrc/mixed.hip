@@ -282,11 +282,16 @@ hipError_t emit_v(gpaw_poisson* ps, MixedState* m, VLevels& V, int l, bool zero0
         // below level 0, phi_l was zeroed by the restriction onto it
         e = relax_v(ps, V, l, pre, cur, alt, st, nullptr, l > 0);
         if (e != hipSuccess) return e;
-        e = launch_stencil(op->st, V.dt, ST_RESID, cur, V.res[l], V.rho[l], op->g, ph, 1, 0.0,
-                           nullptr, nullptr, true, st);
-        if (e != hipSuccess) return e;
-        e = launch_transfer(ps->rst[l]->t, V.dt, TR_STORE, V.res[l], V.rho[l + 1],
-                            V.phiA[l + 1], ph, 1, st);
+        if (ps->fuse_restrict && resid_restrict_ok(op->st, op->g, ps->rst[l]->t)) {
+            e = launch_resid_restrict(op->st, V.dt, cur, V.rho[l], V.rho[l + 1], V.phiA[l + 1],
+                                      op->g, ps->rst[l]->t, st);
+        } else {
+            e = launch_stencil(op->st, V.dt, ST_RESID, cur, V.res[l], V.rho[l], op->g, ph, 1, 0.0,
+                               nullptr, nullptr, true, st);
+            if (e == hipSuccess)
+                e = launch_transfer(ps->rst[l]->t, V.dt, TR_STORE, V.res[l], V.rho[l + 1],
+                                    V.phiA[l + 1], ph, 1, st);
+        }
         if (e != hipSuccess) return e;
         void* coarse = nullptr;
         e = emit_v(ps, m, V, l + 1, false, coarse, st);

@@ -124,12 +124,17 @@ hipError_t emit_level(gpaw_poisson* ps, int l, double*& cur0, double*& alt0, hip
         // below level 0, phi_l was zeroed by the restriction onto it
         e = relax(ps, l, ps->pre[l], cur, alt, st, nullptr, l > 0);
         if (e != hipSuccess) return e;
-        e = launch_stencil(op->st, DT_F64, ST_RESID, cur, ps->res[l], ps->rho[l], op->g, ph, 1,
-                           0.0, nullptr, nullptr, true, st);
-        if (e != hipSuccess) return e;
-        // rho_{l+1} = R r_l ; phi_{l+1} = 0
-        e = launch_transfer(ps->rst[l]->t, DT_F64, TR_STORE, ps->res[l], ps->rho[l + 1],
-                            ps->phiA[l + 1], ph, 1, st);
+        // r_l = L_l phi_l - rho_l ; rho_{l+1} = R r_l ; phi_{l+1} = 0
+        if (ps->fuse_restrict && resid_restrict_ok(op->st, op->g, ps->rst[l]->t)) {
+            e = launch_resid_restrict(op->st, DT_F64, cur, ps->rho[l], ps->rho[l + 1],
+                                      ps->phiA[l + 1], op->g, ps->rst[l]->t, st);
+        } else {
+            e = launch_stencil(op->st, DT_F64, ST_RESID, cur, ps->res[l], ps->rho[l], op->g, ph,
+                               1, 0.0, nullptr, nullptr, true, st);
+            if (e == hipSuccess)
+                e = launch_transfer(ps->rst[l]->t, DT_F64, TR_STORE, ps->res[l], ps->rho[l + 1],
+                                    ps->phiA[l + 1], ph, 1, st);
+        }
         if (e != hipSuccess) return e;
         // the next level starts from phiA[l+1]; ccur returns its result buffer
         double* ccur = nullptr;
@@ -366,6 +371,8 @@ extern "C" int gpaw_poisson_new(int levels, gpaw_operator* const* ops,
     ps->fuse_jacobi = !(env && env[0] == '0');
     env = getenv("GPAW_AMD_FOLD_CORRECTION");
     ps->fold_correction = !(env && env[0] == '0');
+    env = getenv("GPAW_AMD_FUSED_RESTRICT");
+    ps->fuse_restrict = !(env && env[0] == '0');
     env = getenv("GPAW_AMD_FUSED_NORM");
     ps->fuse_norm = !(env && env[0] == '0');
     int rc = GPAW_OK;

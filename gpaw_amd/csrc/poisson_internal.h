@@ -56,6 +56,9 @@ struct gpaw_poisson {
     // coarse-grid correction applied by the first fused post-smoothing launch
     // (GPAW_AMD_FOLD_CORRECTION=0: a separate interpolate-and-subtract launch)
     bool fold_correction = true;
+    // the residual and its restriction in one launch (GPAW_AMD_FUSED_RESTRICT=0:
+    // a residual pass, then a restriction pass)
+    bool fuse_restrict = true;
     // in a chunk, a V-cycle's error is summed by the next cycle's first
     // level-0 sweep (GPAW_AMD_FUSED_NORM=0: a norm pass per cycle)
     bool fuse_norm = true;

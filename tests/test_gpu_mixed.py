@@ -180,3 +180,27 @@ def test_preconditioner_vcycle(N, pbc):
     err = np.abs(z32.cpu().numpy() - want).max() / np.abs(want).max()
     print('  FP32 V-cycle vs FP64: %.2e' % err, flush=True)
     assert err < 1e-5
+
+
+@pytest.mark.parametrize('N,pbc', [(64, False), (48, True)])
+def test_vcycle_fused_restrict_identical(N, pbc, monkeypatch):
+    """The residual and its restriction in one launch or in two
+    (GPAW_AMD_FUSED_RESTRICT=0, read when the plan is built): the FP32 and
+    FP64 inner V-cycles give the same bits either way."""
+    print('case fused restrict', N, pbc, flush=True)
+    t = _torch()
+    from gpaw_amd.poisson import MixedPoissonSolver
+    gd, rho = _setup(N, 0.2 / BOHR, pbc, 5, N + 3)
+    rd = t.from_numpy(rho - rho.mean()).to('cuda')
+    out = []
+    for frest in ('1', '0'):
+        monkeypatch.setenv('GPAW_AMD_FUSED_RESTRICT', frest)
+        ms = MixedPoissonSolver(nn=3)
+        ms.set_grid_descriptor(gd)
+        ms.initialize()
+        z = [t.full(tuple(gd.n_c), np.nan, dtype=t.float64, device='cuda') for _ in range(2)]
+        ms.precondition(rd, z[0], 'fp32')
+        ms.precondition(rd, z[1], 'fp64')
+        out.append(z)
+    assert t.equal(out[0][0], out[1][0])
+    assert t.equal(out[0][1], out[1][1])

@@ -264,7 +264,9 @@ def test_fused_jacobi_sweeps_identical(N, pbc, monkeypatch):
     correction launch (GPAW_AMD_FOLD_CORRECTION=0); each V-cycle's error is
     summed by the next cycle's first sweep or by a norm pass of its own
     (GPAW_AMD_FUSED_NORM=0), with the same bits, so the same stopping cycle;
-    all read when the plan is built."""
+    the residual and its restriction are one launch or two
+    (GPAW_AMD_FUSED_RESTRICT=0) with the same bits; all read when the plan
+    is built."""
     print('case fused Jacobi', N, pbc, flush=True)
     torch = _torch()
     from gpaw_amd.grid_descriptor import GridDescriptor
@@ -274,10 +276,13 @@ def test_fused_jacobi_sweeps_identical(N, pbc, monkeypatch):
     rho = _density(og, N + 1, 4, 0.5 / BOHR, 1.5 / BOHR, 2.0 / BOHR)
     rho -= og.integrate(rho) / og.dv / og.size_of_global_array().prod()
     out = []
-    for fused, fold, fnorm in (('1', '1', '1'), ('1', '0', '1'), ('0', '1', '1'), ('1', '1', '0')):
+    for fused, fold, fnorm, frest in (('1', '1', '1', '1'), ('1', '0', '1', '1'),
+                                      ('0', '1', '1', '1'), ('1', '1', '0', '1'),
+                                      ('1', '1', '1', '0')):
         monkeypatch.setenv('GPAW_AMD_FUSED_JACOBI', fused)
         monkeypatch.setenv('GPAW_AMD_FOLD_CORRECTION', fold)
         monkeypatch.setenv('GPAW_AMD_FUSED_NORM', fnorm)
+        monkeypatch.setenv('GPAW_AMD_FUSED_RESTRICT', frest)
         gd = GridDescriptor((N,) * 3, (N * h,) * 3, pbc)
         gps = PoissonSolver(nn=3, relax='J')
         gps.set_grid_descriptor(gd)
