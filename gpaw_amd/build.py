@@ -25,7 +25,7 @@ SOURCES = ['stencil.hip', 'stencil_tile_f64.hip', 'stencil_tile_c128.hip',
            'misc.hip', 'capi.hip', 'poisson.hip', 'coarse.hip', 'mixed.hip', 'gs.hip',
            'precond.hip', 'batch.hip', 'batch_f64.hip', 'batch_c128.hip',
            'batch_f32.hip', 'batch_c64.hip', 'jacobi_fused.hip',
-           'resid_restrict.hip']
+           'resid_restrict.hip', 'jacobi_corr.hip']
 
 FLAGS = ['-O3', '-std=c++17', '-fPIC', '--offload-arch=' + ARCH,
          '-ffp-contract=off', '-Xclang', '-target-feature', '-Xclang',

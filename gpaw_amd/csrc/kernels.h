@@ -135,6 +135,15 @@ hipError_t launch_resid_restrict(const Stencil& s, int dtype, const void* phi, c
                                  void* rho_c, void* phi_c, const Geom& g, const Transfer& tr,
                                  hipStream_t st);
 
+// One weighted-Jacobi sweep of a star stencil (R = 1 .. 3, real data) whose
+// input is first corrected by phi -= I e (K = 2 interpolation `itp` of the
+// coarser level's e onto g): the same values as the interpolate-and-subtract
+// launch followed by the sweep.  `in` itself is not modified.
+bool jacobi_corr_ok(const Stencil& s, const Geom& g, const Transfer& itp);
+hipError_t launch_jacobi_corr(const Stencil& s, int dtype, const void* in, void* out,
+                              const void* rho, const void* corr, const Geom& g, double w,
+                              const Transfer& itp, hipStream_t st);
+
 // Launch one stencil pass.  `partial` (optional) receives one FP64 partial
 // sum of |result|^2 per thread block; `write` = false skips the store.
 hipError_t launch_stencil(const Stencil& s, int dtype, int mode, const void* in, void* out,

@@ -235,8 +235,11 @@ hipError_t relax_v(gpaw_poisson* ps, VLevels& V, int l, int k, void*& cur, void*
         }
     }
     for (int it = 0; it < k; it++) {
-        hipError_t e = launch_stencil(op->st, V.dt, ST_JACOBI, cur, alt, V.rho[l], op->g, ph, 1,
-                                      ps->w[l], nullptr, nullptr, true, st);
+        hipError_t e = corr ? launch_jacobi_corr(op->st, V.dt, cur, alt, V.rho[l], corr, op->g,
+                                                 ps->w[l], ps->itp[l]->t, st)
+                            : launch_stencil(op->st, V.dt, ST_JACOBI, cur, alt, V.rho[l], op->g,
+                                             ph, 1, ps->w[l], nullptr, nullptr, true, st);
+        corr = nullptr;
         if (e != hipSuccess) return e;
         std::swap(cur, alt);
     }
@@ -298,6 +301,13 @@ hipError_t emit_v(gpaw_poisson* ps, MixedState* m, VLevels& V, int l, bool zero0
         if (e != hipSuccess) return e;
         if (l > 0 && ps->fold_correction && ps->fuse_jacobi && jacobi_fused_ok(op->st, op->g) &&
             m->post[l] >= jacobi_fused_sweeps(op->g) && ps->itp[l]->t.K == 2) {
+            e = relax_v(ps, V, l, m->post[l], cur, alt, st, coarse);
+            res = cur;
+            return e;
+        }
+        if (l == 0 && ps->fold_correction && m->post[0] >= 1 &&
+            jacobi_corr_ok(op->st, op->g, ps->itp[0]->t)) {
+            // phi_0 -= I phi_1 inside the first post-smoothing sweep
             e = relax_v(ps, V, l, m->post[l], cur, alt, st, coarse);
             res = cur;
             return e;

@@ -46,6 +46,15 @@ bool folds_correction(const gpaw_poisson* ps, int l)
            ps->itp[l]->t.K == 2;
 }
 
+// Whether level 0's post-smoothing starts with a sweep that applies the
+// coarse-grid correction itself (never the cycle's first sweep, which may
+// sum the norm of the cycle's input).
+bool folds_correction0(const gpaw_poisson* ps, int l)
+{
+    return ps->fold_correction && ps->method == 2 && l == 0 && l < ps->L && ps->pre[0] >= 1 &&
+           ps->post[0] >= 1 && jacobi_corr_ok(ps->ops[0]->st, ps->ops[0]->g, ps->itp[0]->t);
+}
+
 // relax k sweeps on level l, ping-ponging between phi[l] and alt; returns via
 // `cur` which buffer holds the result.  On the levels below 0, runs of
 // 4 (2 on large levels) Jacobi sweeps are one fused launch (the result then sits in the
@@ -73,8 +82,16 @@ hipError_t relax(gpaw_poisson* ps, int l, int k, double*& cur, double*& alt, hip
             double* out = ++ps->rot.done == ps->rot.total ? ps->rot.dst
                                                           : ps->rot.t[(ps->rot.done - 1) & 1];
             int np = 0;
-            hipError_t e = launch_stencil(op->st, DT_F64, ST_JACOBI, cur, out, ps->rho[l], op->g,
-                                          ph, 1, ps->w[l], part, part ? &np : nullptr, true, st);
+            hipError_t e;
+            if (corr) {   // the correction's sweep never sums the cycle's input norm
+                e = part ? hipErrorInvalidValue
+                         : launch_jacobi_corr(op->st, DT_F64, cur, out, ps->rho[l], corr, op->g,
+                                              ps->w[l], ps->itp[l]->t, st);
+                corr = nullptr;
+            } else {
+                e = launch_stencil(op->st, DT_F64, ST_JACOBI, cur, out, ps->rho[l], op->g, ph, 1,
+                                   ps->w[l], part, part ? &np : nullptr, true, st);
+            }
             if (e == hipSuccess && part && np != ps->np0) e = hipErrorInvalidValue;
             if (e != hipSuccess) return e;
             cur = out;
@@ -96,8 +113,12 @@ hipError_t relax(gpaw_poisson* ps, int l, int k, double*& cur, double*& alt, hip
         }
     }
     for (int it = 0; it < k; it++) {
-        hipError_t e = launch_stencil(op->st, DT_F64, ST_JACOBI, cur, alt, ps->rho[l], op->g, ph,
-                                      1, ps->w[l], nullptr, nullptr, true, st);
+        hipError_t e =
+            corr ? launch_jacobi_corr(op->st, DT_F64, cur, alt, ps->rho[l], corr, op->g, ps->w[l],
+                                      ps->itp[l]->t, st)
+                 : launch_stencil(op->st, DT_F64, ST_JACOBI, cur, alt, ps->rho[l], op->g, ph, 1,
+                                  ps->w[l], nullptr, nullptr, true, st);
+        corr = nullptr;
         if (e != hipSuccess) return e;
         double* t = cur;
         cur = alt;
@@ -148,10 +169,16 @@ hipError_t emit_level(gpaw_poisson* ps, int l, double*& cur0, double*& alt0, hip
             alt0 = alt;
             return e;
         }
-        e = launch_transfer(ps->itp[l]->t, DT_F64, TR_SUBTRACT, ccur, cur, nullptr, ph, 1, st);
-        if (e != hipSuccess) return e;
+        if (folds_correction0(ps, l)) {   // phi_0 -= I phi_1 inside the first sweep
+            e = relax(ps, l, ps->post[l], cur, alt, st, ccur);
+        } else {
+            e = launch_transfer(ps->itp[l]->t, DT_F64, TR_SUBTRACT, ccur, cur, nullptr, ph, 1,
+                                st);
+            if (e == hipSuccess) e = relax(ps, l, ps->post[l], cur, alt, st);
+        }
+    } else {
+        e = relax(ps, l, ps->post[l], cur, alt, st);
     }
-    e = relax(ps, l, ps->post[l], cur, alt, st);
     if (e != hipSuccess) return e;
     if (l == 0 && !ps->skip_norm) {
         int np = 0;

@@ -182,19 +182,22 @@ def test_preconditioner_vcycle(N, pbc):
     assert err < 1e-5
 
 
-@pytest.mark.parametrize('N,pbc', [(64, False), (48, True)])
-def test_vcycle_fused_restrict_identical(N, pbc, monkeypatch):
+@pytest.mark.parametrize('N,pbc', [(64, False), (48, True), (96, False)])
+def test_vcycle_fusions_identical(N, pbc, monkeypatch):
     """The residual and its restriction in one launch or in two
-    (GPAW_AMD_FUSED_RESTRICT=0, read when the plan is built): the FP32 and
-    FP64 inner V-cycles give the same bits either way."""
-    print('case fused restrict', N, pbc, flush=True)
+    (GPAW_AMD_FUSED_RESTRICT=0), and the coarse-grid corrections applied by
+    the first post-smoothing sweeps or by launches of their own
+    (GPAW_AMD_FOLD_CORRECTION=0), both read when the plan is built: the FP32
+    and FP64 inner V-cycles give the same bits every way."""
+    print('case V-cycle fusions', N, pbc, flush=True)
     t = _torch()
     from gpaw_amd.poisson import MixedPoissonSolver
     gd, rho = _setup(N, 0.2 / BOHR, pbc, 5, N + 3)
     rd = t.from_numpy(rho - rho.mean()).to('cuda')
     out = []
-    for frest in ('1', '0'):
+    for frest, fold in (('1', '1'), ('0', '1'), ('1', '0')):
         monkeypatch.setenv('GPAW_AMD_FUSED_RESTRICT', frest)
+        monkeypatch.setenv('GPAW_AMD_FOLD_CORRECTION', fold)
         ms = MixedPoissonSolver(nn=3)
         ms.set_grid_descriptor(gd)
         ms.initialize()
@@ -202,5 +205,6 @@ def test_vcycle_fused_restrict_identical(N, pbc, monkeypatch):
         ms.precondition(rd, z[0], 'fp32')
         ms.precondition(rd, z[1], 'fp64')
         out.append(z)
-    assert t.equal(out[0][0], out[1][0])
-    assert t.equal(out[0][1], out[1][1])
+    for z in out[1:]:
+        assert t.equal(out[0][0], z[0])
+        assert t.equal(out[0][1], z[1])

@@ -259,9 +259,10 @@ def test_fused_jacobi_sweeps_identical(N, pbc, monkeypatch):
     """Levels below 0 run their Jacobi sweeps four at a time in one fused
     launch (two at a time on levels of 80^3 and 79^3, whose 12^3 output
     boxes fill the chip), whose first post-smoothing launch also applies the coarse-grid
-    correction; the potential and niter must be bit-identical to single
-    sweeps (GPAW_AMD_FUSED_JACOBI=0) and to fused sweeps after a separate
-    correction launch (GPAW_AMD_FOLD_CORRECTION=0); each V-cycle's error is
+    correction, as does level 0's first post-smoothing sweep; the potential
+    and niter must be bit-identical to single sweeps
+    (GPAW_AMD_FUSED_JACOBI=0) and to sweeps after separate correction
+    launches (GPAW_AMD_FOLD_CORRECTION=0); each V-cycle's error is
     summed by the next cycle's first sweep or by a norm pass of its own
     (GPAW_AMD_FUSED_NORM=0), with the same bits, so the same stopping cycle;
     the residual and its restriction are one launch or two
