@@ -34,13 +34,29 @@ struct CoarsePlan {
     CoarseLevel lev[GPAW_MAX_LEVELS];
 };
 
+// Arguments of the single-level kernel (a plan of one level): its post
+// sweeps from phi = 0, the result stored in `out` (phiA after an even
+// number of sweeps, phiB after an odd one, where single sweeps leave it).
+struct CoarseOne {
+    Geom g;
+    double c[7];
+    double w;
+    int sweeps;
+    const void* rho;   // double, or float for the FP64 solve inside an FP32 V-cycle
+    void* out;
+};
+
 TrArgs make_trargs(const Transfer& tr);
 // neighbour table of one level (6 entries per point, see CoarsePlan::nbtab)
 void coarse_neighbour_table(const Geom& g, short* tab);
-hipError_t launch_coarse_vcycle(const CoarsePlan* d_plan, int lds_bytes, hipStream_t st);
+// h_plan: the host copy of *d_plan (a one-level plan takes the single-level
+// kernel, whose arguments come from it)
+hipError_t launch_coarse_vcycle(const CoarsePlan* d_plan, const CoarsePlan& h_plan, int lds_bytes,
+                                hipStream_t st);
 hipError_t coarse_set_lds_limit(int bytes);
 // FP64 coarse solve inside an FP32 V-cycle: first-level rho / phi are float
 // (LDS mode only)
-hipError_t launch_coarse_vcycle_f32(const CoarsePlan* d_plan, int lds_bytes, hipStream_t st);
+hipError_t launch_coarse_vcycle_f32(const CoarsePlan* d_plan, const CoarsePlan& h_plan,
+                                    int lds_bytes, hipStream_t st);
 
 }  // namespace gpaw

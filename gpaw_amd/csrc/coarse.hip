@@ -242,6 +242,101 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
     }
 }
 
+// The single-coarse-level case (the hierarchy's last level alone, as for a
+// 159^3 problem, whose 19^3 level is too large for one workgroup and 9^3
+// is the last): post-smoothing sweeps only, in a compact kernel whose
+// parameters arrive as kernel arguments.  The generic kernel first copies
+// its plan and neighbour table in, two dependent round trips; here the
+// neighbour indices are computed per point.  Same sweep arithmetic as
+// jacobi_sweeps.
+namespace {
+
+// flat index of (j0, j1, j2), wrapped on periodic axes, -1 outside a zero
+// boundary (the neighbour table's rule)
+__device__ __forceinline__ int one_nb(int j0, int j1, int j2, const Geom& g)
+{
+    const int j[3] = {j0, j1, j2};
+    int w[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        w[a] = j[a];
+        if (w[a] < 0 || w[a] >= g.n[a]) {
+            if (!g.per[a]) return -1;
+            w[a] = w[a] < 0 ? w[a] + g.n[a] : w[a] - g.n[a];
+        }
+    }
+    return (w[0] * g.n[1] + w[1]) * g.n[2] + w[2];
+}
+
+template <class TB>
+__global__ __launch_bounds__(1024) void k_coarse_one(CoarseOne p)
+{
+    extern __shared__ double smem[];
+    const int n = p.g.n[0] * p.g.n[1] * p.g.n[2];
+    double* A = smem;
+    double* B = smem + n;
+    double* rho = smem + 2 * n;
+    Pts q;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int idx = threadIdx.x + 1024 * k;
+        q.idx[k] = idx < n ? idx : -1;
+        int i0 = 0, i1 = 0, i2 = 0;
+        if (idx < n) split(idx, p.g, i0, i1, i2);
+        q.nb[k][0] = one_nb(i0, i1, i2 + 1, p.g);
+        q.nb[k][1] = one_nb(i0, i1, i2 - 1, p.g);
+        q.nb[k][2] = one_nb(i0, i1 + 1, i2, p.g);
+        q.nb[k][3] = one_nb(i0, i1 - 1, i2, p.g);
+        q.nb[k][4] = one_nb(i0 + 1, i1, i2, p.g);
+        q.nb[k][5] = one_nb(i0 - 1, i1, i2, p.g);
+        if (idx < n) {
+            rho[idx] = (double)((const TB*)p.rho)[idx];
+            A[idx] = 0.0;
+        }
+    }
+    __syncthreads();
+    double c[7];
+#pragma unroll
+    for (int m = 0; m < 7; m++) c[m] = p.c[m];
+    const double w = p.w;
+    double* src = A;
+    double* dst = B;
+    for (int it = 0; it < p.sweeps; it++) {
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int idx = q.idx[k];
+            if (idx < 0) continue;
+            const double x = nb_sum(src, q.nb[k], c, 0.0);
+            dst[idx] = (1.0 - w) * src[idx] + w * (rho[idx] - x) / c[0];
+        }
+        __syncthreads();
+        double* t = src;
+        src = dst;
+        dst = t;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+        if (q.idx[k] >= 0) ((TB*)p.out)[q.idx[k]] = (TB)src[q.idx[k]];
+}
+
+}  // namespace
+
+// the single-level kernel's arguments from a one-level plan
+static bool coarse_one_args(const CoarsePlan& h, CoarseOne& a)
+{
+    if (h.nlev != 1) return false;
+    const CoarseLevel& L = h.lev[0];
+    const long n = (long)L.g.n[0] * L.g.n[1] * L.g.n[2];
+    if (n < 1 || n > 2048) return false;
+    a.g = L.g;
+    for (int m = 0; m < 7; m++) a.c[m] = L.c[m];
+    a.w = L.w;
+    a.sweeps = L.post;
+    a.rho = L.rho;
+    a.out = (L.post & 1) ? L.phiB : L.phiA;
+    return true;
+}
+
 static int nb_host(int j0, int j1, int j2, const Geom& g)
 {
     const int j[3] = {j0, j1, j2};
@@ -282,8 +377,16 @@ hipError_t coarse_set_lds_limit(int bytes)
 }
 
 // lds_bytes > 0: the plan keeps its level arrays in LDS (plan->lds)
-hipError_t launch_coarse_vcycle(const CoarsePlan* d_plan, int lds_bytes, hipStream_t st)
+hipError_t launch_coarse_vcycle(const CoarsePlan* d_plan, const CoarsePlan& h_plan, int lds_bytes,
+                                hipStream_t st)
 {
+    CoarseOne one;
+    if (coarse_one_args(h_plan, one)) {
+        const int n = one.g.n[0] * one.g.n[1] * one.g.n[2];
+        launch_kernel(k_coarse_one<double>, dim3(1), dim3(1024), 3 * n * (int)sizeof(double), st,
+                      one);
+        return launch_error();
+    }
     if (lds_bytes > 0)
         launch_kernel(k_coarse_vcycle<double, true>, dim3(1), dim3(1024), lds_bytes, st, d_plan);
     else
@@ -291,8 +394,16 @@ hipError_t launch_coarse_vcycle(const CoarsePlan* d_plan, int lds_bytes, hipStre
     return launch_error();
 }
 
-hipError_t launch_coarse_vcycle_f32(const CoarsePlan* d_plan, int lds_bytes, hipStream_t st)
+hipError_t launch_coarse_vcycle_f32(const CoarsePlan* d_plan, const CoarsePlan& h_plan,
+                                    int lds_bytes, hipStream_t st)
 {
+    CoarseOne one;
+    if (coarse_one_args(h_plan, one)) {
+        const int n = one.g.n[0] * one.g.n[1] * one.g.n[2];
+        launch_kernel(k_coarse_one<float>, dim3(1), dim3(1024), 3 * n * (int)sizeof(double), st,
+                      one);
+        return launch_error();
+    }
     if (lds_bytes <= 0) return hipErrorInvalidValue;   // float levels need LDS mode
     launch_kernel(k_coarse_vcycle<float, true>, dim3(1), dim3(1024), lds_bytes, st, d_plan);
     return launch_error();

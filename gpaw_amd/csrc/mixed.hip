@@ -46,6 +46,7 @@ struct VLevels {
     std::vector<void*> phiA, phiB, rho, res;
     std::vector<void*> owned;
     CoarsePlan* d_coarse = nullptr;     // fused coarse levels (FP64 inside)
+    CoarsePlan h_coarse{};              // its host copy
     bool own_coarse = false;
     int lc = -1;                        // first fused coarse level (-1: none)
 };
@@ -256,8 +257,8 @@ hipError_t emit_v(gpaw_poisson* ps, MixedState* m, VLevels& V, int l, bool zero0
     if (l == V.lc) {   // the fused coarse kernel leaves it where single sweeps would
         const int kc = (l < m->L ? m->pre[l] : 0) + m->post[l];
         res = (kc % 2) ? V.phiB[l] : V.phiA[l];
-        return V.dt == DT_F32 ? launch_coarse_vcycle_f32(V.d_coarse, ps->coarse_lds, st)
-                              : launch_coarse_vcycle(V.d_coarse, ps->coarse_lds, st);
+        return V.dt == DT_F32 ? launch_coarse_vcycle_f32(V.d_coarse, V.h_coarse, ps->coarse_lds, st)
+                              : launch_coarse_vcycle(V.d_coarse, V.h_coarse, ps->coarse_lds, st);
     }
     gpaw_operator* op = ps->ops[l];
     void* cur = V.phiA[l];
@@ -368,6 +369,7 @@ int setup_v32(gpaw_poisson* ps, MixedState* m)
         P.lev[0].rho = (double*)V.rho[ps->lc];
         P.lev[0].phiA = (double*)V.phiA[ps->lc];
         P.lev[0].phiB = (double*)V.phiB[ps->lc];
+        V.h_coarse = P;
         rc = gpaw_check(hipMalloc((void**)&V.d_coarse, sizeof P), "mixed: coarse plan");
         if (!rc) {
             V.own_coarse = true;
@@ -404,6 +406,7 @@ int setup_v64(gpaw_poisson* ps, MixedState* m)
     }
     if (ps->lc > 0) {
         V.d_coarse = ps->d_coarse;
+        V.h_coarse = ps->h_coarse;
         V.lc = ps->lc;
     }
     return GPAW_OK;

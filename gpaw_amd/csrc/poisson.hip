@@ -135,7 +135,7 @@ hipError_t emit_level(gpaw_poisson* ps, int l, double*& cur0, double*& alt0, hip
         // the fused coarse kernel leaves its result where single sweeps would
         const int k = (l < ps->L ? ps->pre[l] : 0) + ps->post[l];
         cur0 = (ps->method == 2 && k % 2) ? ps->phiB[l] : ps->phiA[l];
-        return launch_coarse_vcycle(ps->d_coarse, ps->coarse_lds, st);
+        return launch_coarse_vcycle(ps->d_coarse, ps->h_coarse, ps->coarse_lds, st);
     }
     double* cur = l == 0 ? cur0 : ps->phiA[l];
     double* alt = l == 0 ? alt0 : ps->phiB[l];

@@ -266,8 +266,11 @@ def test_fused_jacobi_sweeps_identical(N, pbc, monkeypatch):
     summed by the next cycle's first sweep or by a norm pass of its own
     (GPAW_AMD_FUSED_NORM=0), with the same bits, so the same stopping cycle;
     the residual and its restriction are one launch or two
-    (GPAW_AMD_FUSED_RESTRICT=0) with the same bits; all read when the plan
-    is built."""
+    (GPAW_AMD_FUSED_RESTRICT=0) with the same bits; the smallest levels run
+    in one workgroup (the single-level kernel when only the last level is
+    that small, as for 160^3) or as separate launches
+    (GPAW_AMD_COARSE_MAX=0) with the same bits; all read when the plan is
+    built."""
     print('case fused Jacobi', N, pbc, flush=True)
     torch = _torch()
     from gpaw_amd.grid_descriptor import GridDescriptor
@@ -277,13 +280,17 @@ def test_fused_jacobi_sweeps_identical(N, pbc, monkeypatch):
     rho = _density(og, N + 1, 4, 0.5 / BOHR, 1.5 / BOHR, 2.0 / BOHR)
     rho -= og.integrate(rho) / og.dv / og.size_of_global_array().prod()
     out = []
-    for fused, fold, fnorm, frest in (('1', '1', '1', '1'), ('1', '0', '1', '1'),
-                                      ('0', '1', '1', '1'), ('1', '1', '0', '1'),
-                                      ('1', '1', '1', '0')):
+    for fused, fold, fnorm, frest, cmax in (('1', '1', '1', '1', '2000'),
+                                            ('1', '0', '1', '1', '2000'),
+                                            ('0', '1', '1', '1', '2000'),
+                                            ('1', '1', '0', '1', '2000'),
+                                            ('1', '1', '1', '0', '2000'),
+                                            ('1', '1', '1', '1', '0')):
         monkeypatch.setenv('GPAW_AMD_FUSED_JACOBI', fused)
         monkeypatch.setenv('GPAW_AMD_FOLD_CORRECTION', fold)
         monkeypatch.setenv('GPAW_AMD_FUSED_NORM', fnorm)
         monkeypatch.setenv('GPAW_AMD_FUSED_RESTRICT', frest)
+        monkeypatch.setenv('GPAW_AMD_COARSE_MAX', cmax)
         gd = GridDescriptor((N,) * 3, (N * h,) * 3, pbc)
         gps = PoissonSolver(nn=3, relax='J')
         gps.set_grid_descriptor(gd)
