@@ -29,6 +29,8 @@
 // interpolated at their unwrapped coordinates: on a periodic axis a shift
 // by the fine period shifts the coarse index by the coarse period, so the
 // value equals the one at the wrapped point.
+#include <cstdlib>
+
 #include "kernels.h"
 #include "transfer_common.h"
 
@@ -38,13 +40,18 @@ TrArgs make_trargs(const Transfer& tr);
 
 namespace {
 
-// A launch runs S = 4 or 2 sweeps on a 16^3 LDS box whose output is the
-// inner B^3 = (16 - 2S)^3 cells: 8^3 outputs for four sweeps, 12^3 for two.
-// Two-sweep launches recompute fewer halo cells per output (4472 cell
-// updates per 1728 outputs against 5984 per 512) at one more launch per
-// four sweeps: the better trade on levels with enough 12^3 boxes to fill
-// the chip.
-constexpr int jf_box(int S) { return 16 - 2 * S; }
+// A launch runs S = 4 or 2 sweeps on an E^3 LDS box (E = 16 or 12) whose
+// output is the inner B^3 = (E - 2S)^3 cells: 8^3 outputs for four sweeps
+// on 16^3, 12^3 for two.  Two-sweep launches recompute fewer halo cells per
+// output (4472 cell updates per 1728 outputs against 5984 per 512) at one
+// more launch per four sweeps: the better trade on levels with enough 12^3
+// boxes to fill the chip.  12^3 boxes (4^3 outputs for four sweeps) serve
+// levels too small for enough 8^3 boxes: more recomputed halo per output,
+// but eight times the blocks.
+constexpr int jf_box(int S, int E) { return E - 2 * S; }
+// threads of a block: one per (y, z) column of the box and run of PX planes
+constexpr int jf_threads(int E) { return E == 16 ? 1024 : 576; }
+constexpr int jf_cb(int E) { return E / 2 + 2; }   // staged coarse values per axis
 
 __device__ __forceinline__ int jf_wrap(int j, int n, int per, bool& out)
 {
@@ -57,11 +64,10 @@ __device__ __forceinline__ int jf_wrap(int j, int n, int per, bool& out)
     return j < 0 ? j + n : j;
 }
 
-// 1024 threads: one (y, z) column of the box per 256 threads, the box's E
-// planes split into four runs of PX planes (four waves per SIMD hide the
-// LDS and division latencies of the sweeps).
-constexpr int JF_NT = 1024;
-constexpr int JF_CB = 10;   // staged coarse values per axis (16 fine cells span <= 9, +1)
+// 16^3 boxes: 1024 threads, one (y, z) column of the box per 256 threads,
+// the box's 16 planes split into four runs of four (four waves per SIMD hide
+// the LDS and division latencies of the sweeps); 12^3 boxes: 576 threads,
+// four runs of three planes.
 
 // index j of a coarse axis of extent n for the staging: wrapped (any number
 // of periods) or flagged outside a zero-boundary axis
@@ -70,16 +76,18 @@ __device__ __forceinline__ int jf_cwrap(int j, int n, int per, bool& out)
     return jf_wrap(j, n, per, out);
 }
 
-template <class V, int S>
-__global__ __launch_bounds__(JF_NT) void k_jacobi_fused(const V* __restrict__ in, V* __restrict__ out,
+template <class V, int S, int E>
+__global__ __launch_bounds__(jf_threads(E)) void k_jacobi_fused(const V* __restrict__ in, V* __restrict__ out,
                                                         const V* __restrict__ rho, Geom g,
                                                         StarCoefs7 c, double w, int nb1, int nb2,
                                                         const V* __restrict__ corr, TrArgs ct,
                                                         int zero_in)
 {
-    constexpr int B = jf_box(S);      // output box edge
-    constexpr int E = B + 2 * S;      // box edge with halo
-    static_assert(E * E == 256, "one thread per (y, z) column of the box and plane run");
+    constexpr int B = jf_box(S, E);   // output box edge
+    constexpr int JF_NT = jf_threads(E);
+    constexpr int JF_CB = jf_cb(E);
+    static_assert(JF_NT % (E * E) == 0 && E % (JF_NT / (E * E)) == 0,
+                  "one thread per (y, z) column of the box and plane run");
     constexpr int PX = E * E * E / JF_NT;   // planes per thread
     __shared__ V P[2][E * E * E];
     __shared__ V C[JF_CB * JF_CB * JF_CB];
@@ -99,7 +107,6 @@ __global__ __launch_bounds__(JF_NT) void k_jacobi_fused(const V* __restrict__ in
     // the coarse values of the correction: unwrapped coarse indices from cm
     int cm0 = 0, cm1 = 0, cm2 = 0;
     if (corr) {
-        static_assert(E == 16, "the staging covers 16 fine cells per axis");
         cm0 = ((o0 + ct.skip[0][0]) >> 1) - ct.lo[0];
         cm1 = ((o1 + ct.skip[1][0]) >> 1) - ct.lo[1];
         cm2 = ((o2 + ct.skip[2][0]) >> 1) - ct.lo[2];
@@ -182,6 +189,21 @@ bool jacobi_fused_ok(const Stencil& s, const Geom& g)
     return s.star_r == 1 && g.n[0] >= 1 && g.n[1] >= 1 && g.n[2] >= 1;
 }
 
+int jacobi_fused_edge(const Geom& g, int sweeps)
+{
+    static const int forced = [] {
+        const char* e = getenv("GPAW_AMD_JF_BOX");
+        return e ? atoi(e) : 0;
+    }();
+    if (forced == 16 || forced == 12) return forced;
+    // 16^3 boxes down to 64 of them (39^3 with four sweeps: 125); below,
+    // 12^3 boxes (23^3: 27 boxes of 16^3 against 216 of 12^3).  Measured
+    // both ways on 47^3, 39^3 and 23^3 levels.
+    const int B = jf_box(sweeps, 16);
+    const long nb = (long)((g.n[0] + B - 1) / B) * ((g.n[1] + B - 1) / B) * ((g.n[2] + B - 1) / B);
+    return nb >= 64 ? 16 : 12;
+}
+
 int jacobi_fused_sweeps(const Geom& g)
 {
     // two sweeps per launch once the 12^3 boxes fill the chip (256 CUs)
@@ -204,25 +226,34 @@ hipError_t launch_jacobi_fused(const Stencil& s, int dtype, const void* in, void
     }
     StarCoefs7 c;
     for (int k = 0; k < 7; k++) c.c[k] = s.coefs[k];
-    const int B = jf_box(sweeps);
+    const int E = jacobi_fused_edge(g, sweeps);
+    const int B = jf_box(sweeps, E);
     const int nb0 = (g.n[0] + B - 1) / B, nb1 = (g.n[1] + B - 1) / B, nb2 = (g.n[2] + B - 1) / B;
     const dim3 grid((unsigned)(nb0 * nb1 * nb2));
-#define GPAW_JF(T, S)                                                                        \
-    launch_kernel(k_jacobi_fused<T, S>, grid, dim3(JF_NT), 0, st, (const T*)in, (T*)out,      \
-                  (const T*)rho, g, c, w, nb1, nb2, (const T*)corr, ct, (int)zero_in)
+#define GPAW_JF(T, S, EE)                                                                    \
+    launch_kernel(k_jacobi_fused<T, S, EE>, grid, dim3(jf_threads(EE)), 0, st, (const T*)in,  \
+                  (T*)out, (const T*)rho, g, c, w, nb1, nb2, (const T*)corr, ct, (int)zero_in)
+#define GPAW_JF_E(T, S)                                                                      \
+    if (E == 16)                                                                             \
+        GPAW_JF(T, S, 16);                                                                   \
+    else                                                                                     \
+        GPAW_JF(T, S, 12);
     if (dtype == DT_F64) {
-        if (sweeps == 4)
-            GPAW_JF(double, 4);
-        else
-            GPAW_JF(double, 2);
+        if (sweeps == 4) {
+            GPAW_JF_E(double, 4)
+        } else {
+            GPAW_JF_E(double, 2)
+        }
     } else if (dtype == DT_F32) {
-        if (sweeps == 4)
-            GPAW_JF(float, 4);
-        else
-            GPAW_JF(float, 2);
+        if (sweeps == 4) {
+            GPAW_JF_E(float, 4)
+        } else {
+            GPAW_JF_E(float, 2)
+        }
     } else {
         return hipErrorInvalidValue;
     }
+#undef GPAW_JF_E
 #undef GPAW_JF
     return launch_error();
 }

@@ -122,6 +122,9 @@ struct Transfer;
 // launch_jacobi_fused is 2 or 4.
 bool jacobi_fused_ok(const Stencil& s, const Geom& g);
 int jacobi_fused_sweeps(const Geom& g);
+// LDS box edge of a fused launch of `sweeps` sweeps on g (16, or 12 on grids
+// with too few 16^3 boxes; GPAW_AMD_JF_BOX=16 or 12 forces one)
+int jacobi_fused_edge(const Geom& g, int sweeps);
 hipError_t launch_jacobi_fused(const Stencil& s, int dtype, const void* in, void* out,
                                const void* rho, const Geom& g, double w, hipStream_t st,
                                const void* corr = nullptr, const Transfer* ctr = nullptr,
