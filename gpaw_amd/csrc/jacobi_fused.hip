@@ -104,25 +104,10 @@ __global__ __launch_bounds__(jf_threads(E)) void k_jacobi_fused(const V* __restr
     const long plane = (long)n1 * n2;
     typedef typename VT<V>::R RT;
     const RT wr = (RT)w;
-    // the coarse values of the correction: unwrapped coarse indices from cm
-    int cm0 = 0, cm1 = 0, cm2 = 0;
-    if (corr) {
-        cm0 = ((o0 + ct.skip[0][0]) >> 1) - ct.lo[0];
-        cm1 = ((o1 + ct.skip[1][0]) >> 1) - ct.lo[1];
-        cm2 = ((o2 + ct.skip[2][0]) >> 1) - ct.lo[2];
-        for (int i = threadIdx.x; i < JF_CB * JF_CB * JF_CB; i += JF_NT) {
-            bool o = false;
-            const int c0 = jf_cwrap(cm0 + i / (JF_CB * JF_CB), ct.gin.n[0], ct.gin.per[0], o);
-            const int c1 = jf_cwrap(cm1 + (i / JF_CB) % JF_CB, ct.gin.n[1], ct.gin.per[1], o);
-            const int c2 = jf_cwrap(cm2 + i % JF_CB, ct.gin.n[2], ct.gin.per[2], o);
-            const V v = corr[o ? 0 : ((long)c0 * ct.gin.n[1] + c1) * ct.gin.n[2] + c2];
-            C[i] = o ? V(0) : v;
-        }
-        __syncthreads();
-    }
-    // load the box (zero outside a zero-boundary axis) and rho of this
-    // thread's cells
-    V r[PX];
+    // every global load is issued before the first is used: this thread's
+    // box cells (zero outside a zero-boundary axis) and their rho, then the
+    // coarse values of the correction
+    V r[PX], v[PX];
     unsigned outside = out12 ? ~0u : 0u;   // bit p: cell (x0 + p, ty, tz) is outside
 #pragma unroll
     for (int p = 0; p < PX; p++) {
@@ -130,20 +115,53 @@ __global__ __launch_bounds__(jf_threads(E)) void k_jacobi_fused(const V* __restr
         const int j0 = jf_wrap(o0 + x0 + p, n0, g.per[0], o);
         if (o) outside |= 1u << p;
         const long idx = (long)j0 * plane + col;
-        V v = zero_in ? V(0) : in[o ? 0 : idx];
+        v[p] = zero_in ? V(0) : in[o ? 0 : idx];
         r[p] = rho[o ? 0 : idx];
+    }
+    // the coarse values: unwrapped coarse indices from cm
+    int cm0 = 0, cm1 = 0, cm2 = 0;
+    if (corr) {
+        constexpr int NCS = JF_CB * JF_CB * JF_CB;
+        constexpr int NC = (NCS + JF_NT - 1) / JF_NT;
+        cm0 = ((o0 + ct.skip[0][0]) >> 1) - ct.lo[0];
+        cm1 = ((o1 + ct.skip[1][0]) >> 1) - ct.lo[1];
+        cm2 = ((o2 + ct.skip[2][0]) >> 1) - ct.lo[2];
+        V cv[NC];
+        unsigned cmask = 0;   // bit k: staged value k of this thread is a zero ghost
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            const int i = threadIdx.x + k * JF_NT;
+            cv[k] = V(0);
+            if (i < NCS) {
+                bool o = false;
+                const int c0 = jf_cwrap(cm0 + i / (JF_CB * JF_CB), ct.gin.n[0], ct.gin.per[0], o);
+                const int c1 = jf_cwrap(cm1 + (i / JF_CB) % JF_CB, ct.gin.n[1], ct.gin.per[1], o);
+                const int c2 = jf_cwrap(cm2 + i % JF_CB, ct.gin.n[2], ct.gin.per[2], o);
+                cv[k] = corr[o ? 0 : ((long)c0 * ct.gin.n[1] + c1) * ct.gin.n[2] + c2];
+                if (o) cmask |= 1u << k;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NC; k++)
+            if (threadIdx.x + k * JF_NT < NCS)
+                C[threadIdx.x + k * JF_NT] = ((cmask >> k) & 1) ? V(0) : cv[k];
+        __syncthreads();
+    }
+#pragma unroll
+    for (int p = 0; p < PX; p++) {
+        const bool o = (outside >> p) & 1;
         if (corr && !o) {
             const int q0 = o0 + x0 + p + ct.skip[0][0], q1 = o1 + ty + ct.skip[1][0],
                       q2 = o2 + tz + ct.skip[2][0];
             const int l0 = (q0 >> 1) - ct.lo[0] - cm0, l1 = (q1 >> 1) - ct.lo[1] - cm1,
                       l2 = (q2 >> 1) - ct.lo[2] - cm2;
-            v = vsub(v, interp2_combine<V>(
-                            [&](int d0, int d1, int d2) {
-                                return C[((l0 + d0) * JF_CB + l1 + d1) * JF_CB + l2 + d2];
-                            },
-                            q0 & 1, q1 & 1, q2 & 1));
+            v[p] = vsub(v[p], interp2_combine<V>(
+                                  [&](int d0, int d1, int d2) {
+                                      return C[((l0 + d0) * JF_CB + l1 + d1) * JF_CB + l2 + d2];
+                                  },
+                                  q0 & 1, q1 & 1, q2 & 1));
         }
-        P[0][((x0 + p) * E + ty) * E + tz] = o ? V(0) : v;
+        P[0][((x0 + p) * E + ty) * E + tz] = o ? V(0) : v[p];
     }
     __syncthreads();
 #pragma unroll
