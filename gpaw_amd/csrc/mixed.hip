@@ -65,15 +65,28 @@ struct MixedState {
     hipGraphExec_t g_vwarm[2] = {nullptr, nullptr};    // one V-cycle from the last result
     hipGraphExec_t g_a[2] = {nullptr, nullptr};        // PCG: q = Lp, alpha, update, |d|^2
     hipGraphExec_t g_b[2] = {nullptr, nullptr};        // PCG: z = V(d), beta, p = z + beta p
-    // k PCG steps (b then a) per launch, k = 1 .. MIXED_CHUNK_MAX
-    std::vector<hipGraphExec_t> g_ba[2];
+    // k PCG steps (b then a) per launch, k = 1 .. MIXED_CHUNK_MAX; [1]: the
+    // same followed by the true residual's norm (chunks expected to converge)
+    std::vector<hipGraphExec_t> g_ba[2][2];
+    // PCG steps the last converged solve took, per inner precision: the
+    // first chunk of the next solve
+    int hint[2] = {0, 0};
     int tuned[2] = {0, 0};              // launch shapes tuned (tuner on)
 };
 
-#define MIXED_CHUNK_MAX 8
+#define MIXED_CHUNK_MAX 16
+#define MIXED_PREDICT_MAX 8   // steps per rate-predicted chunk
 
 // device scalars
-enum { S_DZ = 0, S_PQ = 1, S_DD = 2, S_RR = 4, S_ALPHA = 5, S_N = 8 };
+// S_DZ = 0 marks a (re)start: the next direction update takes p = z.
+// S_D0: |d|^2 of the residual a solve starts from; S_DT: |L phi - rho|^2
+// after a chunk that ends with the true residual.
+// S_HIST + i: |d|^2 after step i of a chunk (all but its last step; the
+// last one's is S_DD).
+enum {
+    S_DZ = 0, S_PQ = 1, S_DD = 2, S_D0 = 3, S_RR = 4, S_ALPHA = 5, S_DT = 6, S_HIST = 8,
+    S_N = S_HIST + MIXED_CHUNK_MAX
+};
 
 namespace {
 
@@ -132,7 +145,8 @@ __global__ __launch_bounds__(256) void k_pcg_update(double* __restrict__ phi, do
     __shared__ double red[256];
     const double pq = npq > 0 ? sum_partials(pqp, npq, red) : S[S_PQ];
     const double dz = sum_partials(dzp, ndz, red);
-    const double alpha = dz / pq;
+    // pq = 0 only when p = 0 (a zero residual): no step
+    const double alpha = pq != 0.0 ? dz / pq : 0.0;
     double s = 0.0;
     for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256) {
         phi[i] = phi[i] - alpha * p[i];
@@ -173,17 +187,27 @@ __global__ __launch_bounds__(256) void k_dot_dz_zq(const double* __restrict__ d,
 
 // p = z + beta p, beta = -alpha (z.q) / (d_old.z_old) (Polak-Ribiere), z.q
 // summed by every block from the nzq partials at zqp, d_old.z_old = S[DZ]
-// (kept by the previous update); first: p = z
+// (kept by the previous update); first, or S[DZ] = 0 (a restart): p = z.
+// hist: block 0 also sums the previous step's |d|^2 partials (ndd at ddp)
+// into *hist, so the host sees |d| after every step of a chunk.
 template <class T>
 __global__ __launch_bounds__(256) void k_p_update(double* __restrict__ p, const T* __restrict__ z,
-                                                  const double* __restrict__ S, i64 n, int first,
-                                                  const double* __restrict__ zqp, int nzq)
+                                                  double* __restrict__ S, i64 n, int first,
+                                                  const double* __restrict__ zqp, int nzq,
+                                                  const double* __restrict__ ddp, int ndd,
+                                                  double* __restrict__ hist)
 {
     __shared__ double red[256];
-    const double zq = first ? 0.0 : sum_partials(zqp, nzq, red);
-    const double beta = first ? 0.0 : -S[S_ALPHA] * zq / S[S_DZ];
+    if (hist && blockIdx.x == 0) {
+        const double v = sum_partials(ddp, ndd, red);
+        if (threadIdx.x == 0) *hist = v;
+    }
+    const double dzo = S[S_DZ];
+    const bool f = first || dzo == 0.0;   // uniform over the grid
+    const double zq = f ? 0.0 : sum_partials(zqp, nzq, red);
+    const double beta = f ? 0.0 : -S[S_ALPHA] * zq / dzo;
     for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256)
-        p[i] = first ? (double)z[i] : (double)z[i] + beta * p[i];
+        p[i] = f ? (double)z[i] : (double)z[i] + beta * p[i];
 }
 
 // The first Jacobi sweep from phi = 0: every stencil term is zero, so the
@@ -451,10 +475,11 @@ void destroy_graphs(MixedState* m, int pr)
         if (*g) hipGraphExecDestroy(*g);
         *g = nullptr;
     }
-    for (hipGraphExec_t& g : m->g_ba[pr]) {
-        if (g) hipGraphExecDestroy(g);
-        g = nullptr;
-    }
+    for (int t = 0; t < 2; t++)
+        for (hipGraphExec_t& g : m->g_ba[pr][t]) {
+            if (g) hipGraphExecDestroy(g);
+            g = nullptr;
+        }
 }
 
 // PCG step part a: q = L p, alpha, phi += alpha p, d -= alpha q, |d|^2.
@@ -501,25 +526,27 @@ hipError_t emit_a(gpaw_poisson* ps, MixedState* m, int pr, hipStream_t st, bool 
 
 // z, beta (flexible), p = z + beta p after the inner V-cycle; first: p = z
 template <class T>
-void emit_dz_p(gpaw_poisson* ps, MixedState* m, int pr, int first, hipStream_t st)
+void emit_dz_p(gpaw_poisson* ps, MixedState* m, int pr, int first, hipStream_t st,
+               double* hist = nullptr)
 {
     const i64 n = ps->npts[0];
     launch_kernel(k_dot_dz_zq<T>, gridn(n), dim3(256), 0, st, ps->res[0],
                   (const T*)z_of(m, pr), m->q, n, m->partial);
     launch_kernel(k_p_update<T>, gridn(n), dim3(256), 0, st, m->p, (const T*)z_of(m, pr),
-                  (const double*)m->S, n, first, (const double*)m->partial + gridn(n).x,
-                  (int)gridn(n).x);
+                  m->S, n, first, (const double*)m->partial + gridn(n).x,
+                  (int)gridn(n).x, (const double*)m->partial + PART_DD, (int)gridn(n).x, hist);
 }
 
-// PCG step part b: z = V(d), beta (flexible), p = z + beta p
-hipError_t emit_b(gpaw_poisson* ps, MixedState* m, int pr, hipStream_t st)
+// PCG step part b: z = V(d), beta (flexible), p = z + beta p; `hist`: where
+// the previous step's |d|^2 goes (nullptr: not summed)
+hipError_t emit_b(gpaw_poisson* ps, MixedState* m, int pr, hipStream_t st, double* hist = nullptr)
 {
     hipError_t e = emit_top(ps, m, pr, true, st);
     if (e != hipSuccess) return e;
     if (pr == 0)
-        emit_dz_p<float>(ps, m, pr, 0, st);
+        emit_dz_p<float>(ps, m, pr, 0, st, hist);
     else
-        emit_dz_p<double>(ps, m, pr, 0, st);
+        emit_dz_p<double>(ps, m, pr, 0, st, hist);
     return launch_error();
 }
 
@@ -534,27 +561,60 @@ int capture_all(gpaw_poisson* ps, int pr)
                            [&](hipStream_t st) { return emit_top(ps, m, pr, false, st); });
     if (!rc) rc = capture_graph(&m->g_a[pr], [&](hipStream_t st) { return emit_a(ps, m, pr, st); });
     if (!rc) rc = capture_graph(&m->g_b[pr], [&](hipStream_t st) { return emit_b(ps, m, pr, st); });
-    if (!rc) m->g_ba[pr].assign(MIXED_CHUNK_MAX, nullptr);
+    if (!rc)
+        for (int t = 0; t < 2; t++) m->g_ba[pr][t].assign(MIXED_CHUNK_MAX, nullptr);
     if (rc) destroy_graphs(m, pr);   // retried from scratch on the next call
     return rc;
 }
 
-// k PCG steps (b, a) in one graph
-int run_ba(gpaw_poisson* ps, MixedState* m, int pr, int k, hipStream_t st)
+// |L phi - rho|^2 (FP64) into S[slot]; phi, rho and the iteration's
+// residual d are left as they are
+hipError_t emit_true_norm(gpaw_poisson* ps, MixedState* m, int slot, hipStream_t st)
 {
-    hipGraphExec_t& g = m->g_ba[pr][k - 1];
+    gpaw_operator* op = ps->ops[0];
+    Phases ph;
+    gpaw_phases(nullptr, ph);
+    int np = 0;
+    hipError_t e = launch_stencil(op->st, DT_F64, ST_RESID, ps->phiA[0], nullptr, ps->rho[0],
+                                  op->g, ph, 1, 0.0, op->partial, &np, false, st);
+    if (e == hipSuccess) e = launch_reduce_partials(op->partial, np, 1.0, m->S + slot, st);
+    return e;
+}
+
+// k PCG steps (b, a) in one graph; `fin`: then the true residual's norm
+// into S[DT] (a chunk expected to reach the tolerance)
+int run_ba(gpaw_poisson* ps, MixedState* m, int pr, int k, bool fin, hipStream_t st)
+{
+    hipGraphExec_t& g = m->g_ba[pr][fin ? 1 : 0][k - 1];
     if (!g) {
         int rc = capture_graph(&g, [&](hipStream_t s) -> hipError_t {
             hipError_t e = hipSuccess;
             for (int i = 0; i < k && e == hipSuccess; i++) {
-                e = emit_b(ps, m, pr, s);
+                e = emit_b(ps, m, pr, s, i > 0 ? m->S + S_HIST + i - 1 : nullptr);
                 if (e == hipSuccess) e = emit_a(ps, m, pr, s, i == k - 1);
             }
+            if (e == hipSuccess && fin) e = emit_true_norm(ps, m, S_DT, s);
             return e;
         });
         if (rc) return rc;
     }
     return gpaw_check(hipGraphLaunch(g, st), "mixed: iterations");
+}
+
+// GPAW_AMD_MIXED_LOG=1: the chunk schedule of every solve on stderr
+bool mixed_log()
+{
+    static const bool on = [] {
+        const char* e = getenv("GPAW_AMD_MIXED_LOG");
+        return e && atoi(e) > 0;
+    }();
+    return on;
+}
+
+// S[DZ] = 0: the next direction update restarts the recursion (p = z)
+int mark_restart(MixedState* m, hipStream_t st)
+{
+    return gpaw_check(hipMemsetAsync(m->S + S_DZ, 0, sizeof(double), st), "mixed: restart");
 }
 
 int read_scalar(MixedState* m, int slot, hipStream_t st, double* v)
@@ -566,9 +626,9 @@ int read_scalar(MixedState* m, int slot, hipStream_t st, double* v)
     return GPAW_OK;
 }
 
-// d = L phi - rho (FP64, stored in res[0]) with |d|^2 into S[DD]; the FP32
-// inner V-cycle also gets d32 = d
-int residual64(gpaw_poisson* ps, MixedState* m, int pr, hipStream_t st)
+// d = L phi - rho (FP64, stored in res[0]) with |d|^2 into S[slot]; the
+// FP32 inner V-cycle also gets d32 = d
+int residual64(gpaw_poisson* ps, MixedState* m, int pr, hipStream_t st, int slot = S_DD)
 {
     gpaw_operator* op = ps->ops[0];
     Phases ph;
@@ -576,7 +636,7 @@ int residual64(gpaw_poisson* ps, MixedState* m, int pr, hipStream_t st)
     int np = 0;
     hipError_t e = launch_stencil(op->st, DT_F64, ST_RESID, ps->phiA[0], ps->res[0], ps->rho[0],
                                   op->g, ph, 1, 0.0, op->partial, &np, true, st);
-    if (e == hipSuccess) e = launch_reduce_partials(op->partial, np, 1.0, m->S + S_DD, st);
+    if (e == hipSuccess) e = launch_reduce_partials(op->partial, np, 1.0, m->S + slot, st);
     if (e == hipSuccess && pr == 0)
         e = launch_cast(m->v[0].rho[0], DT_F32, ps->res[0], DT_F64, ps->npts[0], st);
     return gpaw_check(e, "mixed: residual");
@@ -634,12 +694,13 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
                                         double rel_tol, int maxiter, int mode, int inner,
                                         int* niter, double* relres, void* stream)
 {
-    if (!ps || !phi || !rho || maxiter < 1 || rel_tol <= 0.0 || mode < 0 || mode > 3 ||
+    if (!ps || !phi || !rho || maxiter < 1 || rel_tol <= 0.0 || mode < 0 || mode > 7 ||
         inner < 1)
         return gpaw_fail(GPAW_EINVAL, "mixed: bad argument");
     if (ps->method != 2) return gpaw_fail(GPAW_EUNSUPPORTED, "mixed: Jacobi smoothing only");
     const int pcg = mode & 1;   // else iterative refinement
-    const int pr = mode >> 1;   // inner V-cycle precision: 0 FP32, 1 FP64
+    const int pr = (mode >> 1) & 1;   // inner V-cycle precision: 0 FP32, 1 FP64
+    const bool history = (mode & 4) != 0;   // first chunk from the last solve
     hipStream_t st = (hipStream_t)stream;
     int rc = setup(ps, pr);
     if (rc) return rc;
@@ -664,57 +725,108 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
         e = launch_error();
     }
     if (e != hipSuccess) return gpaw_check(e, "mixed: load");
-    double rr = 0.0, dd = 0.0;
-    // |rho|^2 and the first |d|^2 come back in one readback
-    rc = residual64(ps, m, pr, st);
-    if (!rc) rc = read_scalar(m, S_DD, st, &dd);
-    if (rc) return rc;
-    rr = m->h_S[S_RR];
-    const double target = rel_tol * rel_tol * rr;
-    // loops continue while !(dd <= target): a NaN norm ends them and is
-    // reported below instead of passing for convergence
+    double rr = 0.0, dd = 0.0, target = 0.0;
     int it = 0;
-    while (pcg && !rc && !(dd <= target) && std::isfinite(dd) && it < maxiter) {
-        // (re)start from the true residual d: z = V(d); dz = d.z; p = z
-        rc = gpaw_check(hipGraphLaunch(m->g_vcycle[pr], st), "mixed: V-cycle");
-        if (!rc) {
-            if (pr == 0)
-                emit_dz_p<float>(ps, m, pr, 1, st);
-            else
-                emit_dz_p<double>(ps, m, pr, 1, st);
-            rc = gpaw_check(launch_error(), "mixed: start");
-        }
-        // PCG steps run in chunks of k (b, a) between readbacks of |d|^2; a
-        // chunk may run a few steps past the tolerance, which only makes the
-        // result more accurate.  k follows the observed convergence rate.
-        if (!rc) rc = gpaw_check(hipGraphLaunch(m->g_a[pr], st), "mixed: iteration");
-        if (!rc) rc = read_scalar(m, S_DD, st, &dd);
-        it++;
-        double prev = dd;
-        int k = 2;
-        while (!rc && !(dd <= target) && std::isfinite(dd) && it < maxiter) {
-            if (k > maxiter - it) k = maxiter - it;
-            rc = run_ba(ps, m, pr, k, st);
+    if (pcg) {
+        // d = L phi - rho with |d|^2 into S[D0] and a restart marker, then
+        // the first chunk with no readback in between (on a zero residual
+        // its steps change nothing).  With `history` the first chunk is as
+        // long as the previous solve on this plan took and ends with the
+        // true residual's norm, so a repeated solve of a similar density
+        // reads back once.  Later chunks follow the observed convergence
+        // rate; a chunk may run a few steps past the tolerance, which only
+        // makes the result more accurate.
+        rc = residual64(ps, m, pr, st, S_D0);
+        if (!rc) rc = mark_restart(m, st);
+        if (rc) return rc;
+        const int h = history ? m->hint[pr] : 0;
+        int k = h > 0 ? std::min(h, MIXED_CHUNK_MAX) : 3;
+        bool fin = h > 0 && h <= MIXED_CHUNK_MAX;
+        // |d|^2 after every step since the last (re)start (index 0: the
+        // start) and the step count of that start
+        std::vector<double> hist;
+        int base = 0, cross = 0;   // cross: steps after which |d| first met the target
+        bool first = true;
+        for (;;) {
+            k = std::min(k, maxiter - it);
+            if (k < 1) break;
+            rc = run_ba(ps, m, pr, k, fin, st);
             if (!rc) rc = read_scalar(m, S_DD, st, &dd);
+            if (rc) break;
+            if (first) {   // |rho|^2 and the starting |d|^2 came back too
+                rr = m->h_S[S_RR];
+                target = rel_tol * rel_tol * rr;
+                hist.push_back(m->h_S[S_D0]);
+                first = false;
+            }
+            for (int j = 0; j < k; j++) {
+                const double v = j < k - 1 ? m->h_S[S_HIST + j] : dd;
+                hist.push_back(v);
+                if (!cross && v <= target) cross = base + (int)hist.size() - 1;
+            }
             it += k;
-            // steps still needed at the rate of this chunk (|d|^2 per step)
-            const int done = k;
-            k = MIXED_CHUNK_MAX;
-            if (!rc && dd > target && dd < prev && prev > 0) {
-                const double rate = std::pow(dd / prev, 1.0 / done);
+            if (mixed_log())
+                fprintf(stderr, "mixed: chunk %d%s -> %d steps, |d|^2/target %.3g%s\n", k,
+                        fin ? " +true" : "", it, dd / target,
+                        fin ? "" : " (no true residual)");
+            if (fin && mixed_log()) fprintf(stderr, "mixed:   true/target %.3g\n", m->h_S[S_DT] / target);
+            if (!std::isfinite(dd)) break;
+            // the true residual L phi - rho decides; the recursion's |d| can
+            // drift from it
+            bool have_true = fin;
+            double dt = fin ? m->h_S[S_DT] : 0.0;
+            if (!fin && dd <= target) {
+                rc = gpaw_check(emit_true_norm(ps, m, S_DT, st), "mixed: residual");
+                if (!rc) rc = read_scalar(m, S_DT, st, &dt);
+                if (rc) break;
+                have_true = true;
+            }
+            if (have_true && (!std::isfinite(dt) || dt <= target)) {
+                dd = dt;
+                break;
+            }
+            if (it >= maxiter) break;
+            if (have_true && dd <= target) {
+                // drifted: restart from the true residual (p = z)
+                rc = residual64(ps, m, pr, st);
+                if (!rc) rc = mark_restart(m, st);
+                if (rc) break;
+                dd = dt;
+                hist.assign(1, dt);
+                base = it;
+                cross = 0;
+                k = 3;
+                fin = false;
+                continue;
+            }
+            // the steps still needed at the rate of the last (up to) two
+            // steps, at most 8 per chunk: the rate of the first steps
+            // underestimates the later ones (16 steps predicted where 11
+            // sufficed on 159^3), and a step run past the tolerance costs
+            // more than a readback
+            k = MIXED_PREDICT_MAX;
+            fin = false;
+            const int ns = (int)hist.size() - 1, w = std::min(2, ns);
+            const double old = hist[ns - w];
+            if (w > 0 && dd > target && dd < old && old > 0) {
+                const double rate = std::pow(dd / old, 1.0 / w);
                 if (rate > 0 && rate < 1) {
                     const double need = std::ceil(std::log(target / dd) / std::log(rate));
-                    if (need >= 1 && need < MIXED_CHUNK_MAX) k = (int)need;
+                    if (need >= 1 && need <= MIXED_PREDICT_MAX) {
+                        k = (int)need;
+                        fin = true;
+                    }
                 }
             }
-            prev = dd;
         }
-        // the recurrence's |d| can drift from the true residual: recompute it
-        // (L phi - rho in FP64); if it misses the target, restart from it
-        if (!rc && std::isfinite(dd)) {
-            rc = residual64(ps, m, pr, st);
-            if (!rc) rc = read_scalar(m, S_DD, st, &dd);
-        }
+        if (!rc && dd <= target) m->hint[pr] = cross > 0 ? cross : it;
+    } else {
+        // |rho|^2 and the first |d|^2 come back in one readback
+        rc = residual64(ps, m, pr, st);
+        if (!rc) rc = read_scalar(m, S_DD, st, &dd);
+        if (rc) return rc;
+        rr = m->h_S[S_RR];
+        target = rel_tol * rel_tol * rr;
     }
     while (!pcg && !rc && !(dd <= target) && std::isfinite(dd) && it < maxiter) {
         rc = gpaw_check(hipGraphLaunch(m->g_vcycle[pr], st), "mixed: V-cycle");

@@ -17,6 +17,7 @@
 // cycle the phi_0 buffers swap roles every cycle, so two graphs (one per
 // starting buffer) are captured and alternated.  The host reads the 8-byte
 // error after each cycle to apply the reference's stopping rule exactly.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -587,14 +588,17 @@ extern "C" int gpaw_poisson_solve_neutral(gpaw_poisson* ps, double* phi, const d
     // that meets the stopping rule and takes phi as it was after that
     // cycle.  niter and phi are exactly the reference's.  k follows the
     // observed convergence rate so that a chunk usually ends at the
-    // stopping cycle.
+    // stopping cycle; the first chunk is as long as the previous solve on
+    // this plan took (repeated solves of similar densities, as in an SCF
+    // loop, then read the norms back once).  Chunk lengths never change
+    // niter or phi, only how often the host reads the norms.
     const bool jac = chunk_rotates(ps);
     const double* result = nullptr;
     int state = 0;
     int n = 0;         // V-cycles accepted so far
     double err = 0.0;
     double prev = -1.0, rate = -1.0;
-    int k = 2;
+    int k = ps->hint > 0 ? std::min(ps->hint, GPAW_CHUNK_MAX) : 2;
     for (;;) {
         if (k > maxiter - n) k = maxiter - n;
         rc = run_chunk(ps, k, state, st);
@@ -643,6 +647,7 @@ extern "C" int gpaw_poisson_solve_neutral(gpaw_poisson* ps, double* phi, const d
     if (n == maxiter)
         return gpaw_fail(GPAW_ENOCONV, "Poisson solver did not converge in %d iterations!",
                          maxiter);
+    ps->hint = n;
     return GPAW_OK;
 }
 

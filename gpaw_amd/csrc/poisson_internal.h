@@ -3,7 +3,7 @@
 #pragma once
 #include <vector>
 
-#define GPAW_CHUNK_MAX 16   // V-cycles per chunk
+#define GPAW_CHUNK_MAX 32   // V-cycles per chunk
 
 #include "capi_internal.h"
 #include "coarse.h"
@@ -47,6 +47,9 @@ struct gpaw_poisson {
     std::vector<hipGraphExec_t> chunk;   // index 2 * (k - 1) + state
     int use_graph = 1;
     int tuned = 0;             // launch shapes tuned (when the tuner is on)
+    // V-cycles the last converged solve took: the first chunk of the next
+    // solve runs that many (0: none yet, start with 2)
+    int hint = 0;
     int swaps = 0;             // 1 if level 0 does an odd number of sweeps per cycle
     int lc = -1;               // first level run by the fused coarse kernel (-1: none)
     gpaw::CoarsePlan* d_coarse = nullptr;

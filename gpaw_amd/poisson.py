@@ -242,8 +242,8 @@ class PoissonSolver:
 
     def estimate_memory(self, mem):
         gdbytes = self.gd.bytecount()
-        mem.subnode('V-cycle chunk buffers (phi after each of up to 16 cycles '
-                    'and a sweep buffer)', 17 * gdbytes)
+        mem.subnode('V-cycle chunk buffers (phi after each of up to 32 cycles '
+                    'and a sweep buffer)', 33 * gdbytes)
         nbytes = 0
         for level in range(self.levels + 1):
             nbytes += 4 * gdbytes        # phi pair, rho, residual
@@ -270,11 +270,16 @@ class MixedPoissonSolver(PoissonSolver):
     outer iteration does.  Converged when ||L phi - rho||_2 <= rel_tol *
     ||rho||_2 for the neutral density (checked on the true residual); `eps`
     is accepted for signature compatibility but the relative criterion is
-    the one applied.
+    the one applied.  reuse_history (CG): the first chunk of iterations is
+    as long as the previous converged solve took, so repeated solves of
+    similar densities (an SCF loop) read a norm back once; the step count,
+    and with it the last bits of phi beyond the tolerance, then depend on
+    that history.  False makes every result depend on its inputs only.
     """
 
     def __init__(self, nn=3, relax='J', eps=2e-10, rel_tol=1e-10, outer='pcg',
-                 inner=1, maxiter=200, inner_precision='fp32'):
+                 inner=1, maxiter=200, inner_precision='fp32',
+                 reuse_history=True):
         PoissonSolver.__init__(self, nn, relax, eps)
         if outer not in ('pcg', 'refine'):
             raise ValueError('outer must be "pcg" or "refine"')
@@ -285,6 +290,7 @@ class MixedPoissonSolver(PoissonSolver):
         self.inner = inner
         self.inner_precision = inner_precision
         self.maxiter = maxiter
+        self.reuse_history = reuse_history
 
     def solve_neutral(self, phi, rho, eps=None):
         if self.relax_method != 2 or self.B is not None:
@@ -296,7 +302,8 @@ class MixedPoissonSolver(PoissonSolver):
         rc = _lib.lib().gpaw_poisson_solve_mixed(
             self._plan, P.ptr, R.ptr, float(self.rel_tol), int(self.maxiter),
             (1 if self.outer == 'pcg' else 0) |
-            (2 if self.inner_precision == 'fp64' else 0), int(self.inner),
+            (2 if self.inner_precision == 'fp64' else 0) |
+            (4 if self.reuse_history else 0), int(self.inner),
             ctypes.byref(niter), ctypes.byref(rel), _arrays.stream())
         self.relres = rel.value
         if rc == _lib.GPAW_ENOCONV:

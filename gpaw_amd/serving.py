@@ -216,7 +216,9 @@ class Server:
         if kind == 'poisson':
             from .poisson import MixedPoissonSolver, PoissonSolver
             if self.mixed:
-                ps = MixedPoissonSolver(nn=3, rel_tol=self.rel_tol)
+                # results must not depend on which requests a plan served
+                # before (each equals the same request served alone)
+                ps = MixedPoissonSolver(nn=3, rel_tol=self.rel_tol, reuse_history=False)
             else:
                 ps = PoissonSolver(nn=3)
             ps.set_grid_descriptor(gd)

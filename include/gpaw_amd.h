@@ -157,7 +157,12 @@ int gpaw_poisson_precondition(gpaw_poisson* ps, const double* r, double* z,
  * levels in FP64).  mode bit 0: 0 = iterative refinement with `inner`
  * V-cycles per step, 1 = flexible preconditioned CG with one V-cycle per
  * iteration; mode bit 1: 0 = inner V-cycle in FP32, 1 = the same inner
- * V-cycle in FP64 (for attributing the speed-up).  Stops when
+ * V-cycle in FP64 (for attributing the speed-up); mode bit 2 (CG only):
+ * the first chunk of iterations is as long as the previous converged solve
+ * on this plan took, ending with the true residual, so that a repeated
+ * solve of a similar density reads a norm back once (the iteration count,
+ * and so the last bits of phi, then depend on that history; without the
+ * bit they depend only on the inputs).  Stops when
  * ||L phi - rho||_2 <= rel_tol * ||rho||_2 (rho as given, already
  * neutral), checked on the true residual; *relres receives the final
  * ratio.  A non-finite residual norm (breakdown) returns GPAW_EDEVICE and

@@ -208,3 +208,36 @@ def test_vcycle_fusions_identical(N, pbc, monkeypatch):
     for z in out[1:]:
         assert t.equal(out[0][0], z[0])
         assert t.equal(out[0][1], z[1])
+
+
+@pytest.mark.parametrize('N,pbc', [(64, False), (40, True)])
+def test_solve_history(N, pbc):
+    """reuse_history=True: the first chunk of a solve is as long as the last
+    converged solve's CG steps up to the first step whose |d| met the
+    tolerance, so a repeated solve takes no more steps than the first and
+    still meets 1e-10 (checked with the oracle's FP64 operator).
+    reuse_history=False: repeated solves take the same steps and give the
+    same bits (results depend on the inputs only)."""
+    print('case solve history', N, pbc, flush=True)
+    t = _torch()
+    from gpaw_amd.poisson import MixedPoissonSolver
+    gd, rho = _setup(N, 0.18 / BOHR, pbc, 6, N + 11)
+    rd = t.from_numpy(rho).to('cuda')
+    for reuse in (True, False):
+        ms = MixedPoissonSolver(nn=3, rel_tol=1e-10, reuse_history=reuse)
+        ms.set_grid_descriptor(gd)
+        ms.initialize()
+        runs = []
+        for _ in range(3):
+            phi = t.zeros(tuple(gd.n_c), dtype=t.float64, device='cuda')
+            n = ms.solve(phi, rd)
+            rr = _relres(gd, phi.cpu().numpy(), rho)
+            print('  reuse=%s steps %d relres %.2e' % (reuse, n, rr), flush=True)
+            assert rr <= 1.5e-10 and ms.relres <= 1e-10
+            runs.append((n, phi))
+        if reuse:
+            assert runs[1][0] <= runs[0][0] and runs[2][0] == runs[1][0]
+            assert t.equal(runs[1][1], runs[2][1])
+        else:
+            for n, phi in runs[1:]:
+                assert n == runs[0][0] and t.equal(phi, runs[0][1])
