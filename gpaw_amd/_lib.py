@@ -58,6 +58,7 @@ PROTOTYPES = {
     'gpaw_autotune': (_i, [_i]),
     'gpaw_autotune_report': (_l, [ctypes.c_char_p, _l]),
     'gpaw_stencil_kernel': (_i, [_i]),
+    'gpaw_tile_shape': (_i, [_i, _i, _i]),
     'gpaw_sum': (_i, [_vp, _l, _d, _vp, _vp]),
     'gpaw_dots': (_i, [_vp, _vp, _l, _l, _d, _vp, _vp]),
     'gpaw_dots_noalloc': (_i, [_vp, _vp, _l, _l, _d, _vp, _vp]),

@@ -552,6 +552,14 @@ extern "C" int gpaw_autotune(int enable) { return autotune_set(enable); }
 
 extern "C" long gpaw_autotune_report(char* buf, long len) { return autotune_report(buf, len); }
 
+extern "C" int gpaw_tile_shape(int threads, int tz, int planes)
+{
+    if (tile_shape_set(threads, tz, planes))
+        return gpaw_fail(GPAW_EINVAL, "tile shape: threads %d, columns %d, planes %d", threads, tz,
+                         planes);
+    return GPAW_OK;
+}
+
 extern "C" int gpaw_stencil_kernel(int kind)
 {
     // header numbering: 0 default, 1 tile, 2 box, 3 queue, 4 generic

@@ -93,6 +93,7 @@ int autotune_set(int on);
 // Kernel family of star-stencil launches without partial sums: -1 default,
 // 0 generic, 1 register queue, 2 tile, 3 box; returns the previous value.
 int stencil_kernel_set(int kind);
+int tile_shape_set(int nt, int tz, int q);   // gpaw_tile_shape
 long autotune_report(char* buf, long len);
 
 // Work estimate used to size partial-sum buffers.

@@ -362,6 +362,25 @@ static int forced_kind()
 // sizes the callers' partial buffers.
 static std::atomic<int> g_kind_override{-1};
 
+// Tile shape forced for launches without partial sums (gpaw_tile_shape):
+// threads, columns, planes; threads = 0: none.
+static std::atomic<int> g_tile_nt{0}, g_tile_tz{32}, g_tile_q{1};
+
+int tile_shape_set(int nt, int tz, int q)
+{
+    if (nt == 0) {
+        g_tile_nt.store(0);
+        return 0;
+    }
+    if ((nt != TILE_NT && nt != TILE_NT_MID && nt != TILE_NT_WIDE) || (tz != 32 && tz != 64) ||
+        q < 1)
+        return -1;
+    g_tile_tz.store(tz);
+    g_tile_q.store(q);
+    g_tile_nt.store(nt);
+    return 0;
+}
+
 int stencil_kernel_set(int kind)
 {
     const int prev = g_kind_override.load();
@@ -785,6 +804,17 @@ hipError_t launch_stencil(const Stencil& s, int dtype, int mode, const void* in,
             o.TZ = BOX_TZ;
         } else if (ov == K_TILE) {
             ok = s.star_r >= 1 && tile_shape(g, s.star_r, dtype_size(dtype), nin, o.TZ, o.sh, nt);
+            const int fnt = g_tile_nt.load();
+            // a forced shape where the kernel is compiled for it (wide blocks:
+            // complex data of range <= 3)
+            if (ok && fnt && (fnt == TILE_NT || (dtype_complex(dtype) && s.star_r <= 3))) {
+                TileShape fs;
+                if (tile_geometry(g, s.star_r, dtype_size(dtype), nin, g_tile_tz.load(),
+                                  g_tile_q.load(), fs, fnt)) {
+                    o.TZ = g_tile_tz.load();
+                    o.sh = fs;
+                }
+            }
         } else if (ov == K_QUEUE) {
             ok = s.star_r > 0;
         } else {

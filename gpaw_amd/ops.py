@@ -300,6 +300,12 @@ def autotune(enable=True):
 STENCIL_KERNELS = {'default': 0, 'tile': 1, 'box': 2, 'queue': 3, 'generic': 4}
 
 
+def tile_shape(threads=0, tz=32, planes=1):
+    """Force the tile kernel's launch shape (with stencil_kernel('tile')) for
+    launches without partial sums; threads=0 restores the default shape."""
+    _lib.check(_lib.lib().gpaw_tile_shape(int(threads), int(tz), int(planes)))
+
+
 def stencil_kernel(kind='default'):
     """Kernel family for star-stencil launches without partial sums
     ('default', 'tile', 'box', 'queue' or 'generic'; every family gives the

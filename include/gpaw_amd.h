@@ -261,6 +261,16 @@ long gpaw_autotune_report(char* buf, long len);
  * bit-identical results. */
 int gpaw_stencil_kernel(int kind);
 
+/* Launch shape of the tile kernel, with gpaw_stencil_kernel(1), for launches
+ * without partial sums (tests and timing experiments; no reference
+ * counterpart): `threads` per block (256, 512 or 1024; 512 and 1024 for
+ * complex data of range <= 3 only), `tz` columns (32 or 64) and `planes`
+ * per sweep (>= 1; at least n0 sweeps the whole i0 axis in one run).  A
+ * shape that cannot run a launch leaves the default; threads = 0 clears the
+ * setting.  Returns GPAW_OK or GPAW_EINVAL.  Every shape gives bit-identical
+ * results. */
+int gpaw_tile_shape(int threads, int tz, int planes);
+
 /* -------------------------------------------------------------- helpers */
 /* Deterministic FP64 sum of n doubles times scale into *result (device):
  * gd.integrate / comm.sum of the reference (gpaw/grid_descriptor.py:197-208). */

@@ -133,3 +133,44 @@ def test_pseudo_hamiltonian_every_family(kind):
     with family(kind):
         apply_pseudo_hamiltonian(Laplace(gd, -0.5, 3, complex), dev(vt), dev(psi), got, ph)
     assert np.array_equal(host(got), want)
+
+
+TILE_SHAPES = [(256, 32, 1), (256, 32, 5), (256, 64, 10 ** 6), (512, 32, 10 ** 6),
+               (512, 64, 10 ** 6), (512, 64, 8), (1024, 32, 10 ** 6), (1024, 64, 4)]
+
+
+@pytest.mark.parametrize('threads,tz,planes', TILE_SHAPES)
+@pytest.mark.parametrize('N,pbc,nn,dtype,nin', [
+    ((64, 24, 64), True, 3, complex, 3),     # sweeps over whole periodic i0 axes
+    ((7, 20, 36), True, 3, complex, 2),      # n0 = 2R + 1
+    ((6, 18, 40), True, 3, complex, 2),      # n0 = 2R
+    ((9, 33, 30), True, 2, complex, 2),
+    ((11, 17, 66), (True, False, True), 1, complex, 2),
+    ((20, 30, 40), False, 3, complex, 2),
+    ((40, 36, 44), True, 3, float, 2)])
+def test_tile_shapes_against_oracle(threads, tz, planes, N, pbc, nn, dtype, nin):
+    """The tile kernel forced to each launch shape (block size, columns,
+    planes per sweep, up to the whole i0 axis) against the oracle, bit for
+    bit."""
+    print('case tile shape', threads, tz, planes, N, pbc, nn, dtype.__name__, nin, flush=True)
+    from gpaw_amd import ops
+    rng = np.random.default_rng(sum(N) * 7 + nn)
+    cell = np.array(N) * 0.3
+    og = O.Grid(N, cell, pbc)
+    gd = _gd(N, cell, pbc)
+    c, o = O.laplace_stencil(og, -0.5, nn)
+    a = rng.uniform(-1, 1, (nin,) + tuple(og.n_c))
+    ph = None
+    if dtype == complex:
+        a = a + 1j * rng.uniform(-1, 1, a.shape)
+        ph = og.phase_cd(rng.uniform(-0.5, 0.5, 3))
+    ref = np.zeros_like(a)
+    OOp(c, o, og, dtype).apply(a, ref, ph)
+    b = dev(np.zeros_like(a))
+    ops.tile_shape(threads, tz, planes)
+    try:
+        with family('tile'):
+            _fdop(c, o, gd, dtype).apply(dev(a), b, ph)
+    finally:
+        ops.tile_shape(0)
+    assert np.array_equal(host(b), ref)
