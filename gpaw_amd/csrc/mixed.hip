@@ -20,7 +20,7 @@
 //                         recomputed, and the iteration restarts from it if
 //                         it does not.
 // The inner V-cycle uses the reference hierarchy (gpaw/poisson.py:64-113)
-// with symmetric level-0 smoothing (2 pre + 2 post by default) so that it is
+// with symmetric level-0 smoothing (3 pre + 3 post by default) so that it is
 // a symmetric preconditioner.  It runs in FP32, or -- to measure what the
 // precision itself contributes -- the very same V-cycle in FP64 inside the
 // same outer loop.  Stopping rule: ||d||_2 <= tol * ||rho||_2.
@@ -444,7 +444,7 @@ int setup(gpaw_poisson* ps, int pr)
         m->L = ps->L;
         m->pre = ps->pre;
         m->post = ps->post;
-        int s0 = 2, s1 = 2;
+        int s0 = 3, s1 = 3;
         if (const char* env = getenv("GPAW_AMD_MIXED_SMOOTH")) sscanf(env, "%d,%d", &s0, &s1);
         if ((s0 + s1) % 2) s1++;             // result must land in phiA[0]
         m->pre[0] = s0;

@@ -142,7 +142,7 @@ int gpaw_poisson_vcycles(gpaw_poisson* ps, double* phi, const double* rho,
  * run their own Krylov loop.  No reference counterpart (gpaw/poisson.py
  * iterates V-cycles on phi only).  The reference hierarchy and weights
  * (gpaw/poisson.py:64-113) with the mixed solver's symmetric level-0
- * smoothing (2 pre + 2 post sweeps); inner_precision 0: every level's
+ * smoothing (3 pre + 3 post sweeps); inner_precision 0: every level's
  * sweeps, residuals and transfers in FP32 (the smallest levels in FP64),
  * 1: all in FP64.  r, z: n0*n1*n2 doubles on the device, distinct; the
  * plan must use Jacobi smoothing.  Asynchronous on `stream`. */

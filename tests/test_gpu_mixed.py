@@ -152,7 +152,7 @@ def test_inner_precision_iteration_counts():
 def test_preconditioner_vcycle(N, pbc):
     """gpaw_poisson_precondition: the FP64 inner V-cycle from zero equals the
     oracle's V-cycle (gpaw/poisson.py:245-270) with the mixed solver's
-    2 + 2 level-0 sweeps, bit for bit; the FP32 one agrees with it to the
+    3 + 3 level-0 sweeps, bit for bit; the FP32 one agrees with it to the
     FP32 rounding level (1e-5 relative max-norm)."""
     print('case precondition', N, pbc, flush=True)
     t = _torch()
@@ -171,8 +171,8 @@ def test_preconditioner_vcycle(N, pbc):
     ops_ = O.PoissonSolver(nn=3)
     ops_.set_grid_descriptor(og)
     ops_.initialize()
-    ops_.presmooths[0] = 2
-    ops_.postsmooths[0] = 2
+    ops_.presmooths[0] = 3
+    ops_.postsmooths[0] = 3
     want = og.zeros()
     ops_.run_vcycles(want, r, 1)
     z64 = z64.cpu().numpy()
