@@ -319,6 +319,16 @@ class MixedPoissonSolver(PoissonSolver):
         P.finish()
         return niter.value
 
+    def smoothing_weights(self):
+        """Level-0 (pre, post) Jacobi weights of the inner V-cycle, one per
+        sweep in the order they run."""
+        pre = (ctypes.c_double * 16)()
+        post = (ctypes.c_double * 16)()
+        npre, npost = ctypes.c_int(0), ctypes.c_int(0)
+        _lib.check(_lib.lib().gpaw_poisson_mixed_weights(
+            self._plan, pre, ctypes.byref(npre), post, ctypes.byref(npost)))
+        return list(pre[:npre.value]), list(post[:npost.value])
+
     def precondition(self, r, z, precision=None):
         """z = one inner V-cycle applied to r from z = 0 (the multigrid
         preconditioner of the outer CG): float64 grid functions of the

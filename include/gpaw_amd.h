@@ -142,12 +142,23 @@ int gpaw_poisson_vcycles(gpaw_poisson* ps, double* phi, const double* rho,
  * run their own Krylov loop.  No reference counterpart (gpaw/poisson.py
  * iterates V-cycles on phi only).  The reference hierarchy and weights
  * (gpaw/poisson.py:64-113) with the mixed solver's symmetric level-0
- * smoothing (3 pre + 3 post sweeps); inner_precision 0: every level's
+ * smoothing (gpaw_poisson_mixed_weights); inner_precision 0: every level's
  * sweeps, residuals and transfers in FP32 (the smallest levels in FP64),
  * 1: all in FP64.  r, z: n0*n1*n2 doubles on the device, distinct; the
  * plan must use Jacobi smoothing.  Asynchronous on `stream`. */
 int gpaw_poisson_precondition(gpaw_poisson* ps, const double* r, double* z,
                               int inner_precision, void* stream);
+
+/* The level-0 smoothing of the mixed solver's inner V-cycle (no reference
+ * counterpart): *npre pre- and *npost post-smoothing Jacobi sweeps and the
+ * weight of each, in the order they run (pre / post: room for 16 each, or
+ * null to query the counts).  By default 2 + 2 sweeps whose weights make
+ * the product of the sweeps' factors prod (1 - w_i D^-1 A) the degree-2
+ * Chebyshev polynomial smallest on [lmax/7, lmax] (lmax the largest
+ * eigenvalue of D^-1 A), the post-smoothing weights in reverse order; the
+ * levels below keep the reference's smoothing. */
+int gpaw_poisson_mixed_weights(const gpaw_poisson* ps, double* pre, int* npre,
+                               double* post, int* npost);
 
 /* Mixed-precision solve of the same problem (no reference counterpart: the
  * FP32 / FP64 split of the configuration "mixed-precision multigrid

@@ -148,12 +148,29 @@ def test_inner_precision_iteration_counts():
     assert abs(its['fp32'] - its['fp64']) <= 2
 
 
+def _weighted_level0(ops_, wpre, wpost):
+    """The oracle's V-cycle with the mixed solver's level-0 sweeps: each
+    level-0 relax call (pre, then post, per cycle) runs one Jacobi sweep per
+    weight of its list."""
+    orig = ops_.operators[0].relax
+    calls = [0]
+
+    def relax0(method, f, s, n, w=1.0):
+        ws = wpre if calls[0] % 2 == 0 else wpost
+        calls[0] += 1
+        assert n == len(ws)
+        for wi in ws:
+            orig(method, f, s, 1, wi)
+    ops_.operators[0].relax = relax0
+
+
 @pytest.mark.parametrize('N,pbc', [(64, False), (40, True)])
 def test_preconditioner_vcycle(N, pbc):
     """gpaw_poisson_precondition: the FP64 inner V-cycle from zero equals the
     oracle's V-cycle (gpaw/poisson.py:245-270) with the mixed solver's
-    3 + 3 level-0 sweeps, bit for bit; the FP32 one agrees with it to the
-    FP32 rounding level (1e-5 relative max-norm)."""
+    level-0 sweeps and weights (gpaw_poisson_mixed_weights), bit for bit; the
+    FP32 one agrees with it to the FP32 rounding level (1e-5 relative
+    max-norm)."""
     print('case precondition', N, pbc, flush=True)
     t = _torch()
     from gpaw_amd.poisson import MixedPoissonSolver
@@ -162,6 +179,7 @@ def test_preconditioner_vcycle(N, pbc):
     ms = MixedPoissonSolver(nn=3)
     ms.set_grid_descriptor(gd)
     ms.initialize()
+    wpre, wpost = ms.smoothing_weights()
     z64 = t.full(tuple(gd.n_c), np.nan, dtype=t.float64, device='cuda')
     z32 = t.full(tuple(gd.n_c), np.nan, dtype=t.float64, device='cuda')
     rd = t.from_numpy(r).to('cuda')
@@ -171,8 +189,9 @@ def test_preconditioner_vcycle(N, pbc):
     ops_ = O.PoissonSolver(nn=3)
     ops_.set_grid_descriptor(og)
     ops_.initialize()
-    ops_.presmooths[0] = 3
-    ops_.postsmooths[0] = 3
+    ops_.presmooths[0] = len(wpre)
+    ops_.postsmooths[0] = len(wpost)
+    _weighted_level0(ops_, wpre, wpost)
     want = og.zeros()
     ops_.run_vcycles(want, r, 1)
     z64 = z64.cpu().numpy()
@@ -180,6 +199,67 @@ def test_preconditioner_vcycle(N, pbc):
     err = np.abs(z32.cpu().numpy() - want).max() / np.abs(want).max()
     print('  FP32 V-cycle vs FP64: %.2e' % err, flush=True)
     assert err < 1e-5
+
+
+def _lmax_dense(h_c, nn):
+    """Largest eigenvalue of D^-1 A for the nn Laplacian of spacings h_c on
+    an 8^3 periodic grid (the symbol's maximum, reached at theta = pi on
+    every axis of an even grid), from the dense matrix the oracle's apply
+    builds on unit vectors."""
+    n = 8
+    og = O.Grid((n,) * 3, np.array(h_c) * n, True)
+    lap = O.gpaw_oracle.laplace(og, 1.0, nn)
+    A = np.empty((n ** 3, n ** 3))
+    e = np.zeros((n,) * 3)
+    out = np.zeros((n,) * 3)
+    for i in range(n ** 3):
+        e.flat[i] = 1.0
+        lap.apply(e, out)
+        A[:, i] = out.ravel()
+        e.flat[i] = 0.0
+    d = A[0, 0]
+    return np.linalg.eigvalsh(A / d).max()
+
+
+@pytest.mark.parametrize('h_c', [(0.2, 0.2, 0.2), (0.2, 0.24, 0.22)])
+def test_chebyshev_smoothing_weights(h_c):
+    """The mixed V-cycle's level-0 weights: 2 + 2 sweeps, post in reverse
+    order, and prod_i (1 - w_i x) equals the degree-2 Chebyshev polynomial
+    T_2((b + a - 2x) / (b - a)) / T_2((b + a) / (b - a)) on [0, b] with
+    b = the largest eigenvalue of D^-1 A (dense eigensolve) and a = b / 7.
+    Roots placed at cos(i pi / k) instead, or b taken as the 7-point
+    stencil's 2, each fail the comparison."""
+    _torch()
+    from numpy.polynomial import chebyshev as C
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.poisson import MixedPoissonSolver
+    N = 48
+    gd = GridDescriptor((N,) * 3, np.array(h_c) * N / BOHR, False)
+    ms = MixedPoissonSolver(nn=3)
+    ms.set_grid_descriptor(gd)
+    ms.initialize()
+    wpre, wpost = ms.smoothing_weights()
+    print('weights', wpre, wpost, flush=True)
+    assert len(wpre) == 2 and wpost == wpre[::-1]
+    b = _lmax_dense(np.array(h_c) / BOHR, 3)
+    a = b / 7.0
+    x = np.linspace(0.0, b, 101)
+
+    def cheb(x, a, b):
+        return C.chebval((b + a - 2 * x) / (b - a), [0, 0, 1]) / \
+            C.chebval((b + a) / (b - a), [0, 0, 1])
+
+    def prod(ws):
+        return np.prod([1 - w * x for w in ws], axis=0)
+    assert np.abs(prod(wpre) - cheb(x, a, b)).max() < 1e-12
+    # planted mistakes
+    mid, half = 0.5 * (b + a), 0.5 * (b - a)
+    wrong_roots = [1 / (mid + half * np.cos(i * np.pi / 2)) for i in (1, 2)]
+    assert np.abs(prod(wrong_roots) - cheb(x, a, b)).max() > 1e-3
+    b7 = 2.0
+    wrong_lmax = [1 / (0.5 * (b7 + b7 / 7) + 0.5 * (b7 - b7 / 7) * np.cos((2 * i + 1) * np.pi / 4))
+                  for i in range(2)]
+    assert np.abs(prod(wrong_lmax) - cheb(x, a, b)).max() > 1e-3
 
 
 @pytest.mark.parametrize('N,pbc', [(64, False), (48, True), (96, False)])
