@@ -73,9 +73,11 @@ struct MixedState {
     // k PCG steps (b then a) per launch, k = 1 .. MIXED_CHUNK_MAX; [1]: the
     // same followed by the true residual's norm (chunks expected to converge)
     std::vector<hipGraphExec_t> g_ba[2][2];
-    // PCG steps the last converged solve took, per inner precision: the
-    // first chunk of the next solve
+    // PCG steps the last converged solve took, per inner precision, and the
+    // tolerance it was solved to: the first chunk of the next solve to the
+    // same tolerance
     int hint[2] = {0, 0};
+    double hint_tol[2] = {0.0, 0.0};
     int tuned[2] = {0, 0};              // launch shapes tuned (tuner on)
 };
 
@@ -812,7 +814,7 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
         rc = residual64(ps, m, pr, st, S_D0);
         if (!rc) rc = mark_restart(m, st);
         if (rc) return rc;
-        const int h = history ? m->hint[pr] : 0;
+        const int h = (history && m->hint_tol[pr] == rel_tol) ? m->hint[pr] : 0;
         int k = h > 0 ? std::min(h, MIXED_CHUNK_MAX) : 3;
         bool fin = h > 0 && h <= MIXED_CHUNK_MAX;
         // |d|^2 after every step since the last (re)start (index 0: the
@@ -892,7 +894,10 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
                 }
             }
         }
-        if (!rc && dd <= target) m->hint[pr] = cross > 0 ? cross : it;
+        if (!rc && dd <= target) {
+            m->hint[pr] = cross > 0 ? cross : it;
+            m->hint_tol[pr] = rel_tol;
+        }
     } else {
         // |rho|^2 and the first |d|^2 come back in one readback
         rc = residual64(ps, m, pr, st);

@@ -598,7 +598,7 @@ extern "C" int gpaw_poisson_solve_neutral(gpaw_poisson* ps, double* phi, const d
     int n = 0;         // V-cycles accepted so far
     double err = 0.0;
     double prev = -1.0, rate = -1.0;
-    int k = ps->hint > 0 ? std::min(ps->hint, GPAW_CHUNK_MAX) : 2;
+    int k = (ps->hint > 0 && ps->hint_eps == eps) ? std::min(ps->hint, GPAW_CHUNK_MAX) : 2;
     for (;;) {
         if (k > maxiter - n) k = maxiter - n;
         rc = run_chunk(ps, k, state, st);
@@ -648,6 +648,7 @@ extern "C" int gpaw_poisson_solve_neutral(gpaw_poisson* ps, double* phi, const d
         return gpaw_fail(GPAW_ENOCONV, "Poisson solver did not converge in %d iterations!",
                          maxiter);
     ps->hint = n;
+    ps->hint_eps = eps;
     return GPAW_OK;
 }
 

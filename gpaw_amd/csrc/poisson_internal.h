@@ -47,9 +47,10 @@ struct gpaw_poisson {
     std::vector<hipGraphExec_t> chunk;   // index 2 * (k - 1) + state
     int use_graph = 1;
     int tuned = 0;             // launch shapes tuned (when the tuner is on)
-    // V-cycles the last converged solve took: the first chunk of the next
-    // solve runs that many (0: none yet, start with 2)
+    // V-cycles the last converged solve took and its eps: the first chunk of
+    // the next solve to the same eps runs that many (0: none yet, start with 2)
     int hint = 0;
+    double hint_eps = 0.0;
     int swaps = 0;             // 1 if level 0 does an odd number of sweeps per cycle
     int lc = -1;               // first level run by the fused coarse kernel (-1: none)
     gpaw::CoarsePlan* d_coarse = nullptr;

@@ -170,7 +170,7 @@ int gpaw_poisson_mixed_weights(const gpaw_poisson* ps, double* pre, int* npre,
  * iteration; mode bit 1: 0 = inner V-cycle in FP32, 1 = the same inner
  * V-cycle in FP64 (for attributing the speed-up); mode bit 2 (CG only):
  * the first chunk of iterations is as long as the previous converged solve
- * on this plan took, ending with the true residual, so that a repeated
+ * on this plan to the same rel_tol took, ending with the true residual, so that a repeated
  * solve of a similar density reads a norm back once (the iteration count,
  * and so the last bits of phi, then depend on that history; without the
  * bit they depend only on the inputs).  Stops when
