@@ -321,3 +321,34 @@ def test_solve_history(N, pbc):
         else:
             for n, phi in runs[1:]:
                 assert n == runs[0][0] and t.equal(phi, runs[0][1])
+
+
+def test_mixed_degenerate_inputs():
+    """Error and edge paths of the CG loop: a zero density converges at
+    once with phi left at zero (the guarded step length is 0, not 0/0); a
+    non-finite density reports a breakdown (GPAW_EDEVICE) and leaves phi
+    untouched; too few iterations raise PoissonConvergenceError."""
+    print('case mixed degenerate inputs', flush=True)
+    t = _torch()
+    from gpaw_amd._lib import GpawAmdError
+    from gpaw_amd.poisson import MixedPoissonSolver, PoissonConvergenceError
+    gd, rho = _setup(32, 0.2 / BOHR, False, 4, 3)
+    n = tuple(gd.n_c)
+    ms = MixedPoissonSolver(nn=3, rel_tol=1e-10)
+    ms.set_grid_descriptor(gd)
+    ms.initialize()
+    phi = t.zeros(n, dtype=t.float64, device='cuda')
+    ms.solve_neutral(phi, t.zeros(n, dtype=t.float64, device='cuda'))
+    assert t.count_nonzero(phi).item() == 0 and ms.relres == 0.0
+    bad = t.from_numpy(rho - rho.mean()).to('cuda')
+    bad[3, 4, 5] = float('nan')
+    phi = t.full(n, 7.0, dtype=t.float64, device='cuda')
+    with pytest.raises(GpawAmdError):
+        ms.solve_neutral(phi, bad)
+    assert bool((phi == 7.0).all())
+    few = MixedPoissonSolver(nn=3, rel_tol=1e-10, maxiter=2)
+    few.set_grid_descriptor(gd)
+    few.initialize()
+    with pytest.raises(PoissonConvergenceError):
+        few.solve_neutral(t.zeros(n, dtype=t.float64, device='cuda'),
+                          t.from_numpy(rho - rho.mean()).to('cuda'))
