@@ -299,6 +299,7 @@ int ensure_chunks(gpaw_poisson* ps)
         if (!rc) rc = alloc(&ps->d_chunk_partials, (long long)GPAW_CHUNK_MAX * ps->np0);
     }
     if (!rc) rc = alloc(&ps->d_errs, GPAW_CHUNK_MAX);
+    if (!rc) rc = alloc(&ps->d_sum, 1024);
     if (!rc)
         rc = gpaw_check(hipHostMalloc((void**)&ps->h_errs, sizeof(double) * GPAW_CHUNK_MAX),
                         "poisson: pinned errors");
@@ -313,6 +314,8 @@ int ensure_chunks(gpaw_poisson* ps)
         ps->d_chunk_partials = nullptr;
         if (ps->d_errs) hipFree(ps->d_errs);
         ps->d_errs = nullptr;
+        if (ps->d_sum) hipFree(ps->d_sum);
+        ps->d_sum = nullptr;
     }
     return rc;
 }
@@ -571,6 +574,9 @@ extern "C" int gpaw_poisson_vcycles(gpaw_poisson* ps, double* phi, const double*
     return rc;
 }
 
+static int solve_loaded(gpaw_poisson* ps, double* phi, double eps, int maxiter, int* niter,
+                        double* error, hipStream_t st);
+
 extern "C" int gpaw_poisson_solve_neutral(gpaw_poisson* ps, double* phi, const double* rho,
                                           double eps, int maxiter, int* niter, double* error,
                                           void* stream)
@@ -579,7 +585,48 @@ extern "C" int gpaw_poisson_solve_neutral(gpaw_poisson* ps, double* phi, const d
     hipStream_t st = (hipStream_t)stream;
     int rc = load_inputs(ps, phi, rho, st);
     if (rc) return rc;
-    rc = ensure_chunks(ps);
+    return solve_loaded(ps, phi, eps, maxiter, niter, error, st);
+}
+
+extern "C" int gpaw_poisson_solve(gpaw_poisson* ps, double* phi, const double* rho, double eps,
+                                  int maxiter, double maxcharge, double* charge, int* neutral,
+                                  int* niter, double* error, void* stream)
+{
+    if (!ps || !phi || !rho || !charge || !neutral || maxiter < 1)
+        return gpaw_fail(GPAW_EINVAL, "poisson: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    *neutral = 0;
+    int rc = ensure_chunks(ps);
+    if (rc) return rc;
+    // the charge integral dv * sum(rho) of gd.integrate, the same kernels
+    // and summation order as gpaw_sum
+    const long long n = ps->npts[0];
+    int np = 0;
+    hipError_t e = launch_sum(rho, n, ps->d_sum, &np, st);
+    if (e == hipSuccess) e = launch_reduce_partials(ps->d_sum, np, ps->dv, ps->d_err, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ps->h_err, ps->d_err, sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return gpaw_check(e, "poisson: charge");
+    const double q = ps->h_err[0];
+    *charge = q;
+    if (!(std::fabs(q) <= maxcharge)) return GPAW_OK;   // charged: the caller's branches
+    // neutral: rho minus its mean, as gpaw/poisson.py:130-138 computes it,
+    // written straight into the plan's level-0 rho
+    const double background = q / ps->dv / (double)n;
+    rc = pretune(ps, st);
+    if (rc) return rc;
+    e = launch_sub_scalar(ps->rho[0], rho, background, n, st);
+    if (e == hipSuccess) e = launch_copy(ps->phiA[0], phi, DT_F64, n, st);
+    if (e != hipSuccess) return gpaw_check(e, "poisson: load");
+    *neutral = 1;
+    return solve_loaded(ps, phi, eps, maxiter, niter, error, st);
+}
+
+static int solve_loaded(gpaw_poisson* ps, double* phi, double eps, int maxiter, int* niter,
+                        double* error, hipStream_t st)
+{
+    int rc = ensure_chunks(ps);
     if (rc) return rc;
     // The reference runs `while iterate2() > eps and niter < maxiter:
     // niter += 1` (gpaw/poisson.py:203-209), one V-cycle and one norm per
@@ -668,6 +715,7 @@ extern "C" void gpaw_poisson_free(gpaw_poisson* ps)
     if (ps->tmp2) hipFree(ps->tmp2);
     if (ps->d_chunk_partials) hipFree(ps->d_chunk_partials);
     if (ps->d_errs) hipFree(ps->d_errs);
+    if (ps->d_sum) hipFree(ps->d_sum);
     if (ps->h_errs) hipHostFree(ps->h_errs);
     if (ps->d_coarse) hipFree(ps->d_coarse);
     if (ps->d_coarse_nb) hipFree(ps->d_coarse_nb);

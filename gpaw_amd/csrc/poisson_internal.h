@@ -43,6 +43,7 @@ struct gpaw_poisson {
         int done = 0, total = 0;
     } rot;
     double* d_errs = nullptr;
+    double* d_sum = nullptr;   // 1024 partial sums of the charge integral
     double* h_errs = nullptr;
     std::vector<hipGraphExec_t> chunk;   // index 2 * (k - 1) + state
     int use_graph = 1;

@@ -148,7 +148,15 @@ class PoissonSolver:
             eps = self.eps
         P, R = self._args(phi, rho)
         phi_d, rho_d = P.tensor, R.tensor
-        actual_charge = self.gd.integrate(rho_d)
+        if (charge is None and self.B is None and
+                type(self).solve_neutral is PoissonSolver.solve_neutral):
+            # the charge integral, the neutral branch's mean subtraction and
+            # the solve in one call (same kernels, same bits)
+            niter, actual_charge = self._solve_device(P, R, eps, maxcharge)
+            if niter is not None:
+                return niter
+        else:
+            actual_charge = self.gd.integrate(rho_d)
         background = (actual_charge / self.gd.dv /
                       self.gd.get_size_of_global_array().prod())
         if charge is None:
@@ -184,6 +192,35 @@ class PoissonSolver:
             raise NotImplementedError
         P.finish()
         return niter
+
+    def _solve_device(self, P, R, eps, maxcharge):
+        """gpaw_poisson_solve: (niter, charge) when rho was neutral and the
+        solve ran, else (None, charge) for the charged branches."""
+        self.phis[0] = P.tensor
+        q = ctypes.c_double(0.0)
+        neutral = ctypes.c_int(0)
+        niter = ctypes.c_int(0)
+        err = ctypes.c_double(0.0)
+        rc = _lib.lib().gpaw_poisson_solve(
+            self._plan, P.ptr, R.ptr, float(eps), int(self.maxiter), float(maxcharge),
+            ctypes.byref(q), ctypes.byref(neutral), ctypes.byref(niter), ctypes.byref(err),
+            _arrays.stream())
+        if not neutral.value:
+            _lib.check(rc)
+            return None, q.value
+        self.error = err.value
+        if rc == _lib.GPAW_ENOCONV:
+            P.finish()
+            raise PoissonConvergenceError(
+                'Poisson solver did not converge in %d iterations!' %
+                self.maxiter)
+        _lib.check(rc)
+        if np.all(self.gd.pbc_c):
+            phi_ave = self.gd.integrate(P.tensor) / self.gd.dv
+            phi_ave /= np.prod(self.gd.get_size_of_global_array())
+            ops.sub_scalar(P.tensor, P.tensor, phi_ave)
+        P.finish()
+        return niter.value, q.value
 
     def _args(self, phi, rho):
         """phi (written) and rho as device arguments; both must be float64

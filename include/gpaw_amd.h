@@ -132,6 +132,17 @@ int gpaw_poisson_solve_neutral(gpaw_poisson* ps, double* phi,
                                const double* rho, double eps, int maxiter,
                                int* niter, double* error, void* stream);
 
+/* PoissonSolver.solve (gpaw/poisson.py:124-191) up to its branch on the
+ * charge: *charge = dv * sum(rho) (gd.integrate); if |charge| <= maxcharge
+ * the neutral branch runs here -- rho minus its mean (charge / dv / npts)
+ * into the plan, then solve_neutral as above -- and *neutral = 1; else
+ * nothing more is done, *neutral = 0, and the caller takes the charged
+ * branches.  One host round trip for the charge instead of a readback, a
+ * subtraction and a copy from the caller; the same bits. */
+int gpaw_poisson_solve(gpaw_poisson* ps, double* phi, const double* rho, double eps,
+                       int maxiter, double maxcharge, double* charge, int* neutral,
+                       int* niter, double* error, void* stream);
+
 /* Exactly ncycles V-cycles; errors[i] (host) gets dv*sum(r^2) after
  * cycle i.  Used for fixed-cycle comparisons. */
 int gpaw_poisson_vcycles(gpaw_poisson* ps, double* phi, const double* rho,

@@ -360,3 +360,38 @@ def test_argument_shape_and_dtype_checked():
         gops.sub_scalar(rho, t.zeros(5, dtype=t.float64, device='cuda'), 1.0)
     with pytest.raises(TypeError):
         gops.axpby(rho, rho.float(), 1.0, 1.0)
+
+
+@pytest.mark.parametrize('N,pbc', [(40, False), (32, True)])
+def test_device_solve_equals_python_branches(N, pbc):
+    """PoissonSolver.solve takes the charge integral, the neutral branch's
+    mean subtraction and the solve in one call (gpaw_poisson_solve); the
+    result equals the branch logic of gpaw/poisson.py:124-191 run step by
+    step from Python (gd.integrate, the background subtraction,
+    solve_neutral) bit for bit, with equal niter; a charged density is
+    handed back to the charged branches."""
+    print('case device solve', N, pbc, flush=True)
+    from gpaw_amd import ops
+    og, gd, ops_, gps = _pair(N, 0.2 / BOHR, pbc)
+    rho = _density(og, N + 1, 4, 0.5 / BOHR, 1.5 / BOHR, 2.0 / BOHR)
+    npts = og.size_of_global_array().prod()
+    rho -= og.integrate(rho) / og.dv / npts
+    rho += 5e-7 / og.dv / npts     # a charge of 5e-7: the neutral branch, a mean to remove
+    phi1 = dev(np.zeros(tuple(og.n_c)))
+    n1 = gps.solve(phi1, dev(rho))
+    q = gd.integrate(dev(rho))
+    assert abs(q) <= 1e-6
+    rho_n = dev(rho).clone()
+    ops.sub_scalar(rho_n, dev(rho), q / gd.dv / gd.get_size_of_global_array().prod())
+    phi2 = dev(np.zeros(tuple(og.n_c)))
+    n2 = gps.solve_neutral(phi2, rho_n, eps=gps.eps)
+    assert n1 == n2
+    assert _torch().equal(phi1, phi2)
+    if not pbc:   # charged: the Gaussian-monopole branch, against the oracle
+        rc = rho + 0.3 / og.dv / npts
+        phi = dev(np.zeros(tuple(og.n_c)))
+        n = gps.solve(phi, dev(rc))
+        want = og.zeros()
+        n_ref = ops_.solve(want, rc.copy())
+        assert n == n_ref
+        assert rel_maxnorm(phi.cpu().numpy(), want) < 1e-12
