@@ -769,7 +769,8 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
                                         double rel_tol, int maxiter, int mode, int inner,
                                         int* niter, double* relres, void* stream)
 {
-    if (!ps || !phi || !rho || maxiter < 1 || rel_tol <= 0.0 || mode < 0 || mode > 7 ||
+    // rho == nullptr: the density gpaw_poisson_neutralize left in the plan
+    if (!ps || !phi || (!rho && !ps->rho_loaded) || maxiter < 1 || rel_tol <= 0.0 || mode < 0 || mode > 7 ||
         inner < 1)
         return gpaw_fail(GPAW_EINVAL, "mixed: bad argument");
     if (ps->method != 2) return gpaw_fail(GPAW_EUNSUPPORTED, "mixed: Jacobi smoothing only");
@@ -791,7 +792,8 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
     if (!rc) rc = capture_all(ps, pr);
     if (rc) return rc;
     const i64 n = ps->npts[0];
-    hipError_t e = launch_copy(ps->rho[0], rho, DT_F64, n, st);
+    hipError_t e = rho ? launch_copy(ps->rho[0], rho, DT_F64, n, st) : hipSuccess;
+    ps->rho_loaded = false;
     if (e == hipSuccess) e = launch_copy(ps->phiA[0], phi, DT_F64, n, st);
     if (e == hipSuccess) {   // |rho|^2 for the relative criterion
         launch_kernel(k_dot, gridn(n), dim3(256), 0, st, ps->rho[0], ps->rho[0], n, m->partial);

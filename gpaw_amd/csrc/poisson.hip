@@ -549,6 +549,7 @@ static int load_inputs(gpaw_poisson* ps, const double* phi, const double* rho, h
 {
     int rc = pretune(ps, st);
     if (rc) return rc;
+    ps->rho_loaded = false;
     hipError_t e = launch_copy(ps->rho[0], rho, DT_F64, ps->npts[0], st);
     if (e == hipSuccess) e = launch_copy(ps->phiA[0], phi, DT_F64, ps->npts[0], st);
     return gpaw_check(e, "poisson: load");
@@ -588,14 +589,13 @@ extern "C" int gpaw_poisson_solve_neutral(gpaw_poisson* ps, double* phi, const d
     return solve_loaded(ps, phi, eps, maxiter, niter, error, st);
 }
 
-extern "C" int gpaw_poisson_solve(gpaw_poisson* ps, double* phi, const double* rho, double eps,
-                                  int maxiter, double maxcharge, double* charge, int* neutral,
-                                  int* niter, double* error, void* stream)
+extern "C" int gpaw_poisson_neutralize(gpaw_poisson* ps, const double* rho, double maxcharge,
+                                       double* charge, int* neutral, void* stream)
 {
-    if (!ps || !phi || !rho || !charge || !neutral || maxiter < 1)
-        return gpaw_fail(GPAW_EINVAL, "poisson: bad argument");
+    if (!ps || !rho || !charge || !neutral) return gpaw_fail(GPAW_EINVAL, "poisson: bad argument");
     hipStream_t st = (hipStream_t)stream;
     *neutral = 0;
+    ps->rho_loaded = false;
     int rc = ensure_chunks(ps);
     if (rc) return rc;
     // the charge integral dv * sum(rho) of gd.integrate, the same kernels
@@ -612,14 +612,30 @@ extern "C" int gpaw_poisson_solve(gpaw_poisson* ps, double* phi, const double* r
     *charge = q;
     if (!(std::fabs(q) <= maxcharge)) return GPAW_OK;   // charged: the caller's branches
     // neutral: rho minus its mean, as gpaw/poisson.py:130-138 computes it,
-    // written straight into the plan's level-0 rho
-    const double background = q / ps->dv / (double)n;
+    // written straight into the plan's level-0 rho (the eager pre-tuning
+    // V-cycle, which works on the plan's arrays, runs first)
     rc = pretune(ps, st);
     if (rc) return rc;
+    const double background = q / ps->dv / (double)n;
     e = launch_sub_scalar(ps->rho[0], rho, background, n, st);
-    if (e == hipSuccess) e = launch_copy(ps->phiA[0], phi, DT_F64, n, st);
     if (e != hipSuccess) return gpaw_check(e, "poisson: load");
+    ps->rho_loaded = true;
     *neutral = 1;
+    return GPAW_OK;
+}
+
+extern "C" int gpaw_poisson_solve(gpaw_poisson* ps, double* phi, const double* rho, double eps,
+                                  int maxiter, double maxcharge, double* charge, int* neutral,
+                                  int* niter, double* error, void* stream)
+{
+    if (!ps || !phi || !rho || !charge || !neutral || maxiter < 1)
+        return gpaw_fail(GPAW_EINVAL, "poisson: bad argument");
+    int rc = gpaw_poisson_neutralize(ps, rho, maxcharge, charge, neutral, stream);
+    if (rc || !*neutral) return rc;
+    ps->rho_loaded = false;
+    hipStream_t st = (hipStream_t)stream;
+    rc = gpaw_check(launch_copy(ps->phiA[0], phi, DT_F64, ps->npts[0], st), "poisson: load");
+    if (rc) return rc;
     return solve_loaded(ps, phi, eps, maxiter, niter, error, st);
 }
 

@@ -44,6 +44,9 @@ struct gpaw_poisson {
     } rot;
     double* d_errs = nullptr;
     double* d_sum = nullptr;   // 1024 partial sums of the charge integral
+    // rho[0] holds a density gpaw_poisson_neutralize wrote (a solve may
+    // then take rho = nullptr)
+    bool rho_loaded = false;
     double* h_errs = nullptr;
     std::vector<hipGraphExec_t> chunk;   // index 2 * (k - 1) + state
     int use_graph = 1;

@@ -148,10 +148,10 @@ class PoissonSolver:
             eps = self.eps
         P, R = self._args(phi, rho)
         phi_d, rho_d = P.tensor, R.tensor
-        if (charge is None and self.B is None and
-                type(self).solve_neutral is PoissonSolver.solve_neutral):
-            # the charge integral, the neutral branch's mean subtraction and
-            # the solve in one call (same kernels, same bits)
+        if charge is None and self._device_branch():
+            # the charge integral and the neutral branch's mean subtraction
+            # on the device, the solve from the density left in the plan
+            # (same kernels, same bits)
             niter, actual_charge = self._solve_device(P, R, eps, maxcharge)
             if niter is not None:
                 return niter
@@ -193,6 +193,12 @@ class PoissonSolver:
         P.finish()
         return niter
 
+    def _device_branch(self):
+        """Whether solve() may run its neutral branch on the device: a
+        Laplacian (no Mehrstellen right-hand side) and this class's own
+        solve_neutral."""
+        return self.B is None and type(self).solve_neutral is PoissonSolver.solve_neutral
+
     def _solve_device(self, P, R, eps, maxcharge):
         """gpaw_poisson_solve: (niter, charge) when rho was neutral and the
         solve ran, else (None, charge) for the charged branches."""
@@ -215,12 +221,16 @@ class PoissonSolver:
                 'Poisson solver did not converge in %d iterations!' %
                 self.maxiter)
         _lib.check(rc)
+        return self._finish_neutral(P, niter.value), q.value
+
+    def _finish_neutral(self, P, niter):
+        """The end of solve_neutral: phi's mean removed on periodic grids."""
         if np.all(self.gd.pbc_c):
             phi_ave = self.gd.integrate(P.tensor) / self.gd.dv
             phi_ave /= np.prod(self.gd.get_size_of_global_array())
             ops.sub_scalar(P.tensor, P.tensor, phi_ave)
         P.finish()
-        return niter.value, q.value
+        return niter
 
     def _args(self, phi, rho):
         """phi (written) and rho as device arguments; both must be float64
@@ -329,15 +339,35 @@ class MixedPoissonSolver(PoissonSolver):
         self.maxiter = maxiter
         self.reuse_history = reuse_history
 
+    def _device_branch(self):
+        return self.relax_method == 2 and self.B is None
+
+    def _solve_device(self, P, R, eps, maxcharge):
+        """gpaw_poisson_neutralize, then the mixed solve of the density it
+        left in the plan: (niter, charge), or (None, charge) when charged."""
+        q = ctypes.c_double(0.0)
+        neutral = ctypes.c_int(0)
+        _lib.check(_lib.lib().gpaw_poisson_neutralize(
+            self._plan, R.ptr, float(maxcharge), ctypes.byref(q), ctypes.byref(neutral),
+            _arrays.stream()))
+        if not neutral.value:
+            return None, q.value
+        return self._solve_mixed(P, None), q.value
+
     def solve_neutral(self, phi, rho, eps=None):
         if self.relax_method != 2 or self.B is not None:
             raise NotImplementedError('mixed precision needs Jacobi and a '
                                       'Laplace operator')
         P, R = self._args(phi, rho)
+        return self._solve_mixed(P, R.ptr)
+
+    def _solve_mixed(self, P, rho_ptr):
+        """gpaw_poisson_solve_mixed on phi P and rho at rho_ptr (None: the
+        density gpaw_poisson_neutralize left in the plan)."""
         niter = ctypes.c_int(0)
         rel = ctypes.c_double(0.0)
         rc = _lib.lib().gpaw_poisson_solve_mixed(
-            self._plan, P.ptr, R.ptr, float(self.rel_tol), int(self.maxiter),
+            self._plan, P.ptr, rho_ptr, float(self.rel_tol), int(self.maxiter),
             (1 if self.outer == 'pcg' else 0) |
             (2 if self.inner_precision == 'fp64' else 0) |
             (4 if self.reuse_history else 0), int(self.inner),
@@ -349,12 +379,7 @@ class MixedPoissonSolver(PoissonSolver):
                 'mixed-precision Poisson solver did not converge in %d '
                 'iterations' % self.maxiter)
         _lib.check(rc)
-        if np.all(self.gd.pbc_c):
-            phi_ave = self.gd.integrate(P.tensor) / self.gd.dv
-            phi_ave /= np.prod(self.gd.get_size_of_global_array())
-            ops.sub_scalar(P.tensor, P.tensor, phi_ave)
-        P.finish()
-        return niter.value
+        return self._finish_neutral(P, niter.value)
 
     def smoothing_weights(self):
         """Level-0 (pre, post) Jacobi weights of the inner V-cycle, one per

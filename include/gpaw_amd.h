@@ -132,13 +132,18 @@ int gpaw_poisson_solve_neutral(gpaw_poisson* ps, double* phi,
                                const double* rho, double eps, int maxiter,
                                int* niter, double* error, void* stream);
 
-/* PoissonSolver.solve (gpaw/poisson.py:124-191) up to its branch on the
- * charge: *charge = dv * sum(rho) (gd.integrate); if |charge| <= maxcharge
- * the neutral branch runs here -- rho minus its mean (charge / dv / npts)
- * into the plan, then solve_neutral as above -- and *neutral = 1; else
- * nothing more is done, *neutral = 0, and the caller takes the charged
- * branches.  One host round trip for the charge instead of a readback, a
- * subtraction and a copy from the caller; the same bits. */
+/* The start of PoissonSolver.solve (gpaw/poisson.py:124-138): *charge =
+ * dv * sum(rho) (gd.integrate); if |charge| <= maxcharge, rho minus its
+ * mean (charge / dv / npts) goes into the plan's level-0 density and
+ * *neutral = 1 (a following gpaw_poisson_solve_mixed may then pass
+ * rho = NULL); else *neutral = 0 and nothing else is done (the caller's
+ * charged branches).  One host round trip for the charge; the same kernels
+ * and bits as the reference's steps from Python. */
+int gpaw_poisson_neutralize(gpaw_poisson* ps, const double* rho, double maxcharge,
+                            double* charge, int* neutral, void* stream);
+
+/* gpaw_poisson_neutralize, then for a neutral density solve_neutral as
+ * above (*neutral = 1); a charged one returns with *neutral = 0. */
 int gpaw_poisson_solve(gpaw_poisson* ps, double* phi, const double* rho, double eps,
                        int maxiter, double maxcharge, double* charge, int* neutral,
                        int* niter, double* error, void* stream);
@@ -186,7 +191,8 @@ int gpaw_poisson_mixed_weights(const gpaw_poisson* ps, double* pre, int* npre,
  * and so the last bits of phi, then depend on that history; without the
  * bit they depend only on the inputs).  Stops when
  * ||L phi - rho||_2 <= rel_tol * ||rho||_2 (rho as given, already
- * neutral), checked on the true residual; *relres receives the final
+ * neutral; NULL: the density gpaw_poisson_neutralize left in the plan),
+ * checked on the true residual; *relres receives the final
  * ratio.  A non-finite residual norm (breakdown) returns GPAW_EDEVICE and
  * leaves phi unchanged. */
 int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi,
