@@ -550,9 +550,9 @@ def mixed_vs_fp64(torch, np, N=160, h=0.18, count=32, reps=10):
     res['inner_vcycle'] = {
         'fp32_ms': round(tv['fp32'] * 1e3, 4), 'fp64_ms': round(tv['fp64'] * 1e3, 4),
         'fp32_over_fp64': round(tv['fp32'] / tv['fp64'], 3),
-        'note': 'one preconditioning V-cycle from zero (2 + 2 Chebyshev-weighted level-0 '
-                'sweeps, reference hierarchy below), FP32 including the FP64 <-> FP32 '
-                'conversion of r and z'}
+        'note': 'one preconditioning V-cycle from zero (2 + 2 Chebyshev-weighted sweeps on '
+                'levels 0 and 1, reference hierarchy below), FP32 including the FP64 <-> '
+                'FP32 conversion of r and z'}
     del pc
     res['config'] = ('C4: 160^3 zero boundary (159^3 arrays), h=0.18 A, 32 neutral '
                      'Gaussians, 1e-10 relative residual')

@@ -40,7 +40,7 @@ PROTOTYPES = {
     'gpaw_poisson_solve_neutral': (_i, [_vp, _vp, _vp, _d, _i, _ip, _dp,
                                         _vp]),
     'gpaw_poisson_vcycles': (_i, [_vp, _vp, _vp, _i, _dp, _vp]),
-    'gpaw_poisson_mixed_weights': (_i, [_vp, _dp, _ip, _dp, _ip]),
+    'gpaw_poisson_mixed_weights': (_i, [_vp, _i, _dp, _ip, _dp, _ip]),
     'gpaw_poisson_solve': (_i, [_vp, _vp, _vp, _d, _i, _d, _dp, _ip, _ip, _dp, _vp]),
     'gpaw_poisson_neutralize': (_i, [_vp, _vp, _d, _dp, _ip, _vp]),
     'gpaw_poisson_solve_mixed': (_i, [_vp, _vp, _vp, _d, _i, _i, _i, _ip, _dp,

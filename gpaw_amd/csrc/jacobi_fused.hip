@@ -76,10 +76,15 @@ __device__ __forceinline__ int jf_cwrap(int j, int n, int per, bool& out)
     return jf_wrap(j, n, per, out);
 }
 
+// weight of each fused sweep
+struct JfWeights {
+    double w[4];
+};
+
 template <class V, int S, int E>
 __global__ __launch_bounds__(jf_threads(E)) void k_jacobi_fused(const V* __restrict__ in, V* __restrict__ out,
                                                         const V* __restrict__ rho, Geom g,
-                                                        StarCoefs7 c, double w, int nb1, int nb2,
+                                                        StarCoefs7 c, JfWeights w, int nb1, int nb2,
                                                         const V* __restrict__ corr, TrArgs ct,
                                                         int zero_in)
 {
@@ -103,7 +108,6 @@ __global__ __launch_bounds__(jf_threads(E)) void k_jacobi_fused(const V* __restr
     const int col = j1 * n2 + j2;
     const long plane = (long)n1 * n2;
     typedef typename VT<V>::R RT;
-    const RT wr = (RT)w;
     // every global load is issued before the first is used: this thread's
     // box cells (zero outside a zero-boundary axis) and their rho, then the
     // coarse values of the correction
@@ -181,7 +185,7 @@ __global__ __launch_bounds__(jf_threads(E)) void k_jacobi_fused(const V* __restr
                 xs = vmac(xs, src[cell - E], (RT)c.c[4]);
                 xs = vmac(xs, src[cell + E * E], (RT)c.c[5]);
                 xs = vmac(xs, src[cell - E * E], (RT)c.c[6]);
-                const V f = st_epilogue<ST_JACOBI>(xs, src[cell], r[p], wr, (RT)c.c[0]);
+                const V f = st_epilogue<ST_JACOBI>(xs, src[cell], r[p], (RT)w.w[s - 1], (RT)c.c[0]);
                 dst[cell] = ((outside >> p) & 1) ? V(0) : f;
             }
         }
@@ -231,7 +235,8 @@ int jacobi_fused_sweeps(const Geom& g)
 
 hipError_t launch_jacobi_fused(const Stencil& s, int dtype, const void* in, void* out,
                                const void* rho, const Geom& g, double w, hipStream_t st,
-                               const void* corr, const Transfer* ctr, bool zero_in, int sweeps)
+                               const void* corr, const Transfer* ctr, bool zero_in, int sweeps,
+                               const double* ws)
 {
     if (!jacobi_fused_ok(s, g) || (sweeps != 2 && sweeps != 4)) return hipErrorInvalidValue;
     TrArgs ct{};
@@ -244,13 +249,15 @@ hipError_t launch_jacobi_fused(const Stencil& s, int dtype, const void* in, void
     }
     StarCoefs7 c;
     for (int k = 0; k < 7; k++) c.c[k] = s.coefs[k];
+    JfWeights jw;
+    for (int k = 0; k < 4; k++) jw.w[k] = ws && k < sweeps ? ws[k] : w;
     const int E = jacobi_fused_edge(g, sweeps);
     const int B = jf_box(sweeps, E);
     const int nb0 = (g.n[0] + B - 1) / B, nb1 = (g.n[1] + B - 1) / B, nb2 = (g.n[2] + B - 1) / B;
     const dim3 grid((unsigned)(nb0 * nb1 * nb2));
 #define GPAW_JF(T, S, EE)                                                                    \
     launch_kernel(k_jacobi_fused<T, S, EE>, grid, dim3(jf_threads(EE)), 0, st, (const T*)in,  \
-                  (T*)out, (const T*)rho, g, c, w, nb1, nb2, (const T*)corr, ct, (int)zero_in)
+                  (T*)out, (const T*)rho, g, c, jw, nb1, nb2, (const T*)corr, ct, (int)zero_in)
 #define GPAW_JF_E(T, S)                                                                      \
     if (E == 16)                                                                             \
         GPAW_JF(T, S, 16);                                                                   \

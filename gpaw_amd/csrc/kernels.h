@@ -129,7 +129,8 @@ int jacobi_fused_edge(const Geom& g, int sweeps);
 hipError_t launch_jacobi_fused(const Stencil& s, int dtype, const void* in, void* out,
                                const void* rho, const Geom& g, double w, hipStream_t st,
                                const void* corr = nullptr, const Transfer* ctr = nullptr,
-                               bool zero_in = false, int sweeps = JF_SWEEPS);
+                               bool zero_in = false, int sweeps = JF_SWEEPS,
+                               const double* ws = nullptr);   // per-sweep weights (else w)
 
 // Residual of a 7-point star stencil and its K = 2 restriction in one
 // launch: rho_c = restrict(L phi - rho), and phi_c = 0 when given; the

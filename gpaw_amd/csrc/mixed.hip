@@ -59,6 +59,8 @@ struct MixedState {
     // level-0 Jacobi weights per sweep (empty: the plan's weight for every
     // sweep): pre-smoothing in order, post-smoothing in the reverse order
     std::vector<double> w0pre, w0post;
+    // the same for levels 1 .. (empty: the plan's weight)
+    std::vector<std::vector<double>> wpre, wpost;
     VLevels v[2];                       // [0] FP32, [1] FP64 inner V-cycle
     bool ready[2] = {false, false};
     double *p = nullptr, *q = nullptr;  // FP64 search direction, L p
@@ -257,11 +259,13 @@ hipError_t relax_v(gpaw_poisson* ps, VLevels& V, int l, int k, void*& cur, void*
     // below level 0, runs of 4 (2 on large levels) sweeps are one fused launch (callers
     // follow `cur`, not the sweep parity)
     if (l > 0 && ps->fuse_jacobi && jacobi_fused_ok(op->st, op->g)) {
-        const int S = jacobi_fused_sweeps(op->g);
+        int S = jacobi_fused_sweeps(op->g);
+        if (k < S && k >= 2) S = 2;   // runs of two (schedules with fewer sweeps)
         for (; k >= S; k -= S) {
             hipError_t e = launch_jacobi_fused(op->st, V.dt, cur, alt, V.rho[l], op->g, ps->w[l],
                                                st, corr, corr ? &ps->itp[l]->t : nullptr,
-                                               zero_in, S);
+                                               zero_in, S, wts);
+            if (wts) wts += S;
             corr = nullptr;
             zero_in = false;
             if (e != hipSuccess) return e;
@@ -301,6 +305,10 @@ hipError_t emit_v(gpaw_poisson* ps, MixedState* m, VLevels& V, int l, bool zero0
     int pre = l < m->L ? m->pre[l] : 0;
     const double* wpre = (l == 0 && !m->w0pre.empty()) ? m->w0pre.data() : nullptr;
     const double* wpost = (l == 0 && !m->w0post.empty()) ? m->w0post.data() : nullptr;
+    if (l > 0 && l < (int)m->wpre.size() && !m->wpre[l].empty()) {
+        wpre = m->wpre[l].data();
+        wpost = m->wpost[l].data();
+    }
     if (l == 0 && zero0 && pre > 0 && ps->fuse_jacobi) {
         // the first sweep from zero, element-wise, into alt (as a sweep would)
         if (V.dt == DT_F32)
@@ -339,7 +347,7 @@ hipError_t emit_v(gpaw_poisson* ps, MixedState* m, VLevels& V, int l, bool zero0
         if (e != hipSuccess) return e;
         if (l > 0 && ps->fold_correction && ps->fuse_jacobi && jacobi_fused_ok(op->st, op->g) &&
             m->post[l] >= jacobi_fused_sweeps(op->g) && ps->itp[l]->t.K == 2) {
-            e = relax_v(ps, V, l, m->post[l], cur, alt, st, coarse);
+            e = relax_v(ps, V, l, m->post[l], cur, alt, st, coarse, false, wpost);
             res = cur;
             return e;
         }
@@ -450,6 +458,8 @@ int setup_v64(gpaw_poisson* ps, MixedState* m)
 }
 
 std::vector<double> chebyshev_weights(const Stencil& s, double alpha, int k);
+void mixed_level_schedule(const gpaw_poisson* ps, int l, int& pre, int& post,
+                          std::vector<double>& wpre, std::vector<double>& wpost);
 
 // The inner V-cycle's level-0 smoothing: 2 pre + 2 post Jacobi sweeps whose
 // weights are the Chebyshev smoother for the top 1/7 of the spectrum of
@@ -482,6 +492,40 @@ void mixed_schedule(const gpaw_poisson* ps, int& pre, int& post, std::vector<dou
     }
 }
 
+// Levels below 0: the plan's sweeps and weight, except level 1 when its
+// fused launches do two sweeps (a level whose 12^3 boxes fill the chip,
+// e.g. 79^3 under 159^3): there 2 + 2 Chebyshev-weighted sweeps (the top
+// 1/6 of the spectrum) save two of its four launches per V-cycle at the
+// same CG step count (159^3: 3.09 -> 2.80 ms; on smaller grids, whose level
+// 1 takes four sweeps per launch anyway, 4 + 4 stay: 2 + 2 added a step on
+// 96^3).  GPAW_AMD_MIXED_L1=pre,post,alpha overrides level 1 (alpha <= 1:
+// the plan's weight).
+void mixed_level_schedule(const gpaw_poisson* ps, int l, int& pre, int& post,
+                          std::vector<double>& wpre, std::vector<double>& wpost)
+{
+    pre = l < ps->L ? ps->pre[l] : 0;
+    post = ps->post[l];
+    wpre.clear();
+    wpost.clear();
+    if (l != 1 || ps->L < 2 || ps->lc == 1 || ps->ops[1]->st.star_r <= 0) return;
+    int a = 0, b = 0;
+    double alpha = 0.0;
+    if (jacobi_fused_ok(ps->ops[1]->st, ps->ops[1]->g) &&
+        jacobi_fused_sweeps(ps->ops[1]->g) == 2) {
+        a = b = 2;
+        alpha = 6.0;
+    }
+    if (const char* env = getenv("GPAW_AMD_MIXED_L1")) sscanf(env, "%d,%d,%lf", &a, &b, &alpha);
+    if (a <= 0 || b <= 0 || (a + b) % 2 || a > MIXED_MAX_SWEEPS || b > MIXED_MAX_SWEEPS) return;
+    pre = a;
+    post = b;
+    if (alpha > 1.0) {
+        wpre = chebyshev_weights(ps->ops[1]->st, alpha, a);
+        wpost = chebyshev_weights(ps->ops[1]->st, alpha, b);
+        std::reverse(wpost.begin(), wpost.end());
+    }
+}
+
 int setup(gpaw_poisson* ps, int pr)
 {
     if (!ps->mixed) {
@@ -491,6 +535,10 @@ int setup(gpaw_poisson* ps, int pr)
         m->pre = ps->pre;
         m->post = ps->post;
         mixed_schedule(ps, m->pre[0], m->post[0], m->w0pre, m->w0post);
+        m->wpre.assign(ps->L + 1, {});
+        m->wpost.assign(ps->L + 1, {});
+        for (int l = 1; l <= ps->L; l++)
+            mixed_level_schedule(ps, l, m->pre[l], m->post[l], m->wpre[l], m->wpost[l]);
         const size_t n0 = sizeof(double) * ps->npts[0];
         int rc = gpaw_check(hipMalloc((void**)&m->p, n0), "mixed: p");
         if (!rc) rc = gpaw_check(hipMalloc((void**)&m->q, n0), "mixed: q");
@@ -941,16 +989,20 @@ extern "C" int gpaw_poisson_solve_mixed(gpaw_poisson* ps, double* phi, const dou
     return GPAW_OK;
 }
 
-extern "C" int gpaw_poisson_mixed_weights(const gpaw_poisson* ps, double* pre, int* npre,
-                                          double* post, int* npost)
+extern "C" int gpaw_poisson_mixed_weights(const gpaw_poisson* ps, int level, double* pre,
+                                          int* npre, double* post, int* npost)
 {
-    if (!ps || !npre || !npost) return gpaw_fail(GPAW_EINVAL, "mixed weights: bad argument");
+    if (!ps || !npre || !npost || level < 0 || level > ps->L)
+        return gpaw_fail(GPAW_EINVAL, "mixed weights: bad argument");
     int a = 0, b = 0;
     std::vector<double> wa, wb;
-    mixed_schedule(ps, a, b, wa, wb);
-    *npre = ps->L > 0 ? a : 0;
+    if (level == 0)
+        mixed_schedule(ps, a, b, wa, wb);
+    else
+        mixed_level_schedule(ps, level, a, b, wa, wb);
+    *npre = level < ps->L ? a : 0;
     *npost = b;
-    for (int i = 0; i < *npre && pre; i++) pre[i] = wa.empty() ? ps->w[0] : wa[i];
-    for (int i = 0; i < b && post; i++) post[i] = wb.empty() ? ps->w[0] : wb[i];
+    for (int i = 0; i < *npre && pre; i++) pre[i] = wa.empty() ? ps->w[level] : wa[i];
+    for (int i = 0; i < b && post; i++) post[i] = wb.empty() ? ps->w[level] : wb[i];
     return GPAW_OK;
 }

@@ -381,14 +381,14 @@ class MixedPoissonSolver(PoissonSolver):
         _lib.check(rc)
         return self._finish_neutral(P, niter.value)
 
-    def smoothing_weights(self):
-        """Level-0 (pre, post) Jacobi weights of the inner V-cycle, one per
-        sweep in the order they run."""
+    def smoothing_weights(self, level=0):
+        """(pre, post) Jacobi weights of level `level` in the inner V-cycle,
+        one per sweep in the order they run."""
         pre = (ctypes.c_double * 16)()
         post = (ctypes.c_double * 16)()
         npre, npost = ctypes.c_int(0), ctypes.c_int(0)
         _lib.check(_lib.lib().gpaw_poisson_mixed_weights(
-            self._plan, pre, ctypes.byref(npre), post, ctypes.byref(npost)))
+            self._plan, level, pre, ctypes.byref(npre), post, ctypes.byref(npost)))
         return list(pre[:npre.value]), list(post[:npost.value])
 
     def precondition(self, r, z, precision=None):

@@ -165,15 +165,18 @@ int gpaw_poisson_vcycles(gpaw_poisson* ps, double* phi, const double* rho,
 int gpaw_poisson_precondition(gpaw_poisson* ps, const double* r, double* z,
                               int inner_precision, void* stream);
 
-/* The level-0 smoothing of the mixed solver's inner V-cycle (no reference
- * counterpart): *npre pre- and *npost post-smoothing Jacobi sweeps and the
- * weight of each, in the order they run (pre / post: room for 16 each, or
- * null to query the counts).  By default 2 + 2 sweeps whose weights make
- * the product of the sweeps' factors prod (1 - w_i D^-1 A) the degree-2
- * Chebyshev polynomial smallest on [lmax/7, lmax] (lmax the largest
- * eigenvalue of D^-1 A), the post-smoothing weights in reverse order; the
- * levels below keep the reference's smoothing. */
-int gpaw_poisson_mixed_weights(const gpaw_poisson* ps, double* pre, int* npre,
+/* The smoothing of level `level` in the mixed solver's inner V-cycle (no
+ * reference counterpart): *npre pre- and *npost post-smoothing Jacobi
+ * sweeps and the weight of each, in the order they run (pre / post: room
+ * for 16 each, or null to query the counts).  Level 0 by default: 2 + 2
+ * sweeps whose weights make the product of the sweeps' factors
+ * prod (1 - w_i D^-1 A) the degree-2 Chebyshev polynomial smallest on
+ * [lmax/7, lmax] (lmax the largest eigenvalue of D^-1 A), the
+ * post-smoothing weights in reverse order.  Level 1, when its fused
+ * smoothing launches do two sweeps each (a level 1 of at least 73 points
+ * along each axis, e.g. 79^3 under 159^3): 2 + 2 sweeps on [lmax/6, lmax] the same way.  The other levels
+ * keep the reference's sweeps and weight. */
+int gpaw_poisson_mixed_weights(const gpaw_poisson* ps, int level, double* pre, int* npre,
                                double* post, int* npost);
 
 /* Mixed-precision solve of the same problem (no reference counterpart: the

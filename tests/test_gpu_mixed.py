@@ -148,29 +148,32 @@ def test_inner_precision_iteration_counts():
     assert abs(its['fp32'] - its['fp64']) <= 2
 
 
-def _weighted_level0(ops_, wpre, wpost):
-    """The oracle's V-cycle with the mixed solver's level-0 sweeps: each
-    level-0 relax call (pre, then post, per cycle) runs one Jacobi sweep per
-    weight of its list."""
-    orig = ops_.operators[0].relax
+def _weighted_level(ops_, level, wpre, wpost):
+    """The oracle's V-cycle with the mixed solver's sweeps on one level
+    (not the coarsest): each relax call of that level (pre, then post, per
+    cycle) runs one Jacobi sweep per weight of its list."""
+    ops_.presmooths[level] = len(wpre)
+    ops_.postsmooths[level] = len(wpost)
+    orig = ops_.operators[level].relax
     calls = [0]
 
-    def relax0(method, f, s, n, w=1.0):
+    def relax(method, f, s, n, w=1.0):
         ws = wpre if calls[0] % 2 == 0 else wpost
         calls[0] += 1
         assert n == len(ws)
         for wi in ws:
             orig(method, f, s, 1, wi)
-    ops_.operators[0].relax = relax0
+    ops_.operators[level].relax = relax
 
 
-@pytest.mark.parametrize('N,pbc', [(64, False), (40, True)])
+@pytest.mark.parametrize('N,pbc', [(64, False), (40, True), (160, False)])
 def test_preconditioner_vcycle(N, pbc):
     """gpaw_poisson_precondition: the FP64 inner V-cycle from zero equals the
     oracle's V-cycle (gpaw/poisson.py:245-270) with the mixed solver's
-    level-0 sweeps and weights (gpaw_poisson_mixed_weights), bit for bit; the
-    FP32 one agrees with it to the FP32 rounding level (1e-5 relative
-    max-norm)."""
+    level-0 and level-1 sweeps and weights (gpaw_poisson_mixed_weights), bit
+    for bit; the FP32 one agrees with it to the FP32 rounding level (1e-5
+    relative max-norm).  On 159^3 level 1 (79^3) runs Chebyshev-weighted
+    2 + 2 sweeps, on the smaller grids the reference's 4 + 4."""
     print('case precondition', N, pbc, flush=True)
     t = _torch()
     from gpaw_amd.poisson import MixedPoissonSolver
@@ -180,6 +183,12 @@ def test_preconditioner_vcycle(N, pbc):
     ms.set_grid_descriptor(gd)
     ms.initialize()
     wpre, wpost = ms.smoothing_weights()
+    w1pre, w1post = ms.smoothing_weights(1)
+    print('  level 1 weights', w1pre, w1post, flush=True)
+    if N == 160:
+        assert len(w1pre) == 2 and w1post == w1pre[::-1] and w1pre[0] != w1pre[1]
+    else:
+        assert len(w1pre) == len(w1post) == 4 and len(set(w1pre + w1post)) == 1
     z64 = t.full(tuple(gd.n_c), np.nan, dtype=t.float64, device='cuda')
     z32 = t.full(tuple(gd.n_c), np.nan, dtype=t.float64, device='cuda')
     rd = t.from_numpy(r).to('cuda')
@@ -189,9 +198,11 @@ def test_preconditioner_vcycle(N, pbc):
     ops_ = O.PoissonSolver(nn=3)
     ops_.set_grid_descriptor(og)
     ops_.initialize()
-    ops_.presmooths[0] = len(wpre)
-    ops_.postsmooths[0] = len(wpost)
-    _weighted_level0(ops_, wpre, wpost)
+    _weighted_level(ops_, 0, wpre, wpost)
+    if N == 160:
+        _weighted_level(ops_, 1, w1pre, w1post)
+    else:
+        assert w1pre[0] == ops_.weights[1]
     want = og.zeros()
     ops_.run_vcycles(want, r, 1)
     z64 = z64.cpu().numpy()
