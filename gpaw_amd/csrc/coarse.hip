@@ -36,12 +36,15 @@ enum { PA = 0, PB = 1, RHO = 2, RES = 3 };
 // A level has at most 2048 points (two per thread).  Each thread's points
 // and their six neighbours (z+, z-, y+, y-, x+, x-: the summation order of
 // the 7-point operator, centre first) come from the plan's table; a
-// neighbour outside a zero boundary is -1.
+// neighbour outside a zero boundary is -1 in the table.  ZS (the LDS
+// layout): such a neighbour reads the level's zero slot instead, element n
+// of its phi arrays, which holds 0.0 for the whole kernel.
 struct Pts {
     int idx[2];       // flat index, -1 when the thread has no such point
     int nb[2][6];
 };
 
+template <bool ZS>
 __device__ __forceinline__ void level_points(const CoarseLevel& L, const short* __restrict__ tab,
                                              Pts& p)
 {
@@ -52,22 +55,26 @@ __device__ __forceinline__ void level_points(const CoarseLevel& L, const short* 
         p.idx[k] = idx < n ? idx : -1;
         const short* t = tab + L.nb_off + 6 * (idx < n ? idx : 0);
 #pragma unroll
-        for (int m = 0; m < 6; m++) p.nb[k][m] = t[m];
+        for (int m = 0; m < 6; m++) p.nb[k][m] = (ZS && t[m] < 0) ? n : t[m];
     }
 }
 
 // x + the six neighbour terms in summation order (the coarse levels are
-// real, phases 1, so a neighbour is a plain load or zero)
+// real, phases 1, so a neighbour is a plain load or zero).  A zero
+// neighbour contributes 0.0 * c either way: the same bits with or without
+// the zero slot.
+template <bool ZS>
 __device__ __forceinline__ double nb_sum(const double* __restrict__ a, const int* nb,
                                          const double* c, double x)
 {
-    // every load issued unconditionally (index 0 for a zero neighbour, whose
-    // value is then replaced by 0): no branch between the loads
+    // every load issued unconditionally (without the zero slot: index 0 for
+    // a zero neighbour, whose value is then replaced by 0): no branch
+    // between the loads
     double v[6];
 #pragma unroll
-    for (int m = 0; m < 6; m++) v[m] = a[nb[m] >= 0 ? nb[m] : 0];
+    for (int m = 0; m < 6; m++) v[m] = a[(ZS || nb[m] >= 0) ? nb[m] : 0];
 #pragma unroll
-    for (int m = 0; m < 6; m++) x = x + (nb[m] >= 0 ? v[m] : 0.0) * c[m + 1];
+    for (int m = 0; m < 6; m++) x = x + ((ZS || nb[m] >= 0) ? v[m] : 0.0) * c[m + 1];
     return x;
 }
 
@@ -83,10 +90,10 @@ template <bool LDS>
 __device__ __forceinline__ LevelArrays level_arrays(const CoarseLevel& L, double* smem)
 {
     LevelArrays r;
-    if (LDS) {
+    if (LDS) {   // n + 1 elements each: element n of phi A / B is the zero slot
         const int n = L.g.n[0] * L.g.n[1] * L.g.n[2];
 #pragma unroll
-        for (int k = 0; k < 4; k++) r.a[k] = smem + L.off + k * n;
+        for (int k = 0; k < 4; k++) r.a[k] = smem + L.off + k * (n + 1);
     } else {
         r.a[PA] = L.phiA;
         r.a[PB] = L.phiB;
@@ -96,6 +103,25 @@ __device__ __forceinline__ LevelArrays level_arrays(const CoarseLevel& L, double
     return r;
 }
 
+#ifndef GPAW_COARSE_DIV_RN
+#define GPAW_COARSE_DIV_RN 1
+#endif
+
+// a / d rounded to nearest, from r = 1 / d rounded to nearest: the product
+// q = a r is within an ulp of a / d, its remainder a - q d is exact in one
+// fma, and q + (a - q d) r rounded once is the correctly rounded quotient
+// (Markstein's theorem, binary64 without overflow or underflow).  A zero
+// remainder keeps q, so exact quotients and signed zeros come out as a / d
+// does.  Three operations instead of the division sequence, on a pass that
+// runs on one compute unit.
+__device__ __forceinline__ double div_rn(double a, double d, double r)
+{
+    const double q = a * r;
+    const double e = __builtin_fma(-q, d, a);
+    return e == 0.0 ? q : __builtin_fma(e, r, q);
+}
+
+template <bool ZS>
 __device__ __forceinline__ void jacobi_sweeps(const CoarseLevel& L, const Pts& p,
                                               const LevelArrays& A, int k, int& cur)
 {
@@ -104,6 +130,7 @@ __device__ __forceinline__ void jacobi_sweeps(const CoarseLevel& L, const Pts& p
 #pragma unroll
     for (int m = 0; m < 7; m++) c[m] = L.c[m];
     const double w = L.w;
+    const double rc0 = 1.0 / c[0];
     for (int it = 0; it < k; it++) {
         const double* __restrict__ src = cur == PA ? A.a[PA] : A.a[PB];
         double* __restrict__ dst = cur == PA ? A.a[PB] : A.a[PA];
@@ -111,8 +138,11 @@ __device__ __forceinline__ void jacobi_sweeps(const CoarseLevel& L, const Pts& p
         for (int q = 0; q < 2; q++) {
             const int idx = p.idx[q];
             if (idx < 0) continue;
-            const double x = nb_sum(src, p.nb[q], c, 0.0);
-            dst[idx] = (1.0 - w) * src[idx] + w * (rho[idx] - x) / c[0];
+            const double x = nb_sum<ZS>(src, p.nb[q], c, 0.0);
+            if (GPAW_COARSE_DIV_RN)
+                dst[idx] = (1.0 - w) * src[idx] + div_rn(w * (rho[idx] - x), c[0], rc0);
+            else
+                dst[idx] = (1.0 - w) * src[idx] + w * (rho[idx] - x) / c[0];
         }
         __syncthreads();
         cur ^= 1;
@@ -149,6 +179,13 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
         const int* src = (const int*)P->nbtab;
         for (int i = threadIdx.x; i < P->nb_words; i += blockDim.x) dst[i] = src[i];
         tab = (const short*)dst;
+        // the zero slots (LDS starts undefined)
+        for (int l = threadIdx.x; l < P->nlev; l += blockDim.x) {
+            const CoarseLevel& L = P->lev[l];
+            const int n = L.g.n[0] * L.g.n[1] * L.g.n[2];
+            smem[L.off + n] = 0.0;
+            smem[L.off + (n + 1) + n] = 0.0;
+        }
         // made visible by the barrier after the first level's load below
     }
     Phases ph;
@@ -174,9 +211,9 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
         const CoarseLevel& L = P->lev[l];
         const LevelArrays A = level_arrays<LDS>(L, smem);
         Pts pts;
-        level_points(L, tab, pts);
+        level_points<LDS>(L, tab, pts);
         int cur = PA;
-        jacobi_sweeps(L, pts, A, L.pre, cur);
+        jacobi_sweeps<LDS>(L, pts, A, L.pre, cur);
         const double* __restrict__ phi = cur == PA ? A.a[PA] : A.a[PB];
         const double* __restrict__ rho = A.a[RHO];
         double* __restrict__ res = A.a[RES];
@@ -187,7 +224,7 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
             double c[7];
 #pragma unroll
             for (int m = 0; m < 7; m++) c[m] = L.c[m];
-            res[idx] = nb_sum(phi, pts.nb[q], c, 0.0 + phi[idx] * c[0]) - rho[idx];
+            res[idx] = nb_sum<LDS>(phi, pts.nb[q], c, 0.0 + phi[idx] * c[0]) - rho[idx];
         }
         __syncthreads();
         const CoarseLevel& N = P->lev[l + 1];
@@ -205,9 +242,9 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
         const CoarseLevel& L = P->lev[nl - 1];
         const LevelArrays A = level_arrays<LDS>(L, smem);
         Pts pts;
-        level_points(L, tab, pts);
+        level_points<LDS>(L, tab, pts);
         int cur = PA;
-        jacobi_sweeps(L, pts, A, L.post, cur);
+        jacobi_sweeps<LDS>(L, pts, A, L.post, cur);
     }
     // up: correct from the coarser level, post-smooth
     for (int l = nl - 2; l >= 0; l--) {
@@ -228,8 +265,8 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const CoarsePlan* __rest
         }
         __syncthreads();
         Pts pts;
-        level_points(L, tab, pts);
-        jacobi_sweeps(L, pts, A, L.post, cur);
+        level_points<LDS>(L, tab, pts);
+        jacobi_sweeps<LDS>(L, pts, A, L.post, cur);
     }
     if (LDS) {   // hand the first level's solution back where the caller expects it
         const CoarseLevel& G = P->lev[0];
@@ -306,7 +343,7 @@ __global__ __launch_bounds__(1024) void k_coarse_one(CoarseOne p)
         for (int k = 0; k < 2; k++) {
             const int idx = q.idx[k];
             if (idx < 0) continue;
-            const double x = nb_sum(src, q.nb[k], c, 0.0);
+            const double x = nb_sum<false>(src, q.nb[k], c, 0.0);
             dst[idx] = (1.0 - w) * src[idx] + w * (rho[idx] - x) / c[0];
         }
         __syncthreads();

@@ -501,11 +501,12 @@ static int setup_coarse(gpaw_poisson* ps)
         if (e != hipSuccess) return gpaw_check(e, "poisson: coarse neighbour tables");
         P.nbtab = ps->d_coarse_nb;
     }
-    // keep the coarse arrays (4 per level) and the tables in LDS when they fit
+    // keep the coarse arrays (4 per level, each with one more element: the
+    // zero slot of the phi arrays) and the tables in LDS when they fit
     long long nd = 0;
     for (int l = lc; l <= ps->L; l++) {
         P.lev[l - lc].off = (int)nd;
-        nd += 4 * ps->npts[l];
+        nd += 4 * (ps->npts[l] + 1);
     }
     if (nd * 8 + 4LL * P.nb_words <= 128 * 1024) {
         P.lds = 1;
