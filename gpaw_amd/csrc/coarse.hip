@@ -310,9 +310,14 @@ __global__ __launch_bounds__(1024) void k_coarse_one(CoarseOne p)
 {
     extern __shared__ double smem[];
     const int n = p.g.n[0] * p.g.n[1] * p.g.n[2];
+    // phi A / B with their zero slots (element n), then rho
     double* A = smem;
-    double* B = smem + n;
-    double* rho = smem + 2 * n;
+    double* B = smem + (n + 1);
+    double* rho = smem + 2 * (n + 1);
+    if (threadIdx.x == 0) {
+        A[n] = 0.0;
+        B[n] = 0.0;
+    }
     Pts q;
 #pragma unroll
     for (int k = 0; k < 2; k++) {
@@ -326,6 +331,9 @@ __global__ __launch_bounds__(1024) void k_coarse_one(CoarseOne p)
         q.nb[k][3] = one_nb(i0, i1 - 1, i2, p.g);
         q.nb[k][4] = one_nb(i0 + 1, i1, i2, p.g);
         q.nb[k][5] = one_nb(i0 - 1, i1, i2, p.g);
+#pragma unroll
+        for (int m = 0; m < 6; m++)
+            if (q.nb[k][m] < 0) q.nb[k][m] = n;
         if (idx < n) {
             rho[idx] = (double)((const TB*)p.rho)[idx];
             A[idx] = 0.0;
@@ -336,6 +344,7 @@ __global__ __launch_bounds__(1024) void k_coarse_one(CoarseOne p)
 #pragma unroll
     for (int m = 0; m < 7; m++) c[m] = p.c[m];
     const double w = p.w;
+    const double rc0 = 1.0 / c[0];
     double* src = A;
     double* dst = B;
     for (int it = 0; it < p.sweeps; it++) {
@@ -343,8 +352,11 @@ __global__ __launch_bounds__(1024) void k_coarse_one(CoarseOne p)
         for (int k = 0; k < 2; k++) {
             const int idx = q.idx[k];
             if (idx < 0) continue;
-            const double x = nb_sum<false>(src, q.nb[k], c, 0.0);
-            dst[idx] = (1.0 - w) * src[idx] + w * (rho[idx] - x) / c[0];
+            const double x = nb_sum<true>(src, q.nb[k], c, 0.0);
+            if (GPAW_COARSE_DIV_RN)
+                dst[idx] = (1.0 - w) * src[idx] + div_rn(w * (rho[idx] - x), c[0], rc0);
+            else
+                dst[idx] = (1.0 - w) * src[idx] + w * (rho[idx] - x) / c[0];
         }
         __syncthreads();
         double* t = src;
@@ -420,8 +432,8 @@ hipError_t launch_coarse_vcycle(const CoarsePlan* d_plan, const CoarsePlan& h_pl
     CoarseOne one;
     if (coarse_one_args(h_plan, one)) {
         const int n = one.g.n[0] * one.g.n[1] * one.g.n[2];
-        launch_kernel(k_coarse_one<double>, dim3(1), dim3(1024), 3 * n * (int)sizeof(double), st,
-                      one);
+        launch_kernel(k_coarse_one<double>, dim3(1), dim3(1024),
+                      3 * (n + 1) * (int)sizeof(double), st, one);
         return launch_error();
     }
     if (lds_bytes > 0)
@@ -437,8 +449,8 @@ hipError_t launch_coarse_vcycle_f32(const CoarsePlan* d_plan, const CoarsePlan& 
     CoarseOne one;
     if (coarse_one_args(h_plan, one)) {
         const int n = one.g.n[0] * one.g.n[1] * one.g.n[2];
-        launch_kernel(k_coarse_one<float>, dim3(1), dim3(1024), 3 * n * (int)sizeof(double), st,
-                      one);
+        launch_kernel(k_coarse_one<float>, dim3(1), dim3(1024),
+                      3 * (n + 1) * (int)sizeof(double), st, one);
         return launch_error();
     }
     if (lds_bytes <= 0) return hipErrorInvalidValue;   // float levels need LDS mode

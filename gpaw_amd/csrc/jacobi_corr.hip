@@ -239,7 +239,18 @@ __global__ __launch_bounds__(BOX_NT) void k_jacobi_corr(const V* __restrict__ a,
     }
 }
 
-constexpr int JC_Q = 4;
+// planes per block
+#ifndef GPAW_JC_Q_F32
+#define GPAW_JC_Q_F32 8
+#endif
+#ifndef GPAW_JC_Q_F64
+#define GPAW_JC_Q_F64 4
+#endif
+template <class V>
+constexpr int jc_q()
+{
+    return sizeof(V) == 4 ? GPAW_JC_Q_F32 : GPAW_JC_Q_F64;
+}
 
 template <class V, int R>
 hipError_t jc_launch(const Stencil& s, const void* in, void* out, const void* rho,
@@ -248,7 +259,7 @@ hipError_t jc_launch(const Stencil& s, const void* in, void* out, const void* rh
 {
     TileCoefs sc;
     for (int c = 0; c < 6 * R + 1; c++) sc.c[c] = s.coefs[c];
-    launch_kernel((k_jacobi_corr<V, R, JC_Q>), dim3(sh.nblocks), dim3(BOX_TZ, BOX_NTY), 0, st,
+    launch_kernel((k_jacobi_corr<V, R, jc_q<V>()>), dim3(sh.nblocks), dim3(BOX_TZ, BOX_NTY), 0, st,
                   (const V*)in, (V*)out, (const V*)rho, (const V*)corr, g, sc, w, sh, ct);
     return launch_error();
 }
@@ -265,7 +276,8 @@ bool jacobi_corr_ok(const Stencil& s, const Geom& g, const Transfer& itp)
         if (g.per[a] && (!itp.gin.per[a] || g.n[a] != 2 * itp.gin.n[a])) return false;
     }
     TileShape sh;
-    return box_geometry(g, s.star_r, 1, JC_Q, sh);
+    return box_geometry(g, s.star_r, 1, jc_q<float>(), sh) &&
+           box_geometry(g, s.star_r, 1, jc_q<double>(), sh);
 }
 
 hipError_t launch_jacobi_corr(const Stencil& s, int dtype, const void* in, void* out,
@@ -274,7 +286,7 @@ hipError_t launch_jacobi_corr(const Stencil& s, int dtype, const void* in, void*
 {
     if (!jacobi_corr_ok(s, g, itp)) return hipErrorInvalidValue;
     TileShape sh;
-    box_geometry(g, s.star_r, 1, JC_Q, sh);
+    box_geometry(g, s.star_r, 1, dtype == DT_F32 ? jc_q<float>() : jc_q<double>(), sh);
     const TrArgs ct = make_trargs(itp);
 #define GPAW_JC(T)                                                                   \
     switch (s.star_r) {                                                              \
