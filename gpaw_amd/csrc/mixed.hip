@@ -498,8 +498,9 @@ void mixed_schedule(const gpaw_poisson* ps, int& pre, int& post, std::vector<dou
 // 1/6 of the spectrum) save two of its four launches per V-cycle at the
 // same CG step count (159^3: 3.09 -> 2.80 ms; on smaller grids, whose level
 // 1 takes four sweeps per launch anyway, 4 + 4 stay: 2 + 2 added a step on
-// 96^3).  GPAW_AMD_MIXED_L1=pre,post,alpha overrides level 1 (alpha <= 1:
-// the plan's weight).
+// 96^3; 2 + 2 on levels 2 or 3 of 159^3 took 12 steps).
+// GPAW_AMD_MIXED_L<l>=pre,post,alpha overrides level l below the fused
+// coarse levels (alpha <= 1: the plan's weight).
 void mixed_level_schedule(const gpaw_poisson* ps, int l, int& pre, int& post,
                           std::vector<double>& wpre, std::vector<double>& wpost)
 {
@@ -507,21 +508,23 @@ void mixed_level_schedule(const gpaw_poisson* ps, int l, int& pre, int& post,
     post = ps->post[l];
     wpre.clear();
     wpost.clear();
-    if (l != 1 || ps->L < 2 || ps->lc == 1 || ps->ops[1]->st.star_r <= 0) return;
+    if (l < 1 || l >= ps->L || (ps->lc >= 0 && l >= ps->lc) || ps->ops[l]->st.star_r <= 0) return;
     int a = 0, b = 0;
     double alpha = 0.0;
-    if (jacobi_fused_ok(ps->ops[1]->st, ps->ops[1]->g) &&
+    if (l == 1 && jacobi_fused_ok(ps->ops[1]->st, ps->ops[1]->g) &&
         jacobi_fused_sweeps(ps->ops[1]->g) == 2) {
         a = b = 2;
         alpha = 6.0;
     }
-    if (const char* env = getenv("GPAW_AMD_MIXED_L1")) sscanf(env, "%d,%d,%lf", &a, &b, &alpha);
+    char name[32];
+    snprintf(name, sizeof name, "GPAW_AMD_MIXED_L%d", l);
+    if (const char* env = getenv(name)) sscanf(env, "%d,%d,%lf", &a, &b, &alpha);
     if (a <= 0 || b <= 0 || (a + b) % 2 || a > MIXED_MAX_SWEEPS || b > MIXED_MAX_SWEEPS) return;
     pre = a;
     post = b;
     if (alpha > 1.0) {
-        wpre = chebyshev_weights(ps->ops[1]->st, alpha, a);
-        wpost = chebyshev_weights(ps->ops[1]->st, alpha, b);
+        wpre = chebyshev_weights(ps->ops[l]->st, alpha, a);
+        wpost = chebyshev_weights(ps->ops[l]->st, alpha, b);
         std::reverse(wpost.begin(), wpost.end());
     }
 }
