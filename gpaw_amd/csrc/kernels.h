@@ -126,6 +126,12 @@ int jacobi_fused_sweeps(const Geom& g);
 // LDS box edge of a fused launch of `sweeps` sweeps on g (16, or 12 on grids
 // with too few 16^3 boxes; GPAW_AMD_JF_BOX=16 or 12 forces one)
 int jacobi_fused_edge(const Geom& g, int sweeps);
+// two Jacobi sweeps from phi = 0 (weights w0 then w1) in one box launch, the
+// bits of k_jacobi_from_zero followed by one sweep; real FP64 / FP32, star
+// range 1..3 (hipErrorInvalidValue otherwise: use the two launches)
+bool jacobi_two_from_zero_ok(const Stencil& s, int dtype, const Geom& g);
+hipError_t launch_jacobi_two_from_zero(const Stencil& s, int dtype, const void* rho, void* out,
+                                       const Geom& g, double w0, double w1, hipStream_t st);
 hipError_t launch_jacobi_fused(const Stencil& s, int dtype, const void* in, void* out,
                                const void* rho, const Geom& g, double w, hipStream_t st,
                                const void* corr = nullptr, const Transfer* ctr = nullptr,

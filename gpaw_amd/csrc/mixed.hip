@@ -309,7 +309,15 @@ hipError_t emit_v(gpaw_poisson* ps, MixedState* m, VLevels& V, int l, bool zero0
         wpre = m->wpre[l].data();
         wpost = m->wpost[l].data();
     }
-    if (l == 0 && zero0 && pre > 0 && ps->fuse_jacobi) {
+    if (l == 0 && zero0 && pre >= 2 && ps->fuse_jacobi &&
+        jacobi_two_from_zero_ok(op->st, V.dt, op->g)) {
+        // the first two sweeps from zero in one launch, into cur (where two
+        // single sweeps leave them)
+        e = launch_jacobi_two_from_zero(op->st, V.dt, V.rho[0], cur, op->g,
+                                        wpre ? wpre[0] : ps->w[0], wpre ? wpre[1] : ps->w[0], st);
+        pre -= 2;
+        if (wpre) wpre += 2;
+    } else if (l == 0 && zero0 && pre > 0 && ps->fuse_jacobi) {
         // the first sweep from zero, element-wise, into alt (as a sweep would)
         if (V.dt == DT_F32)
             launch_kernel(k_jacobi_from_zero<float>, gridn(ps->npts[0]), dim3(256), 0, st,

@@ -466,6 +466,23 @@ hipError_t box_launch(const Stencil& s, int dtype, int mode, const void* in, voi
     return hipErrorInvalidValue;
 }
 
+bool jacobi_two_from_zero_ok(const Stencil& s, int dtype, const Geom& g)
+{
+    TileShape sh;
+    return s.star_r >= 1 && s.star_r <= 3 && (dtype == DT_F64 || dtype == DT_F32) &&
+           box_geometry(g, s.star_r, 1, box_default_q(g, s.star_r, 1), sh);
+}
+
+hipError_t launch_jacobi_two_from_zero(const Stencil& s, int dtype, const void* rho, void* out,
+                                       const Geom& g, double w0, double w1, hipStream_t st)
+{
+    if (!jacobi_two_from_zero_ok(s, dtype, g)) return hipErrorInvalidValue;
+    TileShape sh;
+    box_geometry(g, s.star_r, 1, box_default_q(g, s.star_r, 1), sh);
+    return dtype == DT_F64 ? box_fz_f64(s, rho, out, g, w0, w1, sh, st)
+                           : box_fz_f32(s, rho, out, g, w0, w1, sh, st);
+}
+
 int tile_tz(int)
 {
     // 32 columns (8 waves of rows per block): measured faster than 64 on

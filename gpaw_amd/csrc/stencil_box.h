@@ -42,11 +42,17 @@ struct BoxGeom {
     static constexpr int LDS = Q * LH * LW;               // Q plane buffers
 };
 
-template <class V, int R, int Q, int MODE, bool NORM, bool WRITE>
+// FZ (ST_JACOBI, real data): `in` is rho and every loaded value v first
+// becomes the sweep from phi = 0 with weight w0, the element-wise
+// st_epilogue(0, 0, v) of k_jacobi_from_zero; the kernel then runs the
+// second sweep on those values, so two sweeps from zero take one launch and
+// give the bits of the two launches.
+template <class V, int R, int Q, int MODE, bool NORM, bool WRITE, bool FZ = false>
 __global__ __launch_bounds__(BOX_NT) void k_star_box(const V* __restrict__ in, V* __restrict__ out,
                                                      const V* __restrict__ aux, Geom g, Phases ph,
                                                      TileCoefs sc, double w, TileShape sh,
-                                                     i64 ng, double* __restrict__ partial)
+                                                     i64 ng, double* __restrict__ partial,
+                                                     double w0)
 {
     typedef typename VT<V>::R RT;
     typedef BoxGeom<V, R, Q> BG;
@@ -128,15 +134,18 @@ __global__ __launch_bounds__(BOX_NT) void k_star_box(const V* __restrict__ in, V
         for (int k = 0; k < NH; k++) vtouch(hq[p][k]);
         if (MODE != ST_APPLY) vtouch(sn[p]);
     }
+    auto fz = [&](V v) -> V {
+        return FZ ? st_epilogue<ST_JACOBI>(vzero<V>(), vzero<V>(), v, (RT)w0, (RT)sc.c[0]) : v;
+    };
 #pragma unroll
-    for (int p = 0; p < Q + 2 * R; p++) q[p] = ghost_fix(q[p], qf[p], qr, ph);
+    for (int p = 0; p < Q + 2 * R; p++) q[p] = fz(ghost_fix(q[p], qf[p], qr, ph));
 #pragma unroll
     for (int p = 0; p < Q; p++) {
         V* B = lds + p * (LH * LW);
         B[(R + tyi) * LW + R + tx] = q[R + p];
 #pragma unroll
         for (int k = 0; k < NH; k++)
-            if (hoff[k] >= 0) B[hoff[k]] = ghost_fix(hq[p][k], qf[R + p], hr[k], ph);
+            if (hoff[k] >= 0) B[hoff[k]] = fz(ghost_fix(hq[p][k], qf[R + p], hr[k], ph));
     }
     __syncthreads();
 
@@ -207,5 +216,9 @@ int box_default_q(const Geom& g, int R, int nin);
 hipError_t box_launch(const Stencil& s, int dtype, int mode, const void* in, void* out,
                       const void* aux, const Geom& g, const Phases& ph, double w,
                       const TileShape& sh, double* partial, bool write, hipStream_t st);
+hipError_t box_fz_f64(const Stencil& s, const void* rho, void* out, const Geom& g, double w0,
+                      double w1, const TileShape& sh, hipStream_t st);
+hipError_t box_fz_f32(const Stencil& s, const void* rho, void* out, const Geom& g, double w0,
+                      double w1, const TileShape& sh, hipStream_t st);
 
 }  // namespace gpaw

@@ -10,4 +10,10 @@ hipError_t box_launch_f64(const Stencil& s, int mode, const void* in, void* out,
     return box_dispatch<double>(s, mode, in, out, aux, g, ph, w, sh, partial, write, st);
 }
 
+hipError_t box_fz_f64(const Stencil& s, const void* rho, void* out, const Geom& g, double w0,
+                      double w1, const TileShape& sh, hipStream_t st)
+{
+    return box_fz<double>(s, rho, out, g, w0, w1, sh, st);
+}
+
 }  // namespace gpaw

@@ -15,8 +15,48 @@ hipError_t box_go(const Stencil& s, const void* in, void* out, const void* aux, 
     const i64 ng = (i64)g.n[0] * g.n[1] * g.n[2];
     launch_kernel((k_star_box<V, R, Q, MODE, NORM, WRITE>), dim3(sh.nblocks),
                   dim3(BOX_TZ, BOX_NTY), 0, st, (const V*)in, (V*)out, (const V*)aux, g, ph, sc, w,
-                  sh, ng, partial);
+                  sh, ng, partial, 0.0);
     return launch_error();
+}
+
+// two Jacobi sweeps from phi = 0 (weights w0, w1) in one launch (FZ)
+template <class V, int R, int Q>
+hipError_t box_fz_go(const Stencil& s, const void* rho, void* out, const Geom& g, double w0,
+                     double w1, const TileShape& sh, hipStream_t st)
+{
+    TileCoefs sc;
+    for (int c = 0; c < 6 * R + 1; c++) sc.c[c] = s.coefs[c];
+    const i64 ng = (i64)g.n[0] * g.n[1] * g.n[2];
+    Phases ph;   // real data: no phases
+    for (int a = 0; a < 3; a++)
+        for (int e = 0; e < 2; e++) {
+            ph.re[a][e] = 1.0;
+            ph.im[a][e] = 0.0;
+        }
+    launch_kernel((k_star_box<V, R, Q, ST_JACOBI, false, true, true>), dim3(sh.nblocks),
+                  dim3(BOX_TZ, BOX_NTY), 0, st, (const V*)rho, (V*)out, (const V*)rho, g, ph, sc,
+                  w1, sh, ng, (double*)nullptr, w0);
+    return launch_error();
+}
+
+template <class V>
+hipError_t box_fz(const Stencil& s, const void* rho, void* out, const Geom& g, double w0,
+                  double w1, const TileShape& sh, hipStream_t st)
+{
+#define GPAW_FZ_Q(R)                                                              \
+    switch (sh.q) {                                                               \
+    case 2: return box_fz_go<V, R, 2>(s, rho, out, g, w0, w1, sh, st);            \
+    case 4: return box_fz_go<V, R, 4>(s, rho, out, g, w0, w1, sh, st);            \
+    case 8: return box_fz_go<V, R, 8>(s, rho, out, g, w0, w1, sh, st);            \
+    }                                                                             \
+    return hipErrorInvalidValue;
+    switch (s.star_r) {
+    case 1: GPAW_FZ_Q(1)
+    case 2: GPAW_FZ_Q(2)
+    case 3: GPAW_FZ_Q(3)
+    }
+#undef GPAW_FZ_Q
+    return hipErrorInvalidValue;
 }
 
 template <class V, int R, int Q, int MODE>
